@@ -1,0 +1,294 @@
+#!/usr/bin/env python3
+"""Frame-scoring benchmark on one MI355X.
+
+Workload (the plain-FP32 video configuration): frames of 1024x1024 pixels x
+113 bands, BSQ fp32, drawn by the seeded generator (Dirichlet mix of 8
+endmembers, per-frame drift of the mix, 8-target library planted at 0.5%
+each, gain U(0.3,1.2), noise N(0,0.005)). One step = one frame through the
+whole per-frame path, with the frame already resident in HBM:
+    moments (FP64, matrix cores) -> statistics build (FP64, one workgroup)
+    -> SAM + CEM from one cube read -> threshold+NMS+compact for each.
+`value` is frames x pixels / wall time of the timed steps (Mpix/s).
+
+Per-kernel rates come from HIP events on the launch stream, averaged over
+repeats, and the roofline entry is for the kernel that takes the largest
+share of a step. The CPU baseline times the reference's own CPU scorer
+(oracle/_ref, built from the reference sources) - or the oracle restatement
+when that is absent - on a bounded sample of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "hsi-detectors-cuda_amd")
+for _p in (REPO, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP16_MFMA_PEAK_TFLOPS = 2500.0
+
+H_, W_, B_, T_ = 1024, 1024, 113, 8
+LOADING = 1e-6
+SAM_THRESHOLD = 0.10
+CEM_THRESHOLD = 0.5
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--frames", type=int, default=8, help="distinct resident frames cycled")
+    ap.add_argument("--repeats", type=int, default=20, help="per-kernel timing repeats")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-rows", type=int, default=64, help="rows of a frame in the CPU sample")
+    return ap.parse_args()
+
+
+def time_on_stream(torch, fn, repeats):
+    """Average ms of fn() over `repeats`, bracketed by events on the current stream."""
+    fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(repeats):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / repeats
+
+
+def cpu_baseline(host_frame, library, rows):
+    """Reference CPU scorer (or its restatement) on `rows` rows of a frame."""
+    import ctypes as C
+    sample_px = rows * W_
+    sub = np.ascontiguousarray(host_frame[:, :sample_px])
+    ref_path = os.path.join(REPO, "oracle", "_ref", "libhsiref.so")
+    if os.path.exists(ref_path):
+        kind = "reference"
+        ref = C.CDLL(ref_path)
+        f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        i32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        ref.ref_library_norms.argtypes = [f32, C.c_int, C.c_int, f32]
+        ref.ref_sam_best.argtypes = [f32, C.c_int, C.c_int, C.c_int, f32, f32, C.c_int, f32, i32]
+        ref.ref_build_statistics.restype = C.c_int
+        ref.ref_build_statistics.argtypes = [f32, C.c_int, C.c_int, C.c_int, f32, C.c_int, C.c_int,
+                                             C.c_double, f32, f32, f32, f32, f32, C.c_char_p, C.c_int]
+        ref.ref_cem_best.argtypes = [f32, C.c_int, C.c_int, C.c_int, f32, f32, f32, f32, f32,
+                                     C.c_int, f32, i32]
+        norms = np.empty(T_, np.float32)
+        ref.ref_library_norms(library, T_, B_, norms)
+        mean = np.empty(B_, np.float32)
+        white = np.empty(B_ * B_, np.float32)
+        tw = np.empty(T_ * B_, np.float32)
+        tn = np.empty(T_, np.float32)
+        cw = np.empty(T_ * B_, np.float32)
+        err = C.create_string_buffer(256)
+        score = np.empty(sample_px, np.float32)
+        target = np.empty(sample_px, np.int32)
+        t0 = time.perf_counter()
+        ok = ref.ref_build_statistics(sub, rows, W_, B_, library, T_, B_, LOADING, mean, white, tw,
+                                      tn, cw, err, 256)
+        t1 = time.perf_counter()
+        ref.ref_sam_best(sub, rows, W_, B_, library, norms, T_, score, target)
+        t2 = time.perf_counter()
+        ref.ref_cem_best(sub, rows, W_, B_, mean, white, tw, tn, cw, T_, score, target)
+        t3 = time.perf_counter()
+        if not ok:
+            raise RuntimeError(err.value.decode())
+    else:
+        kind = "port"
+        import oracle
+        t0 = time.perf_counter()
+        count, s, so = oracle.moments(sub)
+        st = oracle.build_statistics(count, s, so, library, LOADING)
+        t1 = time.perf_counter()
+        oracle.sam_best(sub, library)
+        t2 = time.perf_counter()
+        oracle.cem_best(sub, st)
+        t3 = time.perf_counter()
+    total = t3 - t0
+    return {
+        "value": round(sample_px / total / 1e6, 4),
+        "unit": "Mpix/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"rows 0-{rows - 1} of frame 0 ({rows}x{W_}x{B_}, {T_} targets): moments+"
+                  f"build_statistics, sam_best_cpu, cem_best_cpu, single thread",
+        "seconds": round(total, 3),
+        "per_detector_mpix_s": {"stats": round(sample_px / (t1 - t0) / 1e6, 4),
+                                "sam": round(sample_px / (t2 - t1) / 1e6, 4),
+                                "cem": round(sample_px / (t3 - t2) / 1e6, 4)},
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box in this setup has a single MI355X, so "
+              f"--gpus {args.gpus} is not supported (the path would shard whole frames across "
+              f"GPUs with no exchange).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import hsi_mi355x as H
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    pixels = H_ * W_
+    stride = H.plane_stride(H_, W_)
+    cfg = H.synth_config(H_, W_, B_, T_, drift=0.2, drift_period=300)
+    library = H.synth_library(cfg)
+    log(f"generating {args.frames} frames {H_}x{W_}x{B_} on the host")
+    frames = []
+    host0 = None
+    for f in range(args.frames):
+        host = H.synth_frame(cfg, f, threads=16)
+        if f == 0:
+            host0 = host
+        frames.append(torch.from_numpy(host).cuda())
+    torch.cuda.synchronize()
+    log("frames resident in HBM")
+
+    ctx = H.Context(0)
+    ctx.set_library(library)
+    ctx.reserve(pixels, B_)
+    f32 = dict(dtype=torch.float32, device="cuda")
+    i32 = dict(dtype=torch.int32, device="cuda")
+    sam_s, sam_t = torch.empty(pixels, **f32), torch.empty(pixels, **i32)
+    cem_s, cem_t = torch.empty(pixels, **f32), torch.empty(pixels, **i32)
+    cap = 4096
+    sam_d, cem_d = torch.empty((cap, 4), **i32), torch.empty((cap, 4), **i32)
+    sam_n, cem_n = torch.empty(1, **i32), torch.empty(1, **i32)
+    p_sam = H.DetectionParams(SAM_THRESHOLD, 2, cap, False)
+    p_cem = H.DetectionParams(CEM_THRESHOLD, 2, cap, True)
+
+    def stats(cube):
+        ctx.stats_accumulate(cube, H_, W_, B_)
+
+    def build():
+        ctx.stats_build(LOADING)
+
+    def step(i):
+        cube = frames[i % len(frames)]
+        ctx.stats_accumulate(cube, H_, W_, B_)
+        ctx.stats_build(LOADING)
+        ctx.score_sam_cem(cube, H_, W_, B_, sam_s, sam_t, cem_s, cem_t)
+        ctx.detect(sam_s, sam_t, H_, W_, p_sam, sam_d, sam_n)
+        ctx.detect(cem_s, cem_t, H_, W_, p_cem, cem_d, cem_n)
+
+    step(0)
+    ctx.stats_check()
+    torch.cuda.synchronize()
+    log("warm-up")
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.stats_check()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = pixels * args.steps / (t1 - t0) / 1e6
+    log(f"step {ms_step:.3f} ms, {value:.1f} Mpix/s")
+
+    # Per-kernel timing on the launch stream.
+    c0 = frames[0]
+    r = args.repeats
+    ms = {
+        "moments": time_on_stream(torch, lambda: stats(c0), r),
+        "build": time_on_stream(torch, build, r),
+        "sam_cem_fused": time_on_stream(torch, lambda: ctx.score_sam_cem(c0, H_, W_, B_, sam_s, sam_t, cem_s, cem_t), r),
+        "sam": time_on_stream(torch, lambda: ctx.score(H.Detector.SAM, c0, H_, W_, B_, sam_s, sam_t), r),
+        "cem": time_on_stream(torch, lambda: ctx.score(H.Detector.CEM, c0, H_, W_, B_, cem_s, cem_t), r),
+        "detect": time_on_stream(torch, lambda: ctx.detect(sam_s, sam_t, H_, W_, p_sam, sam_d, sam_n), r),
+        "rx": time_on_stream(torch, lambda: ctx.score(H.Detector.RX, c0, H_, W_, B_, cem_s), r),
+        "ace": time_on_stream(torch, lambda: ctx.score(H.Detector.ACE, c0, H_, W_, B_, cem_s, cem_t, sam_s), r),
+    }
+    for k, v in ms.items():
+        log(f"  {k:14s} {v * 1e3:9.1f} us")
+
+    cube_bytes = B_ * 4
+    # Algorithmic counts per pixel (bytes read + written; FP64 flops).
+    bytes_px = {"sam": cube_bytes + 8, "cem": cube_bytes + 8, "sam_cem_fused": cube_bytes + 16,
+                "moments": cube_bytes, "rx": cube_bytes + 4, "ace": cube_bytes + 12, "detect": 8}
+    flops_px = {"sam": 2 * B_ * (T_ + 1), "cem": 2 * B_ * T_, "sam_cem_fused": 2 * B_ * (2 * T_ + 1),
+                "moments": B_ * (B_ + 1), "rx": B_ * B_ + 4 * B_,
+                "ace": B_ * B_ + 4 * B_ + 2 * B_ * T_}
+    detectors = {}
+    for k in ("sam", "cem", "sam_cem_fused", "rx", "ace", "moments"):
+        sec = ms[k] / 1e3
+        gbs = bytes_px[k] * pixels / sec / 1e9
+        tfl = flops_px[k] * pixels / sec / 1e12
+        detectors[k] = {"mpix_s": round(pixels / sec / 1e6, 1), "us": round(ms[k] * 1e3, 1),
+                        "hbm_gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "fp64_tflops": round(tfl, 2), "fp64_frac": round(tfl / FP64_PEAK_TFLOPS, 4)}
+
+    # Dominant kernel of the step: moments (FP64 matrix cores) vs the fused
+    # SAM+CEM stream (HBM).
+    step_kernels = {"moments": ms["moments"], "sam_cem_fused": ms["sam_cem_fused"],
+                    "build": ms["build"], "detect": 2 * ms["detect"]}
+    dom = max(step_kernels, key=step_kernels.get)
+    if dom == "moments":
+        roof = {"kernel": "moments_f64_kernel", "bound": "mfma",
+                "achieved": round(flops_px["moments"] * pixels / (ms["moments"] / 1e3) / 1e12, 3),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "per_unit": f"{flops_px['moments']} FP64 flop/pixel (B(B+1)), {pixels} pixels/launch"}
+    elif dom == "sam_cem_fused":
+        roof = {"kernel": "sam_cem_kernel", "bound": "hbm",
+                "achieved": round(bytes_px["sam_cem_fused"] * pixels / (ms["sam_cem_fused"] / 1e3) / 1e9, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "per_unit": f"{bytes_px['sam_cem_fused']} B/pixel, {pixels} pixels/launch"}
+    else:
+        roof = {"kernel": dom, "bound": "latency", "achieved": None, "peak": None, "unit": None}
+    if roof.get("achieved") is not None:
+        roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    roof["traffic"] = None
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline")
+        cpu = cpu_baseline(host0, library, args.cpu_rows)
+        log(f"  CPU {cpu['value']} Mpix/s ({cpu['kind']}, {cpu['seconds']} s)")
+
+    line = {
+        "metric": "Mpix/s per detector vs HBM/MFMA roofline; mixed+learned vs plain fp32",
+        "value": round(value, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 in, f64 accumulate",
+        "data": "synthetic (seeded generator, frames resident in HBM)",
+        "config": {"workload": "plain FP32 video frame: moments+build+SAM+CEM+detect",
+                   "height": H_, "width": W_, "bands": B_, "targets": T_,
+                   "resident_frames": args.frames, "loading": LOADING},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "detectors": detectors,
+        "stage_us": {k: round(v * 1e3, 1) for k, v in ms.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

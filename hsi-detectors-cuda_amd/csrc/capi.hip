@@ -1,0 +1,542 @@
+// capi.hip - the C ABI (include/hsi_mi355x.h): context, validation, dispatch.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace hsi_amd;
+
+struct hsi_ctx {
+  int device = 0;
+  std::string error;
+
+  // Target library.
+  int num_targets = 0;
+  int lib_bands = 0;
+  int lib_stride = 1;
+  float* d_targets = nullptr;  // [kMaxTargets][kMaxBands] capacity, [T][B] used
+  float* d_norms = nullptr;    // [kMaxTargets]
+  double* d_sam_lib = nullptr; // [kMaxBands][kMaxTargets] capacity
+
+  // Moments and statistics, allocated once at the largest supported size.
+  int moment_bands = 0;        // bands of the last accumulation, 0 if none
+  int stats_bands = 0;         // bands of the current statistics, 0 if none
+  int stats_targets = 0;       // targets the statistics were built with
+  double last_loading = 0.0;
+  double* d_count = nullptr;
+  double* d_sum = nullptr;
+  double* d_sum_outer = nullptr;
+  float* d_mean = nullptr;
+  float* d_whitener = nullptr;
+  float* d_target_white = nullptr;
+  float* d_target_norm2 = nullptr;
+  float* d_cem_weight = nullptr;
+  unsigned* d_status = nullptr;
+  double* d_build_work = nullptr;
+  double* d_cem_lib = nullptr;
+  float* d_white_blocks = nullptr;
+  float* d_ace_u = nullptr;
+
+  // Scratch sized per frame.
+  double* d_mom_scratch = nullptr;
+  std::size_t mom_scratch_bytes = 0;
+  unsigned* d_det_scratch = nullptr;
+  std::size_t det_scratch_bytes = 0;
+};
+
+namespace {
+
+hsi_status fail(hsi_ctx* ctx, hsi_status status, const std::string& message) {
+  if (ctx) ctx->error = message;
+  return status;
+}
+
+hsi_status device_fail(hsi_ctx* ctx, const char* what, hipError_t err) {
+  return fail(ctx, HSI_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(err));
+}
+
+#define HSI_HIP(ctx, expr, what)                          \
+  do {                                                    \
+    const hipError_t hsi_err_ = (expr);                   \
+    if (hsi_err_ != hipSuccess) return device_fail(ctx, what, hsi_err_); \
+  } while (0)
+
+template <typename T>
+hipError_t alloc(T** ptr, std::size_t count) {
+  return hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 256);
+}
+
+hipError_t grow(void** ptr, std::size_t* have, std::size_t need) {
+  if (*have >= need && *ptr) return hipSuccess;
+  if (*ptr) {
+    hipError_t err = hipFree(*ptr);
+    if (err != hipSuccess) return err;
+    *ptr = nullptr;
+    *have = 0;
+  }
+  hipError_t err = hipMalloc(ptr, need);
+  if (err == hipSuccess) *have = need;
+  return err;
+}
+
+hsi_status check_cube(hsi_ctx* ctx, const void* d_cube, int height, int width, int bands,
+                      std::size_t stride, hsi_dtype dtype) {
+  if (!d_cube) return fail(ctx, HSI_ERR_INVALID_ARG, "null cube");
+  if (height <= 0 || width <= 0 || bands <= 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "cube shape must be positive");
+  if (bands > kMaxBands)
+    return fail(ctx, HSI_ERR_CAPACITY, "cube has " + std::to_string(bands) +
+                                           " bands; at most " + std::to_string(kMaxBands) + " are supported");
+  const std::size_t pixels = static_cast<std::size_t>(height) * width;
+  if (stride < pixels || stride % 8 != 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "plane stride must be at least height*width and a multiple of 8 (use hsi_plane_stride)");
+  if (dtype != HSI_F32 && dtype != HSI_F16) return fail(ctx, HSI_ERR_INVALID_ARG, "unknown cube dtype");
+  if (reinterpret_cast<std::uintptr_t>(d_cube) % 16 != 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "cube must be 16-byte aligned");
+  return HSI_OK;
+}
+
+hsi_status set_device(hsi_ctx* ctx) {
+  HSI_HIP(ctx, hipSetDevice(ctx->device), "select device");
+  return HSI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hsi_status_name(hsi_status status) {
+  switch (status) {
+    case HSI_OK: return "ok";
+    case HSI_ERR_INVALID_ARG: return "invalid argument";
+    case HSI_ERR_CAPACITY: return "capacity exceeded";
+    case HSI_ERR_NOT_POSITIVE_DEFINITE: return "not positive definite";
+    case HSI_ERR_DEVICE: return "device error";
+    case HSI_ERR_NO_STATISTICS: return "no statistics";
+    case HSI_ERR_NO_LIBRARY: return "no library";
+  }
+  return "unknown status";
+}
+
+const char* hsi_ctx_last_error(const hsi_ctx* ctx) { return ctx ? ctx->error.c_str() : ""; }
+
+size_t hsi_plane_stride(int height, int width) {
+  if (height <= 0 || width <= 0) return 0;
+  return round_up(static_cast<std::size_t>(height) * static_cast<std::size_t>(width), 8);
+}
+
+int hsi_max_targets(void) { return kMaxTargets; }
+int hsi_max_bands(void) { return kMaxBands; }
+
+void hsi_ctx_destroy(hsi_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  void* ptrs[] = {ctx->d_targets, ctx->d_norms, ctx->d_sam_lib, ctx->d_count, ctx->d_sum,
+                  ctx->d_sum_outer, ctx->d_mean, ctx->d_whitener, ctx->d_target_white,
+                  ctx->d_target_norm2, ctx->d_cem_weight, ctx->d_status, ctx->d_build_work,
+                  ctx->d_cem_lib, ctx->d_white_blocks, ctx->d_ace_u, ctx->d_mom_scratch,
+                  ctx->d_det_scratch};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete ctx;
+}
+
+hsi_status hsi_ctx_create(int device, hsi_ctx** out) {
+  if (!out) return HSI_ERR_INVALID_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return HSI_ERR_DEVICE;
+  hsi_ctx* ctx = new hsi_ctx();
+  ctx->device = device;
+  const std::size_t B = kMaxBands, T = kMaxTargets;
+  hipError_t err = hipSetDevice(device);
+  if (err == hipSuccess) err = alloc(&ctx->d_targets, T * B);
+  if (err == hipSuccess) err = alloc(&ctx->d_norms, T);
+  if (err == hipSuccess) err = alloc(&ctx->d_sam_lib, B * T);
+  if (err == hipSuccess) err = alloc(&ctx->d_count, 1);
+  if (err == hipSuccess) err = alloc(&ctx->d_sum, B);
+  if (err == hipSuccess) err = alloc(&ctx->d_sum_outer, B * B);
+  if (err == hipSuccess) err = alloc(&ctx->d_mean, B);
+  if (err == hipSuccess) err = alloc(&ctx->d_whitener, B * B);
+  if (err == hipSuccess) err = alloc(&ctx->d_target_white, T * B);
+  if (err == hipSuccess) err = alloc(&ctx->d_target_norm2, T);
+  if (err == hipSuccess) err = alloc(&ctx->d_cem_weight, T * B);
+  if (err == hipSuccess) err = alloc(&ctx->d_status, 1);
+  if (err == hipSuccess) err = alloc(&ctx->d_build_work, build_work_doubles(kMaxBands, kMaxTargets));
+  if (err == hipSuccess) err = alloc(&ctx->d_cem_lib, B * T);
+  if (err == hipSuccess) err = alloc(&ctx->d_white_blocks, rxace_white_floats(kMaxBands));
+  if (err == hipSuccess) err = alloc(&ctx->d_ace_u, rxace_target_floats(kMaxBands, kMaxTargets));
+  if (err == hipSuccess) err = hipMemset(ctx->d_sum_outer, 0, B * B * sizeof(double));
+  if (err == hipSuccess) err = hipMemset(ctx->d_status, 0, sizeof(unsigned));
+  if (err == hipSuccess) err = hipDeviceSynchronize();
+  if (err != hipSuccess) {
+    hsi_ctx_destroy(ctx);
+    return HSI_ERR_DEVICE;
+  }
+  *out = ctx;
+  return HSI_OK;
+}
+
+hsi_status hsi_ctx_reserve(hsi_ctx* ctx, size_t max_pixels, int max_bands) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (max_pixels == 0 || max_bands <= 0 || max_bands > kMaxBands)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "reserve: bad size");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  std::size_t mom = 0;
+  for (int b = 1; b <= max_bands; ++b) {
+    const std::size_t need = moments_scratch_doubles(max_pixels, b) * sizeof(double);
+    if (need > mom) mom = need;
+  }
+  HSI_HIP(ctx, grow(reinterpret_cast<void**>(&ctx->d_mom_scratch), &ctx->mom_scratch_bytes, mom),
+          "reserve moment scratch");
+  HSI_HIP(ctx, grow(reinterpret_cast<void**>(&ctx->d_det_scratch), &ctx->det_scratch_bytes,
+                    detect_scratch_words(max_pixels) * sizeof(unsigned)),
+          "reserve detect scratch");
+  return HSI_OK;
+}
+
+hsi_status hsi_set_library(hsi_ctx* ctx, const float* values, const float* norms, int num_targets,
+                           int bands) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (!values || num_targets <= 0 || bands <= 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "library needs at least one target and one band");
+  if (num_targets > kMaxTargets || bands > kMaxBands)
+    return fail(ctx, HSI_ERR_CAPACITY,
+                "library of " + std::to_string(num_targets) + " targets x " + std::to_string(bands) +
+                    " bands exceeds the supported " + std::to_string(kMaxTargets) + " x " +
+                    std::to_string(kMaxBands));
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  // Norms as SpectralLibrary::norms forms them: double sum, float root.
+  std::vector<float> n(static_cast<std::size_t>(num_targets));
+  for (int t = 0; t < num_targets; ++t) {
+    if (norms) {
+      n[t] = norms[t];
+    } else {
+      double acc = 0.0;
+      for (int b = 0; b < bands; ++b) {
+        const double v = values[static_cast<std::size_t>(t) * bands + b];
+        acc += v * v;
+      }
+      n[t] = static_cast<float>(std::sqrt(acc));
+    }
+  }
+  const int ls = sam_cem_lib_stride(num_targets);
+  std::vector<double> lib(static_cast<std::size_t>(bands) * ls, 0.0);
+  for (int t = 0; t < num_targets; ++t)
+    for (int b = 0; b < bands; ++b)
+      lib[static_cast<std::size_t>(b) * ls + t] = values[static_cast<std::size_t>(t) * bands + b];
+  HSI_HIP(ctx, hipMemcpy(ctx->d_targets, values, sizeof(float) * num_targets * bands,
+                         hipMemcpyHostToDevice), "upload library");
+  HSI_HIP(ctx, hipMemcpy(ctx->d_norms, n.data(), sizeof(float) * num_targets, hipMemcpyHostToDevice),
+          "upload norms");
+  HSI_HIP(ctx, hipMemcpy(ctx->d_sam_lib, lib.data(), sizeof(double) * lib.size(), hipMemcpyHostToDevice),
+          "upload library layout");
+  ctx->num_targets = num_targets;
+  ctx->lib_bands = bands;
+  ctx->lib_stride = ls;
+  ctx->stats_bands = 0;  // statistics embed the library; rebuild them
+  return HSI_OK;
+}
+
+hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const float* d_cube, int height, int width, int bands,
+                                size_t plane_stride, hsi_precision precision, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, HSI_F32);
+  if (st != HSI_OK) return st;
+  if (precision != HSI_PLAIN && precision != HSI_MIXED)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "unknown precision");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  const std::size_t pixels = static_cast<std::size_t>(height) * width;
+  HSI_HIP(ctx, grow(reinterpret_cast<void**>(&ctx->d_mom_scratch), &ctx->mom_scratch_bytes,
+                    moments_scratch_doubles(pixels, bands) * sizeof(double)),
+          "allocate moment scratch");
+  HSI_HIP(ctx, launch_moments(d_cube, pixels, bands, plane_stride, precision, ctx->d_mom_scratch,
+                              ctx->d_count, ctx->d_sum, ctx->d_sum_outer,
+                              static_cast<hipStream_t>(stream)),
+          "accumulate moments");
+  ctx->moment_bands = bands;
+  return HSI_OK;
+}
+
+static hsi_status refresh_layouts(hsi_ctx* ctx, int bands, int targets, hipStream_t stream) {
+  LayoutArgs la;
+  la.bands = bands;
+  la.num_targets = targets;
+  la.lib_stride = sam_cem_lib_stride(targets > 0 ? targets : 1);
+  la.whitener = ctx->d_whitener;
+  la.target_white = ctx->d_target_white;
+  la.cem_weight = ctx->d_cem_weight;
+  la.mean = ctx->d_mean;
+  la.cem_lib = targets > 0 ? ctx->d_cem_lib : nullptr;
+  la.white_blocks = ctx->d_white_blocks;
+  la.ace_u = ctx->d_ace_u;
+  HSI_HIP(ctx, launch_layouts(la, stream), "derive statistics layouts");
+  return HSI_OK;
+}
+
+hsi_status hsi_stats_build(hsi_ctx* ctx, double diagonal_loading, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (ctx->moment_bands <= 0) return fail(ctx, HSI_ERR_NO_STATISTICS, "no scene statistics accumulated yet");
+  const int bands = ctx->moment_bands;
+  const int targets = ctx->num_targets;
+  if (targets > 0 && ctx->lib_bands != bands)
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "library has " + std::to_string(ctx->lib_bands) + " bands but the statistics cover " +
+                    std::to_string(bands));
+  if (!(diagonal_loading >= 0.0) || !std::isfinite(diagonal_loading))
+    return fail(ctx, HSI_ERR_INVALID_ARG, "diagonal loading must be finite and non-negative");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  BuildArgs a;
+  a.bands = bands;
+  a.num_targets = targets;
+  a.loading = diagonal_loading;
+  a.count = ctx->d_count;
+  a.sum = ctx->d_sum;
+  a.sum_outer = ctx->d_sum_outer;
+  a.targets = ctx->d_targets;
+  a.work = ctx->d_build_work;
+  a.mean = ctx->d_mean;
+  a.whitener = ctx->d_whitener;
+  a.target_white = ctx->d_target_white;
+  a.target_norm2 = ctx->d_target_norm2;
+  a.cem_weight = ctx->d_cem_weight;
+  a.status = ctx->d_status;
+  HSI_HIP(ctx, launch_build(a, s), "build statistics");
+  hsi_status st = refresh_layouts(ctx, bands, targets, s);
+  if (st != HSI_OK) return st;
+  ctx->stats_bands = bands;
+  ctx->stats_targets = targets;
+  ctx->last_loading = diagonal_loading;
+  return HSI_OK;
+}
+
+hsi_status hsi_stats_check(hsi_ctx* ctx, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (ctx->stats_bands <= 0) return fail(ctx, HSI_ERR_NO_STATISTICS, "no statistics built");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, hipStreamSynchronize(static_cast<hipStream_t>(stream)), "wait for statistics");
+  unsigned status = 0;
+  HSI_HIP(ctx, hipMemcpy(&status, ctx->d_status, sizeof(unsigned), hipMemcpyDeviceToHost),
+          "read statistics status");
+  if (status == 0) return HSI_OK;
+  char loading[64];
+  std::snprintf(loading, sizeof loading, "%f", ctx->last_loading);
+  if (status & HSI_STATUS_COV_NOT_PD)
+    return fail(ctx, HSI_ERR_NOT_POSITIVE_DEFINITE,
+                std::string("scene covariance is not positive definite at loading ") + loading +
+                    "; hyperspectral bands are strongly correlated, so try a larger --loading");
+  return fail(ctx, HSI_ERR_NOT_POSITIVE_DEFINITE,
+              std::string("scene autocorrelation is not positive definite at loading ") + loading +
+                  "; try a larger --loading");
+}
+
+hsi_status hsi_stats_read(hsi_ctx* ctx, float* mean, float* whitener, float* target_white,
+                          float* target_norm2, float* cem_weight, double* count, double* sum,
+                          double* sum_outer) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, hipDeviceSynchronize(), "wait for device");
+  const std::size_t mb = static_cast<std::size_t>(ctx->moment_bands);
+  if (count || sum || sum_outer) {
+    if (mb == 0) return fail(ctx, HSI_ERR_NO_STATISTICS, "no moments accumulated");
+    if (count) HSI_HIP(ctx, hipMemcpy(count, ctx->d_count, sizeof(double), hipMemcpyDeviceToHost), "read count");
+    if (sum) HSI_HIP(ctx, hipMemcpy(sum, ctx->d_sum, mb * sizeof(double), hipMemcpyDeviceToHost), "read sum");
+    if (sum_outer)
+      HSI_HIP(ctx, hipMemcpy(sum_outer, ctx->d_sum_outer, mb * mb * sizeof(double), hipMemcpyDeviceToHost),
+              "read sum_outer");
+  }
+  const std::size_t b = static_cast<std::size_t>(ctx->stats_bands);
+  const std::size_t t = static_cast<std::size_t>(ctx->stats_targets);
+  if (mean || whitener || target_white || target_norm2 || cem_weight) {
+    if (b == 0) return fail(ctx, HSI_ERR_NO_STATISTICS, "no statistics built");
+    if (mean) HSI_HIP(ctx, hipMemcpy(mean, ctx->d_mean, b * sizeof(float), hipMemcpyDeviceToHost), "read mean");
+    if (whitener)
+      HSI_HIP(ctx, hipMemcpy(whitener, ctx->d_whitener, b * b * sizeof(float), hipMemcpyDeviceToHost),
+              "read whitener");
+    if (t > 0) {
+      if (target_white)
+        HSI_HIP(ctx, hipMemcpy(target_white, ctx->d_target_white, t * b * sizeof(float), hipMemcpyDeviceToHost),
+                "read target_white");
+      if (target_norm2)
+        HSI_HIP(ctx, hipMemcpy(target_norm2, ctx->d_target_norm2, t * sizeof(float), hipMemcpyDeviceToHost),
+                "read target_norm2");
+      if (cem_weight)
+        HSI_HIP(ctx, hipMemcpy(cem_weight, ctx->d_cem_weight, t * b * sizeof(float), hipMemcpyDeviceToHost),
+                "read cem_weight");
+    }
+  }
+  return HSI_OK;
+}
+
+hsi_status hsi_stats_upload(hsi_ctx* ctx, int bands, const float* mean, const float* whitener,
+                            const float* target_white, const float* target_norm2,
+                            const float* cem_weight) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (bands <= 0 || bands > kMaxBands || !mean || !whitener)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "upload needs bands, mean and whitener");
+  int targets = 0;
+  if (ctx->num_targets > 0 && ctx->lib_bands == bands && target_white && target_norm2 && cem_weight)
+    targets = ctx->num_targets;
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  const std::size_t b = static_cast<std::size_t>(bands);
+  const std::size_t t = static_cast<std::size_t>(targets);
+  HSI_HIP(ctx, hipMemcpy(ctx->d_mean, mean, b * sizeof(float), hipMemcpyHostToDevice), "upload mean");
+  HSI_HIP(ctx, hipMemcpy(ctx->d_whitener, whitener, b * b * sizeof(float), hipMemcpyHostToDevice),
+          "upload whitener");
+  if (t > 0) {
+    HSI_HIP(ctx, hipMemcpy(ctx->d_target_white, target_white, t * b * sizeof(float), hipMemcpyHostToDevice),
+            "upload target_white");
+    HSI_HIP(ctx, hipMemcpy(ctx->d_target_norm2, target_norm2, t * sizeof(float), hipMemcpyHostToDevice),
+            "upload target_norm2");
+    HSI_HIP(ctx, hipMemcpy(ctx->d_cem_weight, cem_weight, t * b * sizeof(float), hipMemcpyHostToDevice),
+            "upload cem_weight");
+  }
+  HSI_HIP(ctx, hipMemset(ctx->d_status, 0, sizeof(unsigned)), "reset status");
+  hsi_status st = refresh_layouts(ctx, bands, targets, nullptr);
+  if (st != HSI_OK) return st;
+  HSI_HIP(ctx, hipDeviceSynchronize(), "upload statistics");
+  ctx->stats_bands = bands;
+  ctx->stats_targets = targets;
+  return HSI_OK;
+}
+
+hsi_status hsi_score_sam_cem(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype, int height, int width,
+                             int bands, size_t plane_stride, float* d_sam_score, int32_t* d_sam_target,
+                             float* d_cem_score, int32_t* d_cem_target, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, dtype);
+  if (st != HSI_OK) return st;
+  if (!d_sam_score && !d_cem_score) return HSI_OK;
+  if (ctx->num_targets <= 0) return fail(ctx, HSI_ERR_NO_LIBRARY, "empty spectral library");
+  if (ctx->lib_bands != bands)
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "library has " + std::to_string(ctx->lib_bands) + " bands but the cube has " +
+                    std::to_string(bands));
+  if (d_cem_score && (ctx->stats_bands != bands || ctx->stats_targets != ctx->num_targets))
+    return fail(ctx, HSI_ERR_NO_STATISTICS, "CEM needs statistics built for this library and band count");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  SamCemArgs a;
+  a.cube = d_cube;
+  a.dtype = dtype;
+  a.pixels = static_cast<std::size_t>(height) * width;
+  a.bands = bands;
+  a.stride = plane_stride;
+  a.num_targets = ctx->num_targets;
+  a.lib_stride = ctx->lib_stride;
+  a.sam_lib = ctx->d_sam_lib;
+  a.sam_norms = ctx->d_norms;
+  a.cem_lib = ctx->d_cem_lib;
+  a.sam_score = d_sam_score;
+  a.sam_target = d_sam_target;
+  a.cem_score = d_cem_score;
+  a.cem_target = d_cem_target;
+  HSI_HIP(ctx, launch_sam_cem(a, static_cast<hipStream_t>(stream)), "score SAM/CEM");
+  return HSI_OK;
+}
+
+hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precision, const void* d_cube,
+                     hsi_dtype dtype, int height, int width, int bands, size_t plane_stride,
+                     float* d_score, int32_t* d_target, float* d_aux, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (!d_score) return fail(ctx, HSI_ERR_INVALID_ARG, "null score map");
+  if (precision != HSI_PLAIN && precision != HSI_MIXED)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "unknown precision");
+  switch (detector) {
+    case HSI_DET_SAM:
+    case HSI_DET_CEM:
+      if (precision != HSI_PLAIN)
+        return fail(ctx, HSI_ERR_INVALID_ARG, "the mixed form applies to RX and ACE only");
+      return detector == HSI_DET_SAM
+                 ? hsi_score_sam_cem(ctx, d_cube, dtype, height, width, bands, plane_stride, d_score,
+                                     d_target, nullptr, nullptr, stream)
+                 : hsi_score_sam_cem(ctx, d_cube, dtype, height, width, bands, plane_stride, nullptr,
+                                     nullptr, d_score, d_target, stream);
+    case HSI_DET_RX:
+    case HSI_DET_ACE:
+      break;
+    default:
+      return fail(ctx, HSI_ERR_INVALID_ARG, "unknown detector");
+  }
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, dtype);
+  if (st != HSI_OK) return st;
+  if (bands > kMaxRxBands)
+    return fail(ctx, HSI_ERR_CAPACITY, "RX/ACE support at most " + std::to_string(kMaxRxBands) +
+                                           " bands (the whitener tile must fit in LDS)");
+  if (ctx->stats_bands != bands)
+    return fail(ctx, HSI_ERR_NO_STATISTICS, "RX/ACE need statistics built for this band count");
+  if (detector == HSI_DET_ACE && ctx->stats_targets <= 0)
+    return fail(ctx, HSI_ERR_NO_LIBRARY, "ACE needs a target library");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  RxAceArgs a;
+  a.cube = d_cube;
+  a.dtype = dtype;
+  a.pixels = static_cast<std::size_t>(height) * width;
+  a.bands = bands;
+  a.stride = plane_stride;
+  a.num_targets = detector == HSI_DET_ACE ? ctx->stats_targets : 0;
+  a.mean = ctx->d_mean;
+  a.white_blocks = ctx->d_white_blocks;
+  a.ace_u = ctx->d_ace_u;
+  a.target_norm2 = ctx->d_target_norm2;
+  if (detector == HSI_DET_RX) {
+    a.rx = d_score;
+  } else {
+    a.rx = d_aux;
+    a.ace_score = d_score;
+    a.ace_target = d_target;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const hipError_t err = precision == HSI_PLAIN ? launch_rxace_plain(a, s) : launch_rxace_mixed(a, s);
+  if (err == hipErrorNotSupported)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "mixed precision RX/ACE is not available in this build");
+  HSI_HIP(ctx, err, "score RX/ACE");
+  return HSI_OK;
+}
+
+hsi_status hsi_detect(hsi_ctx* ctx, const float* d_score, const int32_t* d_target, int height, int width,
+                      hsi_detection_params params, hsi_detection* d_out, uint32_t* d_count,
+                      hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (!d_score || !d_out || !d_count || height <= 0 || width <= 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "detect: null buffer or bad shape");
+  if (params.max_detections < 0 || params.nms_radius < 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "detect: negative capacity or radius");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  const std::size_t pixels = static_cast<std::size_t>(height) * width;
+  HSI_HIP(ctx, grow(reinterpret_cast<void**>(&ctx->d_det_scratch), &ctx->det_scratch_bytes,
+                    detect_scratch_words(pixels) * sizeof(unsigned)),
+          "allocate detect scratch");
+  HSI_HIP(ctx, launch_detect(d_score, d_target, height, width, params, d_out, d_count,
+                             ctx->d_det_scratch, static_cast<hipStream_t>(stream)),
+          "detect");
+  return HSI_OK;
+}
+
+hsi_status hsi_narrow_to_half(const float* d_src, void* d_dst, size_t count, hsi_stream stream) {
+  if (!d_src || !d_dst) return HSI_ERR_INVALID_ARG;
+  if (count == 0) return HSI_OK;
+  if (launch_narrow_to_half(d_src, static_cast<_Float16*>(d_dst), count, static_cast<hipStream_t>(stream)) !=
+      hipSuccess)
+    return HSI_ERR_DEVICE;
+  return HSI_OK;
+}
+
+hsi_status hsi_transpose_bip_to_bsq(const float* d_src, float* d_dst, int height, int width, int bands,
+                                    size_t plane_stride, hsi_stream stream) {
+  if (!d_src || !d_dst || height <= 0 || width <= 0 || bands <= 0) return HSI_ERR_INVALID_ARG;
+  if (plane_stride < static_cast<std::size_t>(height) * width) return HSI_ERR_INVALID_ARG;
+  if (launch_transpose_bip_to_bsq(d_src, d_dst, height, width, bands, plane_stride,
+                                  static_cast<hipStream_t>(stream)) != hipSuccess)
+    return HSI_ERR_DEVICE;
+  return HSI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// detect.hip - threshold, non-maximum suppression and raster-order compaction.
+//
+// The reference kernel (detect.cu:43-87) reserves output slots with one
+// atomic per warp, so its list order changes run to run, and it ignores the
+// score polarity. This version follows detect_cpu (sam_cpu.cpp:86-138)
+// exactly, including order:
+//   1. flag:  every pixel tests the threshold and, if it passes, its
+//             (2r+1)^2 neighbourhood; each wave64 ballots its survivors into
+//             one 64-bit mask word and each workgroup writes its count.
+//   2. scan:  one workgroup turns the per-workgroup counts into exclusive
+//             offsets and writes the total found.
+//   3. write: each survivor's slot is its workgroup offset plus the
+//             popcount of the survivors before it, so detections land in
+//             raster order; slots past capacity are counted, not written.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace hsi_amd {
+namespace {
+
+constexpr int kDetBlock = 256;
+constexpr int kScanThreads = 1024;
+
+__device__ __forceinline__ bool better(float a, float b, bool higher) { return higher ? a > b : a < b; }
+
+__device__ __forceinline__ bool passes(const float* __restrict__ score, std::size_t p, int width,
+                                       int height, float threshold, int radius, bool higher) {
+  const float self = score[p];
+  if (!(higher ? self >= threshold : self <= threshold)) return false;
+  if (radius <= 0) return true;
+  const int x = static_cast<int>(p % static_cast<std::size_t>(width));
+  const int y = static_cast<int>(p / static_cast<std::size_t>(width));
+  for (int dy = -radius; dy <= radius; ++dy) {
+    const int ny = y + dy;
+    if (ny < 0 || ny >= height) continue;
+    for (int dx = -radius; dx <= radius; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      const int nx = x + dx;
+      if (nx < 0 || nx >= width) continue;
+      const std::size_t q = static_cast<std::size_t>(ny) * width + nx;
+      const float other = score[q];
+      if (better(other, self, higher)) return false;
+      if (other == self && q < p) return false;
+    }
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(kDetBlock) void detect_flag_kernel(
+    const float* __restrict__ score, int width, int height, float threshold, int radius,
+    int higher, unsigned long long* __restrict__ masks, unsigned* __restrict__ block_counts) {
+  __shared__ unsigned wave_counts[kDetBlock / kWave];
+  const std::size_t pixels = static_cast<std::size_t>(width) * height;
+  const std::size_t p = blockIdx.x * static_cast<std::size_t>(kDetBlock) + threadIdx.x;
+  const bool pass = p < pixels && passes(score, p, width, height, threshold, radius, higher != 0);
+  const unsigned long long ballot = __ballot(pass);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    masks[blockIdx.x * static_cast<std::size_t>(kDetBlock / kWave) + wave] = ballot;
+    wave_counts[wave] = static_cast<unsigned>(__popcll(ballot));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned total = 0;
+    for (int w = 0; w < kDetBlock / kWave; ++w) total += wave_counts[w];
+    block_counts[blockIdx.x] = total;
+  }
+}
+
+// Exclusive scan of the per-block counts in one workgroup.
+__global__ __launch_bounds__(kScanThreads) void detect_scan_kernel(
+    const unsigned* __restrict__ block_counts, unsigned* __restrict__ block_offsets,
+    unsigned blocks, unsigned* __restrict__ total_out) {
+  __shared__ unsigned partial[kScanThreads];
+  const unsigned per = (blocks + kScanThreads - 1) / kScanThreads;
+  const unsigned begin = threadIdx.x * per;
+  unsigned local = 0;
+  for (unsigned i = 0; i < per; ++i) {
+    const unsigned b = begin + i;
+    if (b < blocks) local += block_counts[b];
+  }
+  partial[threadIdx.x] = local;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over the thread sums.
+  for (unsigned off = 1; off < kScanThreads; off <<= 1) {
+    const unsigned v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0u;
+    __syncthreads();
+    partial[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned run = partial[threadIdx.x] - local;
+  for (unsigned i = 0; i < per; ++i) {
+    const unsigned b = begin + i;
+    if (b < blocks) {
+      block_offsets[b] = run;
+      run += block_counts[b];
+    }
+  }
+  if (threadIdx.x == kScanThreads - 1) *total_out = partial[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kDetBlock) void detect_write_kernel(
+    const float* __restrict__ score, const int* __restrict__ target, int width, int height,
+    const unsigned long long* __restrict__ masks, const unsigned* __restrict__ block_offsets,
+    int capacity, hsi_detection* __restrict__ out) {
+  const std::size_t pixels = static_cast<std::size_t>(width) * height;
+  const std::size_t p = blockIdx.x * static_cast<std::size_t>(kDetBlock) + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const unsigned long long* block_masks = masks + blockIdx.x * static_cast<std::size_t>(kDetBlock / kWave);
+  const unsigned long long mine = block_masks[wave];
+  if (p >= pixels || !((mine >> lane) & 1ull)) return;
+  unsigned rank = block_offsets[blockIdx.x];
+  for (int w = 0; w < wave; ++w) rank += static_cast<unsigned>(__popcll(block_masks[w]));
+  rank += static_cast<unsigned>(__popcll(mine & ((1ull << lane) - 1ull)));
+  if (rank < static_cast<unsigned>(capacity)) {
+    hsi_detection d;
+    d.x = static_cast<int>(p % static_cast<std::size_t>(width));
+    d.y = static_cast<int>(p / static_cast<std::size_t>(width));
+    d.target = target ? target[p] : 0;
+    d.score = score[p];
+    out[rank] = d;
+  }
+}
+
+}  // namespace
+
+std::size_t detect_scratch_words(std::size_t pixels) {
+  const std::size_t blocks = div_up(pixels, kDetBlock);
+  // masks (2 words each), counts, offsets
+  return blocks * (kDetBlock / kWave) * 2 + 2 * blocks;
+}
+
+hipError_t launch_detect(const float* score, const int* target, int height, int width,
+                         hsi_detection_params params, hsi_detection* out, unsigned* count,
+                         unsigned* scratch, hipStream_t stream) {
+  const std::size_t pixels = static_cast<std::size_t>(width) * height;
+  const std::size_t blocks = div_up(pixels, kDetBlock);
+  auto* masks = reinterpret_cast<unsigned long long*>(scratch);
+  unsigned* counts = scratch + blocks * (kDetBlock / kWave) * 2;
+  unsigned* offsets = counts + blocks;
+  hipLaunchKernelGGL(detect_flag_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kDetBlock), 0,
+                     stream, score, width, height, params.threshold, params.nms_radius,
+                     params.higher_is_better, masks, counts);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(detect_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, counts, offsets,
+                     static_cast<unsigned>(blocks), count);
+  err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(detect_write_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kDetBlock), 0,
+                     stream, score, target, width, height, masks, offsets, params.max_detections,
+                     out);
+  return hipGetLastError();
+}
+
+}  // namespace hsi_amd

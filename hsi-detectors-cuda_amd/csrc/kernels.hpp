@@ -1,0 +1,130 @@
+// kernels.hpp - launchers shared between the kernel files and the C ABI.
+#pragma once
+
+#include "common.hpp"
+
+namespace hsi_amd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+// ---- SAM / CEM ---------------------------------------------------------------
+struct SamCemArgs {
+  const void* cube = nullptr;
+  hsi_dtype dtype = HSI_F32;
+  std::size_t pixels = 0;
+  int bands = 0;
+  std::size_t stride = 0;
+  int num_targets = 0;
+  int lib_stride = 1;              // sam_cem_lib_stride(num_targets)
+  const double* sam_lib = nullptr;  // [bands][lib_stride] targets, zero padded
+  const float* sam_norms = nullptr; // [num_targets]
+  const double* cem_lib = nullptr;  // [bands][lib_stride] CEM filters, zero padded
+  float* sam_score = nullptr;
+  int* sam_target = nullptr;
+  float* cem_score = nullptr;
+  int* cem_target = nullptr;
+};
+int sam_cem_lib_stride(int num_targets);
+hipError_t launch_sam_cem(const SamCemArgs& args, hipStream_t stream);
+
+// ---- statistics --------------------------------------------------------------
+// Bands padded for the 16x16 matrix-core blocks, with one extra "ones" band
+// at index `bands` so the same accumulation yields sum(x) and the count.
+__host__ __device__ inline int moment_blocks(int bands) { return (bands + 1 + 15) / 16; }
+inline int moment_pairs(int bands) {
+  const int nb = moment_blocks(bands);
+  return nb * (nb + 1) / 2;
+}
+// Scratch doubles for the per-workgroup partials.
+std::size_t moments_scratch_doubles(std::size_t pixels, int bands);
+hipError_t launch_moments(const float* cube, std::size_t pixels, int bands, std::size_t stride,
+                          hsi_precision precision, double* scratch, double* count, double* sum,
+                          double* sum_outer, hipStream_t stream);
+
+struct BuildArgs {
+  int bands = 0;
+  int num_targets = 0;
+  double loading = 0.0;
+  const double* count = nullptr;
+  const double* sum = nullptr;
+  const double* sum_outer = nullptr;
+  const float* targets = nullptr;  // [T][B]
+  double* work = nullptr;          // build_work_doubles(bands, T)
+  float* mean = nullptr;
+  float* whitener = nullptr;
+  float* target_white = nullptr;
+  float* target_norm2 = nullptr;
+  float* cem_weight = nullptr;
+  unsigned* status = nullptr;
+};
+std::size_t build_work_doubles(int bands, int num_targets);
+hipError_t launch_build(const BuildArgs& args, hipStream_t stream);
+
+// Derived layouts the scoring kernels read, refreshed after every build or
+// upload: CEM filters as doubles band-major for the SAM/CEM kernel; the
+// whitener as doubles in 16x16 lower blocks, and the whitened targets, for
+// the RX/ACE kernels.
+struct LayoutArgs {
+  int bands = 0;
+  int num_targets = 0;
+  int lib_stride = 1;
+  const float* whitener = nullptr;
+  const float* target_white = nullptr;
+  const float* cem_weight = nullptr;
+  const float* mean = nullptr;
+  double* cem_lib = nullptr;        // [bands][lib_stride]
+  float* white_blocks = nullptr;    // rxace_white_floats(bands)
+  float* ace_u = nullptr;           // rxace_target_floats(bands, T)
+  _Float16* white_hi = nullptr;     // rxace_white_halves(bands)
+  _Float16* white_lo = nullptr;
+  _Float16* ace_u_h = nullptr;      // rxace_target_halves(bands, T)
+  float* white_scale = nullptr;     // per 16-row block scale of the fp16 whitener
+  double* white_mean = nullptr;     // W * mean, [nb*16]
+};
+hipError_t launch_layouts(const LayoutArgs& args, hipStream_t stream);
+
+// ---- RX / ACE ----------------------------------------------------------------
+__host__ __device__ inline int rx_blocks(int bands) { return (bands + 15) / 16; }
+std::size_t rxace_white_floats(int bands);
+std::size_t rxace_target_floats(int bands, int num_targets);
+std::size_t rxace_white_halves(int bands);
+std::size_t rxace_target_halves(int bands, int num_targets);
+
+struct RxAceArgs {
+  const void* cube = nullptr;
+  hsi_dtype dtype = HSI_F32;
+  std::size_t pixels = 0;
+  int bands = 0;
+  std::size_t stride = 0;
+  int num_targets = 0;  // 0 for RX only
+  const float* mean = nullptr;
+  const float* white_blocks = nullptr;
+  const float* ace_u = nullptr;
+  const float* target_norm2 = nullptr;
+  const _Float16* white_hi = nullptr;
+  const _Float16* white_lo = nullptr;
+  const _Float16* ace_u_h = nullptr;
+  const float* white_scale = nullptr;
+  const double* white_mean = nullptr;
+  float* rx = nullptr;
+  float* ace_score = nullptr;
+  int* ace_target = nullptr;
+};
+hipError_t launch_rxace_plain(const RxAceArgs& args, hipStream_t stream);
+hipError_t launch_rxace_mixed(const RxAceArgs& args, hipStream_t stream);
+
+// ---- detect ------------------------------------------------------------------
+std::size_t detect_scratch_words(std::size_t pixels);
+hipError_t launch_detect(const float* score, const int* target, int height, int width,
+                         hsi_detection_params params, hsi_detection* out, unsigned* count,
+                         unsigned* scratch, hipStream_t stream);
+
+// ---- ingest ------------------------------------------------------------------
+hipError_t launch_narrow_to_half(const float* src, _Float16* dst, std::size_t count,
+                                 hipStream_t stream);
+hipError_t launch_transpose_bip_to_bsq(const float* src, float* dst, int height, int width,
+                                       int bands, std::size_t stride, hipStream_t stream);
+
+}  // namespace hsi_amd

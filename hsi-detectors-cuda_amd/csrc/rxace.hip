@@ -1,0 +1,269 @@
+// rxace.hip - RX and ACE as a whitening GEMM on the gfx950 matrix cores.
+//
+// For a tile of 16 pixels, Z = L^-1 (X - mean) is a (B x B lower) x (B x 16)
+// product. It is computed one 16-band row block at a time,
+//     Z_I = sum over J <= I of W_IJ Y_J,
+// with v_mfma_f64_16x16x4 (FP64 in, FP64 accumulate: the precision the CPU
+// oracle uses in rx_cpu / ace_best_cpu, detectors_cpu.cpp:95-156). Each
+// finished Z_I block feeds
+//   RX:  sum of squares down the block's rows (||z||^2), and
+//   ACE: z . u_t for up to 32 targets, as four more MFMAs per target tile
+//        that consume the Z_I accumulator registers directly as the B
+//        operand: in K-step r lane l holds row 4*(l/16)+r of Z_I, so the
+//        target tile's A operand is laid out with the same permutation.
+// The whitener blocks and the target tile live in LDS as floats (the
+// reference narrows them to float: detector.cpp:83-84) and are widened per
+// MFMA; each wave keeps its pixel tile's centred spectrum in registers in the
+// MFMA B-operand layout and walks tiles persistently.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace hsi_amd {
+namespace {
+
+constexpr int kRxWaves = 4;
+
+__device__ __forceinline__ double cvt_in(float v) { return static_cast<double>(v); }
+__device__ __forceinline__ double cvt_in(_Float16 v) { return static_cast<double>(static_cast<float>(v)); }
+
+// Combine two (score, target) candidates so the larger score wins and, on
+// equal scores, the smaller target index - which is what a strict `>` scan
+// in target order keeps (ace_best_cpu, detectors_cpu.cpp:147-154).
+__device__ __forceinline__ void ace_merge(double& s, int& t, double s2, int t2) {
+  if (t2 < 0) return;
+  if (t < 0 || s2 > s || (s2 == s && t2 < t)) {
+    s = s2;
+    t = t2;
+  }
+}
+
+template <typename In, int NB, int TT>
+__global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
+    const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
+    int num_targets, const float* __restrict__ mean, const float* __restrict__ white_blocks,
+    const float* __restrict__ ace_u, const float* __restrict__ target_norm2,
+    float* __restrict__ rx_out, float* __restrict__ ace_score, int* __restrict__ ace_target) {
+  constexpr int NPAIR = NB * (NB + 1) / 2;
+  constexpr int WFLOATS = NPAIR * 256;
+  constexpr int UFLOATS = (TT > 0 ? TT : 1) * NB * 256;
+  __shared__ float w_lds[WFLOATS];
+  __shared__ float u_lds[UFLOATS];
+  __shared__ float mean_lds[NB * 16];
+
+  for (int e = threadIdx.x; e < WFLOATS; e += blockDim.x) w_lds[e] = white_blocks[e];
+  if constexpr (TT > 0)
+    for (int e = threadIdx.x; e < UFLOATS; e += blockDim.x) u_lds[e] = ace_u[e];
+  for (int e = threadIdx.x; e < NB * 16; e += blockDim.x) mean_lds[e] = e < bands ? mean[e] : 0.f;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;   // lane group: K index of the A/B operands
+  const int n = lane & 15;   // pixel column within the tile
+  const std::size_t tiles = div_up(pixels, 16);
+  const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(kRxWaves) + (threadIdx.x >> 6);
+  const std::size_t waves = static_cast<std::size_t>(gridDim.x) * kRxWaves;
+
+  double uu[TT > 0 ? TT : 1][4];
+  if constexpr (TT > 0) {
+#pragma unroll
+    for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int t = 16 * tt + 4 * g + r;
+        uu[tt][r] = t < num_targets ? static_cast<double>(target_norm2[t]) : 0.0;
+      }
+  }
+
+  for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
+    const std::size_t p = tile * 16 + n;
+    const bool live = p < pixels;
+    // Z = W Y accumulated one input band block J at a time, so only that
+    // block of the centred spectrum is live: y[kk] = x[16J+4kk+g][p] - mean.
+    d4 z[NB];
+#pragma unroll
+    for (int I = 0; I < NB; ++I) z[I] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int J = 0; J < NB; ++J) {
+      double y[4];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int b = 16 * J + 4 * kk + g;
+        double v = 0.0;
+        if (live && b < bands)
+          v = cvt_in(cube[static_cast<std::size_t>(b) * stride + p]) - static_cast<double>(mean_lds[b]);
+        y[kk] = v;
+      }
+#pragma unroll
+      for (int I = 0; I < NB; ++I) {
+        if (I >= J) {  // lower blocks only; uniform across the wave
+          const float* blk = w_lds + (I * (I + 1) / 2 + J) * 256;
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk)
+            z[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(blk[kk * 64 + lane]),
+                                                        y[kk], z[I], 0, 0, 0);
+        }
+      }
+    }
+
+    double zz = 0.0;
+    d4 zu[TT > 0 ? TT : 1];
+#pragma unroll
+    for (int tt = 0; tt < (TT > 0 ? TT : 1); ++tt) zu[tt] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int I = 0; I < NB; ++I) {
+      zz += z[I][0] * z[I][0] + z[I][1] * z[I][1] + z[I][2] * z[I][2] + z[I][3] * z[I][3];
+      if constexpr (TT > 0) {
+#pragma unroll
+        for (int tt = 0; tt < TT; ++tt) {
+          const float* ub = u_lds + (tt * NB + I) * 256;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            zu[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(ub[r * 64 + lane]),
+                                                          z[I][r], zu[tt], 0, 0, 0);
+        }
+      }
+    }
+    // ||z||^2 for pixel n: the four lane groups hold disjoint rows.
+    zz += __shfl_xor(zz, 16, 64);
+    zz += __shfl_xor(zz, 32, 64);
+
+    if constexpr (TT > 0) {
+      // Lane (g, n) holds z.u for targets 16*tt + 4*g + r of pixel n.
+      double best = -1.0;
+      int best_t = -1;
+#pragma unroll
+      for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int t = 16 * tt + 4 * g + r;
+          const double u2 = uu[tt][r];
+          if (t < num_targets && u2 > 0.0 && zz > 0.0) {
+            const double s = (zu[tt][r] * zu[tt][r]) / (u2 * zz);
+            if (s == s) ace_merge(best, best_t, s, t);
+          }
+        }
+      // Merge across the four lane groups of the same pixel.
+#pragma unroll
+      for (int off = 16; off <= 32; off <<= 1) {
+        const double s2 = __shfl_xor(best, off, 64);
+        const int t2 = __shfl_xor(best_t, off, 64);
+        ace_merge(best, best_t, s2, t2);
+      }
+      if (live && g == 0) {
+        const double clamped = best < 0.0 ? 0.0 : (1.0 < best ? 1.0 : best);
+        if (ace_score) ace_score[p] = static_cast<float>(clamped);
+        if (ace_target) ace_target[p] = best_t;
+      }
+    }
+    if (live && g == 0 && rx_out) rx_out[p] = static_cast<float>(zz);
+  }
+}
+
+template <typename In, int NB, int TT>
+hipError_t launch_plain_t(const RxAceArgs& a, hipStream_t stream) {
+  const std::size_t tiles = div_up(a.pixels, 16);
+  std::size_t wgs = div_up(tiles, kRxWaves);
+  if (wgs > 1024) wgs = 1024;
+  hipLaunchKernelGGL((rxace_plain_kernel<In, NB, TT>), dim3(static_cast<unsigned>(wgs)),
+                     dim3(kRxWaves * 64), 0, stream, static_cast<const In*>(a.cube), a.pixels,
+                     a.bands, a.stride, a.num_targets, a.mean, a.white_blocks, a.ace_u,
+                     a.target_norm2, a.rx, a.ace_score, a.ace_target);
+  return hipGetLastError();
+}
+
+template <typename In, int NB>
+hipError_t launch_plain_nb(const RxAceArgs& a, hipStream_t stream) {
+  if (a.num_targets == 0 || (!a.ace_score && !a.ace_target)) return launch_plain_t<In, NB, 0>(a, stream);
+  if (a.num_targets <= 16) return launch_plain_t<In, NB, 1>(a, stream);
+  return launch_plain_t<In, NB, 2>(a, stream);
+}
+
+template <typename In>
+hipError_t launch_plain_typed(const RxAceArgs& a, hipStream_t stream) {
+  switch (rx_blocks(a.bands)) {
+#define HSI_NB(N) case N: return launch_plain_nb<In, N>(a, stream);
+    HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8)
+    HSI_NB(9) HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14)
+#undef HSI_NB
+  }
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Derived layouts.
+
+__global__ void layouts_kernel(LayoutArgs a) {
+  const int n = a.bands;
+  const int nb = rx_blocks(n);
+  const int npair = nb * (nb + 1) / 2;
+  const int T = a.num_targets;
+  const int tiles = T > 16 ? 2 : 1;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int threads = gridDim.x * blockDim.x;
+  // CEM filters, band-major doubles, zero padded to lib_stride.
+  if (a.cem_lib) {
+    for (int e = tid; e < n * a.lib_stride; e += threads) {
+      const int b = e / a.lib_stride;
+      const int t = e % a.lib_stride;
+      a.cem_lib[e] = t < T ? static_cast<double>(a.cem_weight[static_cast<std::size_t>(t) * n + b]) : 0.0;
+    }
+  }
+  // Whitener blocks in A-operand order: [pair][kk][lane] =
+  // W[16I + lane%16][16J + 4kk + lane/16].
+  for (int e = tid; e < npair * 256; e += threads) {
+    const int q = e / 256;
+    const int kk = (e % 256) / 64;
+    const int lane = e % 64;
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= q) ++I;
+    const int J = q - I * (I + 1) / 2;
+    const int row = 16 * I + (lane & 15);
+    const int col = 16 * J + 4 * kk + (lane >> 4);
+    a.white_blocks[e] = (row < n && col < n) ? a.whitener[static_cast<std::size_t>(row) * n + col] : 0.f;
+  }
+  // Whitened targets in the permuted A-operand order: [tt][I][r][lane] =
+  // U[16tt + lane%16][16I + 4(lane/16) + r].
+  if (T > 0 && a.ace_u) {
+    for (int e = tid; e < tiles * nb * 256; e += threads) {
+      const int tt = e / (nb * 256);
+      const int I = (e / 256) % nb;
+      const int r = (e % 256) / 64;
+      const int lane = e % 64;
+      const int t = 16 * tt + (lane & 15);
+      const int b = 16 * I + 4 * (lane >> 4) + r;
+      a.ace_u[e] = (t < T && b < n) ? a.target_white[static_cast<std::size_t>(t) * n + b] : 0.f;
+    }
+  }
+}
+
+}  // namespace
+
+std::size_t rxace_white_floats(int bands) {
+  const int nb = rx_blocks(bands);
+  return static_cast<std::size_t>(nb) * (nb + 1) / 2 * 256;
+}
+std::size_t rxace_target_floats(int bands, int num_targets) {
+  return static_cast<std::size_t>(num_targets > 16 ? 2 : 1) * rx_blocks(bands) * 256;
+}
+std::size_t rxace_white_halves(int bands) { return rxace_white_floats(bands); }
+std::size_t rxace_target_halves(int bands, int num_targets) {
+  return rxace_target_floats(bands, num_targets);
+}
+
+hipError_t launch_rxace_plain(const RxAceArgs& a, hipStream_t stream) {
+  if (a.dtype == HSI_F16) return launch_plain_typed<_Float16>(a, stream);
+  return launch_plain_typed<float>(a, stream);
+}
+
+hipError_t launch_rxace_mixed(const RxAceArgs& a, hipStream_t stream) {
+  (void)a;
+  (void)stream;
+  return hipErrorNotSupported;
+}
+
+hipError_t launch_layouts(const LayoutArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(layouts_kernel, dim3(64), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace hsi_amd

@@ -1,0 +1,255 @@
+// sam_cem.hip - SAM and CEM scoring over a BSQ cube (gfx950).
+//
+// Both detectors are bandwidth-bound: per pixel they read `bands` elements
+// once and do 2 FLOP per band per target. The kernel streams the cube with
+// 16-byte loads (four FP32 or four FP16 pixels per lane per band, so one
+// wave64 load instruction moves a contiguous 1 KiB / 512 B run of a band
+// plane) and scores every target from that one read.
+//
+// Arithmetic is FP64: each float (or half) is widened exactly, every product
+// x*t of two floats is exact in double, and sums run in band order - the
+// same values the reference CPU scorer forms (sam_cpu.cpp:48-77,
+// detectors_cpu.cpp:158-187), so scores agree to the last bit apart from the
+// final libm-vs-device acos. On gfx950 an FP64 FMA issues at the same rate
+// as a non-packed FP32 FMA, so exactness costs no throughput here.
+//
+// The target library is uniform across the wavefront: it is stored as
+// doubles band-major, lib[b * NT + t], and the compiler reads each band's NT
+// values with scalar loads into SGPRs that feed the FMAs directly - the MI355X
+// counterpart of the reference's __constant__ broadcast (sam.cu:19-23).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace hsi_amd {
+namespace {
+
+template <typename In, int PPT>
+struct VecOf;
+template <>
+struct VecOf<float, 4> {
+  using type = float4;
+};
+template <>
+struct VecOf<float, 2> {
+  using type = float2;
+};
+template <>
+struct VecOf<_Float16, 4> {
+  typedef _Float16 type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct VecOf<_Float16, 2> {
+  typedef _Float16 type __attribute__((ext_vector_type(2)));
+};
+
+template <typename V, int PPT>
+__device__ __forceinline__ void widen(const V& v, double (&x)[PPT]) {
+  if constexpr (PPT == 4) {
+    x[0] = static_cast<double>(static_cast<float>(v.x));
+    x[1] = static_cast<double>(static_cast<float>(v.y));
+    x[2] = static_cast<double>(static_cast<float>(v.z));
+    x[3] = static_cast<double>(static_cast<float>(v.w));
+  } else {
+    x[0] = static_cast<double>(static_cast<float>(v.x));
+    x[1] = static_cast<double>(static_cast<float>(v.y));
+  }
+}
+
+// Best-match state for one pixel, in angle order without taking an acos per
+// target. kind 0: nothing yet (angle +inf); 1: degenerate (pixel or target
+// without direction, angle = float(pi/2) as angle_of returns); 2: a clamped
+// cosine. A candidate replaces the best only when its angle is strictly
+// smaller, so the first target keeps a tie, as sam_best_cpu's `a < best`.
+struct SamBest {
+  int kind = 0;
+  double cosine = 0.0;
+  int target = -1;
+};
+
+__device__ __forceinline__ void sam_offer(SamBest& best, bool degenerate, double c, int t) {
+  if (degenerate) {
+    if (best.kind == 0) {
+      best.kind = 1;
+      best.target = t;
+    } else if (best.kind == 2 && kMaxSamAngle < acos(best.cosine)) {
+      best.kind = 1;
+      best.target = t;
+    }
+    return;
+  }
+  if (!(c == c)) return;  // NaN never wins a strict comparison
+  if (best.kind == 0) {
+    best.kind = 2;
+    best.cosine = c;
+    best.target = t;
+  } else if (best.kind == 1) {
+    if (acos(c) < kMaxSamAngle) {
+      best.kind = 2;
+      best.cosine = c;
+      best.target = t;
+    }
+  } else if (c > best.cosine) {
+    // acos is decreasing, so a larger cosine is a smaller angle - unless the
+    // two cosines are close enough to round to the same angle (only within a
+    // few 1e-16 of each other), where the earlier target must keep the tie.
+    if (c - best.cosine > 4e-16 || acos(c) < acos(best.cosine)) {
+      best.cosine = c;
+      best.target = t;
+    }
+  }
+}
+
+template <typename In, int NT, int PPT, bool DO_SAM, bool DO_CEM>
+__global__ __launch_bounds__(256) void sam_cem_kernel(
+    const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
+    const double* __restrict__ slib, const float* __restrict__ snorm,
+    const double* __restrict__ clib, int num_targets, float* __restrict__ s_score,
+    int* __restrict__ s_target, float* __restrict__ c_score, int* __restrict__ c_target) {
+  using V = typename VecOf<In, PPT>::type;
+  const std::size_t p0 = (blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x) * PPT;
+  if (p0 >= pixels) return;
+
+  double sq[PPT];
+  double sd[DO_SAM ? NT : 1][PPT];
+  double cd[DO_CEM ? NT : 1][PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    sq[k] = 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if constexpr (DO_SAM) sd[t][k] = 0.0;
+      if constexpr (DO_CEM) cd[t][k] = 0.0;
+    }
+  }
+
+  const In* base = cube + p0;
+  constexpr int U = 4;  // band planes in flight per lane
+  int b = 0;
+  for (; b + U <= bands; b += U) {
+    V v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      v[u] = *reinterpret_cast<const V*>(base + static_cast<std::size_t>(b + u) * stride);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      double x[PPT];
+      widen<V, PPT>(v[u], x);
+      const double* sl = slib + static_cast<std::size_t>(b + u) * NT;
+      const double* cl = clib + static_cast<std::size_t>(b + u) * NT;
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        if constexpr (DO_SAM) sq[k] = __fma_rn(x[k], x[k], sq[k]);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          if constexpr (DO_SAM) sd[t][k] = __fma_rn(x[k], sl[t], sd[t][k]);
+          if constexpr (DO_CEM) cd[t][k] = __fma_rn(cl[t], x[k], cd[t][k]);
+        }
+      }
+    }
+  }
+  for (; b < bands; ++b) {
+    const V v = *reinterpret_cast<const V*>(base + static_cast<std::size_t>(b) * stride);
+    double x[PPT];
+    widen<V, PPT>(v, x);
+    const double* sl = slib + static_cast<std::size_t>(b) * NT;
+    const double* cl = clib + static_cast<std::size_t>(b) * NT;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      if constexpr (DO_SAM) sq[k] = __fma_rn(x[k], x[k], sq[k]);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        if constexpr (DO_SAM) sd[t][k] = __fma_rn(x[k], sl[t], sd[t][k]);
+        if constexpr (DO_CEM) cd[t][k] = __fma_rn(cl[t], x[k], cd[t][k]);
+      }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const std::size_t p = p0 + k;
+    if (p >= pixels) break;
+    if constexpr (DO_SAM) {
+      // cosine = dot / (sqrt(|x|^2) * |t|), clamped, as angle_of
+      // (sam_cpu.cpp:13-18): correctly rounded sqrt, product and quotient.
+      const double root = sqrt(sq[k]);
+      SamBest best;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        if (t < num_targets) {
+          const double norm = static_cast<double>(snorm[t]);
+          const bool degenerate = sq[k] <= 0.0 || norm <= 0.0;
+          double c = sd[t][k] / (root * norm);
+          c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+          sam_offer(best, degenerate, c, t);
+        }
+      }
+      double angle = __longlong_as_double(0x7ff0000000000000ll);  // +inf
+      if (best.kind == 1) angle = kMaxSamAngle;
+      else if (best.kind == 2) angle = acos(best.cosine);
+      s_score[p] = static_cast<float>(angle);
+      if (s_target) s_target[p] = best.target;
+    }
+    if constexpr (DO_CEM) {
+      double best = -__longlong_as_double(0x7ff0000000000000ll);
+      int best_t = -1;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        if (t < num_targets && cd[t][k] > best) {
+          best = cd[t][k];
+          best_t = t;
+        }
+      }
+      c_score[p] = static_cast<float>(best);
+      if (c_target) c_target[p] = best_t;
+    }
+  }
+}
+
+template <typename In, int NT, int PPT, bool S, bool Cm>
+hipError_t launch_one(const SamCemArgs& a, hipStream_t stream) {
+  const std::size_t groups = div_up(a.pixels, PPT);
+  const unsigned blocks = static_cast<unsigned>(div_up(groups, 256));
+  hipLaunchKernelGGL((sam_cem_kernel<In, NT, PPT, S, Cm>), dim3(blocks), dim3(256), 0, stream,
+                     static_cast<const In*>(a.cube), a.pixels, a.bands, a.stride, a.sam_lib,
+                     a.sam_norms, a.cem_lib, a.num_targets, a.sam_score, a.sam_target,
+                     a.cem_score, a.cem_target);
+  return hipGetLastError();
+}
+
+template <typename In, int NT, int PPT>
+hipError_t launch_mode(const SamCemArgs& a, hipStream_t stream) {
+  const bool s = a.sam_score != nullptr;
+  const bool c = a.cem_score != nullptr;
+  if (s && c) return launch_one<In, NT, PPT, true, true>(a, stream);
+  if (s) return launch_one<In, NT, PPT, true, false>(a, stream);
+  return launch_one<In, NT, PPT, false, true>(a, stream);
+}
+
+template <typename In>
+hipError_t launch_typed(const SamCemArgs& a, hipStream_t stream) {
+  switch (a.lib_stride) {
+    case 1: return launch_mode<In, 1, 4>(a, stream);
+    case 2: return launch_mode<In, 2, 4>(a, stream);
+    case 4: return launch_mode<In, 4, 4>(a, stream);
+    case 8: return launch_mode<In, 8, 4>(a, stream);
+    case 16: return launch_mode<In, 16, 2>(a, stream);
+    case 32: return launch_mode<In, 32, 2>(a, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int sam_cem_lib_stride(int num_targets) {
+  int nt = 1;
+  while (nt < num_targets) nt *= 2;
+  return nt;
+}
+
+hipError_t launch_sam_cem(const SamCemArgs& a, hipStream_t stream) {
+  if (!a.sam_score && !a.cem_score) return hipSuccess;
+  if (a.dtype == HSI_F16) return launch_typed<_Float16>(a, stream);
+  return launch_typed<float>(a, stream);
+}
+
+}  // namespace hsi_amd

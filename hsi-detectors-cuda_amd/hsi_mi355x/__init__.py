@@ -1,0 +1,343 @@
+"""Python binding of the MI355X frame-scoring library (include/hsi_mi355x.h).
+
+A thin ctypes layer over ``lib/libhsi_mi355x.so``: every call goes through the
+C ABI. PyTorch supplies device memory and the stream (tensor ``data_ptr()``
+and ``torch.cuda.current_stream().cuda_stream``); it computes nothing here.
+There is no CPU fallback: if the library is not built, or no GPU is visible,
+the calls raise.
+
+Names follow the reference's host API: ``Detector`` mirrors hsi::Detector
+(include/hsi/detector.hpp:23-42), ``traits_of`` mirrors DetectorTraits
+(detector.hpp:44-55), ``DetectionParams`` mirrors hsi::DetectionParams
+(include/hsi/types.hpp:122-140) and ``Detection`` hsi::Detection
+(types.hpp:101-106).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libhsi_mi355x.so")
+
+
+class HsiError(RuntimeError):
+    """A non-OK status from the C ABI, with the context's message."""
+
+    def __init__(self, status, message):
+        super().__init__(f"{status_name(status)}: {message}")
+        self.status = status
+
+
+OK, ERR_INVALID_ARG, ERR_CAPACITY, ERR_NOT_POSITIVE_DEFINITE, ERR_DEVICE, ERR_NO_STATISTICS, \
+    ERR_NO_LIBRARY = range(7)
+F32, F16 = 0, 1
+PLAIN, MIXED = 0, 1
+
+
+class Detector(enum.IntEnum):
+    SAM = 0
+    RX = 1
+    ACE = 2
+    CEM = 3
+
+
+@dataclass(frozen=True)
+class DetectorTraits:
+    name: str
+    needs_targets: bool
+    needs_statistics: bool
+    higher_is_better: bool
+    compute_bound: bool
+
+
+_TRAITS = {
+    Detector.SAM: DetectorTraits("sam", True, False, False, False),
+    Detector.RX: DetectorTraits("rx", False, True, True, True),
+    Detector.ACE: DetectorTraits("ace", True, True, True, True),
+    Detector.CEM: DetectorTraits("cem", True, True, True, False),
+}
+
+
+def traits_of(detector):
+    return _TRAITS[Detector(detector)]
+
+
+def parse_detector(name):
+    for d, t in _TRAITS.items():
+        if t.name == name:
+            return d
+    raise ValueError(f"unknown detector '{name}' (sam, rx, ace, cem)")
+
+
+class _Detection(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("target", C.c_int32), ("score", C.c_float)]
+
+
+DETECTION_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("target", "<i4"), ("score", "<f4")])
+
+
+class DetectionParams(C.Structure):
+    """hsi_detection_params; defaults as hsi::DetectionParams."""
+
+    _fields_ = [("threshold", C.c_float), ("nms_radius", C.c_int32),
+                ("max_detections", C.c_int32), ("higher_is_better", C.c_int32)]
+
+    def __init__(self, threshold=0.10, nms_radius=2, max_detections=4096, higher_is_better=False):
+        super().__init__(threshold, nms_radius, max_detections, int(bool(higher_is_better)))
+
+
+class SynthConfig(C.Structure):
+    _fields_ = [("height", C.c_int32), ("width", C.c_int32), ("bands", C.c_int32),
+                ("num_targets", C.c_int32), ("num_endmembers", C.c_int32),
+                ("drift_period", C.c_int32), ("plant_fraction", C.c_double),
+                ("anomaly_fraction", C.c_double), ("noise_sigma", C.c_double),
+                ("drift", C.c_double), ("seed", C.c_uint64)]
+
+
+_lib = None
+_vp = C.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                           f"(or `make -C hsi-detectors-cuda_amd`)")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "hsi_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+        "hsi_ctx_destroy": (None, [_vp]),
+        "hsi_ctx_last_error": (C.c_char_p, [_vp]),
+        "hsi_status_name": (C.c_char_p, [C.c_int]),
+        "hsi_ctx_reserve": (C.c_int, [_vp, C.c_size_t, C.c_int]),
+        "hsi_plane_stride": (C.c_size_t, [C.c_int, C.c_int]),
+        "hsi_max_targets": (C.c_int, []),
+        "hsi_max_bands": (C.c_int, []),
+        "hsi_set_library": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int]),
+        "hsi_stats_accumulate": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t,
+                                           C.c_int, _vp]),
+        "hsi_stats_build": (C.c_int, [_vp, C.c_double, _vp]),
+        "hsi_stats_check": (C.c_int, [_vp, _vp]),
+        "hsi_stats_read": (C.c_int, [_vp] + [_vp] * 8),
+        "hsi_stats_upload": (C.c_int, [_vp, C.c_int] + [_vp] * 5),
+        "hsi_score": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.c_size_t, _vp, _vp, _vp, _vp]),
+        "hsi_score_sam_cem": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t,
+                                        _vp, _vp, _vp, _vp, _vp]),
+        "hsi_detect": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, DetectionParams, _vp, _vp, _vp]),
+        "hsi_narrow_to_half": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+        "hsi_transpose_bip_to_bsq": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t,
+                                               _vp]),
+        "hsi_rx_threshold": (C.c_double, [C.c_int, C.c_double]),
+        "hsi_synth_default": (None, [C.POINTER(SynthConfig)]),
+        "hsi_synth_library": (C.c_int, [C.POINTER(SynthConfig), _vp]),
+        "hsi_synth_anomaly": (C.c_int, [C.POINTER(SynthConfig), _vp]),
+        "hsi_synth_frame": (C.c_int, [C.POINTER(SynthConfig), C.c_uint64, _vp, C.c_size_t, _vp,
+                                      C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def status_name(status):
+    return lib().hsi_status_name(status).decode()
+
+
+def plane_stride(height, width):
+    return int(lib().hsi_plane_stride(height, width))
+
+
+def rx_threshold(bands, rate):
+    v = lib().hsi_rx_threshold(bands, rate)
+    if v != v:
+        raise ValueError("rx_threshold: bands must be positive and rate in (0, 1)")
+    return v
+
+
+# ---- synthetic frames ---------------------------------------------------------
+
+def synth_config(height=256, width=256, bands=113, num_targets=4, **kw):
+    cfg = SynthConfig()
+    lib().hsi_synth_default(C.byref(cfg))
+    cfg.height, cfg.width, cfg.bands, cfg.num_targets = height, width, bands, num_targets
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def synth_library(cfg):
+    out = np.empty((cfg.num_targets, cfg.bands), np.float32)
+    if lib().hsi_synth_library(C.byref(cfg), out.ctypes.data) != 0:
+        raise ValueError("bad synthetic configuration")
+    return out
+
+
+def synth_anomaly(cfg):
+    out = np.empty(cfg.bands, np.float32)
+    if lib().hsi_synth_anomaly(C.byref(cfg), out.ctypes.data) != 0:
+        raise ValueError("bad synthetic configuration")
+    return out
+
+
+def synth_frame(cfg, frame=0, stride=None, truth=False, threads=0, out=None):
+    """BSQ float32 frame as a (bands, stride) array; optionally the truth mask."""
+    pixels = cfg.height * cfg.width
+    stride = plane_stride(cfg.height, cfg.width) if stride is None else stride
+    cube = np.empty((cfg.bands, stride), np.float32) if out is None else out
+    mask = np.empty(pixels, np.uint8) if truth else None
+    rc = lib().hsi_synth_frame(C.byref(cfg), frame, cube.ctypes.data, stride,
+                               None if mask is None else mask.ctypes.data, threads)
+    if rc != 0:
+        raise ValueError("bad synthetic configuration")
+    return (cube, mask) if truth else cube
+
+
+# ---- device helpers -------------------------------------------------------------
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: the MI355X library has no CPU path")
+    return torch
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+    return C.c_void_p(getattr(stream, "cuda_stream", stream))
+
+
+def to_device_cube(cube, height, width, dtype="f32", device="cuda"):
+    """Host BSQ cube (bands, H, W) or (bands, >=H*W) -> device (bands, stride) tensor."""
+    torch = _torch()
+    cube = np.asarray(cube)
+    bands = cube.shape[0]
+    flat = cube.reshape(bands, -1)
+    pixels = height * width
+    stride = plane_stride(height, width)
+    host = np.zeros((bands, stride), np.float32)
+    host[:, :pixels] = flat[:, :pixels]
+    t = torch.from_numpy(host).to(device)
+    if dtype == "f16":
+        t = t.half()
+    return t
+
+
+class Context:
+    """hsi_ctx: a device, a target library and scene statistics."""
+
+    def __init__(self, device=0):
+        _torch()
+        self._lib = lib()
+        h = C.c_void_p()
+        st = self._lib.hsi_ctx_create(device, C.byref(h))
+        if st != OK:
+            raise HsiError(st, f"cannot create a context on device {device}")
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            self._lib.hsi_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != OK:
+            raise HsiError(st, self._lib.hsi_ctx_last_error(self.handle).decode())
+
+    def reserve(self, max_pixels, max_bands):
+        self._check(self._lib.hsi_ctx_reserve(self.handle, max_pixels, max_bands))
+
+    def set_library(self, values, norms=None):
+        values = np.ascontiguousarray(values, np.float32)
+        t, b = values.shape
+        nptr = None
+        if norms is not None:
+            norms = np.ascontiguousarray(norms, np.float32)
+            nptr = norms.ctypes.data
+        self._check(self._lib.hsi_set_library(self.handle, values.ctypes.data, nptr, t, b))
+        self.num_targets, self.bands = t, b
+
+    # statistics
+    def stats_accumulate(self, cube, height, width, bands, precision=PLAIN, stream=None):
+        self._check(self._lib.hsi_stats_accumulate(self.handle, _ptr(cube), height, width, bands,
+                                                   cube.stride(0), precision, _stream(stream)))
+
+    def stats_build(self, loading, stream=None):
+        self._check(self._lib.hsi_stats_build(self.handle, loading, _stream(stream)))
+
+    def stats_check(self, stream=None):
+        self._check(self._lib.hsi_stats_check(self.handle, _stream(stream)))
+
+    def stats_read(self, bands, num_targets):
+        out = {
+            "mean": np.zeros(bands, np.float32),
+            "whitener": np.zeros((bands, bands), np.float32),
+            "target_white": np.zeros((max(num_targets, 1), bands), np.float32),
+            "target_norm2": np.zeros(max(num_targets, 1), np.float32),
+            "cem_weight": np.zeros((max(num_targets, 1), bands), np.float32),
+            "count": np.zeros(1, np.float64),
+            "sum": np.zeros(bands, np.float64),
+            "sum_outer": np.zeros((bands, bands), np.float64),
+        }
+        self._check(self._lib.hsi_stats_read(self.handle, *[out[k].ctypes.data for k in (
+            "mean", "whitener", "target_white", "target_norm2", "cem_weight", "count", "sum",
+            "sum_outer")]))
+        return out
+
+    def stats_upload(self, bands, mean, whitener, target_white=None, target_norm2=None,
+                     cem_weight=None):
+        def p(a):
+            return None if a is None else np.ascontiguousarray(a, np.float32)
+        arrs = [p(mean), p(whitener), p(target_white), p(target_norm2), p(cem_weight)]
+        self._keep = arrs
+        self._check(self._lib.hsi_stats_upload(
+            self.handle, bands, *[None if a is None else a.ctypes.data for a in arrs]))
+
+    # scoring
+    def score(self, detector, cube, height, width, bands, score, target=None, aux=None,
+              precision=PLAIN, stream=None):
+        dtype = F16 if str(cube.dtype) == "torch.float16" else F32
+        self._check(self._lib.hsi_score(self.handle, int(detector), precision, _ptr(cube), dtype,
+                                        height, width, bands, cube.stride(0), _ptr(score),
+                                        _ptr(target), _ptr(aux), _stream(stream)))
+
+    def score_sam_cem(self, cube, height, width, bands, sam_score=None, sam_target=None,
+                      cem_score=None, cem_target=None, stream=None):
+        dtype = F16 if str(cube.dtype) == "torch.float16" else F32
+        self._check(self._lib.hsi_score_sam_cem(self.handle, _ptr(cube), dtype, height, width,
+                                                bands, cube.stride(0), _ptr(sam_score),
+                                                _ptr(sam_target), _ptr(cem_score),
+                                                _ptr(cem_target), _stream(stream)))
+
+    def detect(self, score, target, height, width, params, out, count, stream=None):
+        self._check(self._lib.hsi_detect(self.handle, _ptr(score), _ptr(target), height, width,
+                                         params, _ptr(out), _ptr(count), _stream(stream)))
+
+
+def detections_to_numpy(out_tensor, count):
+    """Device detection buffer (int32 [cap, 4]) -> structured array of `count` rows."""
+    raw = out_tensor[:count].cpu().numpy()
+    return raw.view(np.uint8).reshape(-1, 16).copy().view(DETECTION_DTYPE).reshape(-1)

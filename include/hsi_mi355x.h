@@ -1,0 +1,242 @@
+/*
+ * hsi_mi355x.h - C ABI of the MI355X (gfx950) four-detector frame-scoring
+ * library: SAM, CEM, RX and ACE, their scene statistics, and
+ * threshold-and-compact.
+ *
+ * This is the drop-in boundary for the reference's device layer. It replaces
+ * the C++ launchers in include/hsi/sam_cuda.hpp, stats_cuda.hpp and
+ * detect_cuda.hpp, whose signatures carry cudaError_t/cudaStream_t, with
+ * plain pointers, sizes and status codes. Each declaration names the
+ * reference interface it stands in for.
+ *
+ * Conventions (kept from the reference, include/hsi/sam_cuda.hpp:59-96):
+ *  - Device buffers (cubes, score maps, detection lists, counters) belong to
+ *    the caller. The context owns only what it derives: the target library
+ *    in device layouts, the scene statistics, and reusable scratch.
+ *  - Launch functions are asynchronous on the caller's stream (`hsi_stream`
+ *    is a hipStream_t; NULL is the default stream). Functions documented as
+ *    synchronous block the host.
+ *  - Cubes are band-sequential (BSQ) with a padded plane stride: element
+ *    (band b, pixel p) is at cube[b * plane_stride + p], pixel p = y*W + x.
+ *  - Errors come back as hsi_status; hsi_ctx_last_error() holds the message.
+ *
+ * A context is single-threaded: calls on one context must not race.
+ */
+#ifndef HSI_MI355X_H
+#define HSI_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference launchers return cudaErrorInvalidValue for
+ * bad shapes, null pointers and an oversize library (sam.cu:347-353,
+ * 371-376; detect.cu:91-97); build_statistics returns false with a message
+ * naming --loading (detector.cpp:72-77, 97-100). */
+typedef enum hsi_status {
+  HSI_OK = 0,
+  HSI_ERR_INVALID_ARG = 1,
+  HSI_ERR_CAPACITY = 2,                /* library larger than supported */
+  HSI_ERR_NOT_POSITIVE_DEFINITE = 3,   /* statistics failed; raise loading */
+  HSI_ERR_DEVICE = 4,                  /* HIP runtime error */
+  HSI_ERR_NO_STATISTICS = 5,           /* RX/ACE/CEM before statistics exist */
+  HSI_ERR_NO_LIBRARY = 6               /* SAM/ACE/CEM before a library is set */
+} hsi_status;
+
+/* Detectors, in the order of hsi::Detector (include/hsi/detector.hpp:23-42). */
+typedef enum hsi_detector {
+  HSI_DET_SAM = 0,
+  HSI_DET_RX = 1,
+  HSI_DET_ACE = 2,
+  HSI_DET_CEM = 3
+} hsi_detector;
+
+/* Element type of a device cube. F16 replaces the reference's SamVariant::Half
+ * cube (sam_cuda.hpp:31, sam.cu:210-274). */
+typedef enum hsi_dtype { HSI_F32 = 0, HSI_F16 = 1 } hsi_dtype;
+
+/* Arithmetic form. PLAIN accumulates in FP64 (the oracle's precision) on an
+ * FP32 or FP16 cube. MIXED runs the RX/ACE whitening and the covariance on
+ * FP16 matrix cores with FP32 accumulation and FP64 cross-tile reduction. */
+typedef enum hsi_precision { HSI_PLAIN = 0, HSI_MIXED = 1 } hsi_precision;
+
+/* 16-byte detection record, layout-identical to hsi::Detection
+ * (include/hsi/types.hpp:101-106). */
+typedef struct hsi_detection {
+  int32_t x;
+  int32_t y;
+  int32_t target;
+  float score;
+} hsi_detection;
+
+/* hsi::DetectionParams (include/hsi/types.hpp:122-140) with the polarity as
+ * an int: 0 = lower is better (SAM), 1 = higher is better (RX, ACE, CEM). */
+typedef struct hsi_detection_params {
+  float threshold;
+  int32_t nms_radius;
+  int32_t max_detections;
+  int32_t higher_is_better;
+} hsi_detection_params;
+
+typedef struct hsi_ctx hsi_ctx;
+typedef void* hsi_stream;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Create a context on HIP device `device`. Replaces the process-global
+ * constant-memory library of the reference (sam.cu:22-23): every context
+ * holds its own library and statistics. */
+hsi_status hsi_ctx_create(int device, hsi_ctx** out);
+void hsi_ctx_destroy(hsi_ctx* ctx);
+/* Message for the last failing call on this context ("" if none). */
+const char* hsi_ctx_last_error(const hsi_ctx* ctx);
+const char* hsi_status_name(hsi_status status);
+
+/* Pre-size the context's scratch for frames up to max_pixels x max_bands so
+ * that later launches never allocate (needed before stream capture).
+ * Synchronous. */
+hsi_status hsi_ctx_reserve(hsi_ctx* ctx, size_t max_pixels, int max_bands);
+
+/* ---- geometry ----------------------------------------------------------- */
+
+/* Plane stride in elements of a device BSQ cube: pixels rounded up to a
+ * multiple of 8, so every band plane starts 16-byte aligned for FP32 and
+ * FP16 alike. Replaces bsq_plane_stride (sam_cuda.hpp:41-52, sam.cu:338-343). */
+size_t hsi_plane_stride(int height, int width);
+
+/* ---- target library ----------------------------------------------------- */
+
+/* Largest library a context accepts (targets), and the largest band count. */
+int hsi_max_targets(void);
+int hsi_max_bands(void);
+
+/* Install a target library: `values` is target-major [num_targets x bands]
+ * host memory and `norms` the per-target L2 norms (float, as
+ * SpectralLibrary::norms computes them, types.cpp:68-76), or NULL to have
+ * them computed the same way. Synchronous. Replaces upload_targets
+ * (sam_cuda.hpp:59-70, sam.cu:347-369), without its global state and with
+ * no reciprocal-norm shortcut. */
+hsi_status hsi_set_library(hsi_ctx* ctx, const float* values, const float* norms,
+                           int num_targets, int bands);
+
+/* ---- scene statistics --------------------------------------------------- */
+
+/* Accumulate the scene's raw first and second moments over a device cube
+ * (count, sum[B], lower triangle of sum x x^T) in FP64, replacing any
+ * previous accumulation. PLAIN forms each product exactly in FP64; MIXED
+ * shifts by the frame mean, forms products on FP16 matrix cores from a
+ * hi/lo split with FP32 tile accumulators, and reduces tiles in FP64.
+ * Replaces launch_reset_moments + launch_accumulate_mean +
+ * launch_accumulate_moments (stats_cuda.hpp:141-165, stats.cu:174-214) and
+ * SpectralMoments::add over a frame (detectors_cpu.cpp:77-93). */
+hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const float* d_cube, int height,
+                                int width, int bands, size_t plane_stride,
+                                hsi_precision precision, hsi_stream stream);
+
+/* Derive DetectorStatistics on the device in FP64 from the accumulated
+ * moments and the installed library: covariance and autocorrelation with
+ * diagonal loading, their Cholesky factors, the whitener L^-1, whitened
+ * centred targets and their squared norms, and the CEM filters
+ * R^-1 d / (d^T R^-1 d); narrowed to float as the reference does. A failed
+ * factorisation is recorded in a device status word (read it with
+ * hsi_stats_check). Replaces build_statistics (detector.cpp:47-152),
+ * SpectralMoments::mean/covariance/correlation (linalg.cpp:45-79) and
+ * cholesky/triangular_inverse_lower (linalg.cpp:105-162). Asynchronous. */
+hsi_status hsi_stats_build(hsi_ctx* ctx, double diagonal_loading, hsi_stream stream);
+
+/* Synchronous: wait for `stream`, then report HSI_OK or
+ * HSI_ERR_NOT_POSITIVE_DEFINITE with the reference's message (which names
+ * --loading) in hsi_ctx_last_error. */
+hsi_status hsi_stats_check(hsi_ctx* ctx, hsi_stream stream);
+
+/* Synchronous copy of the statistics to host memory; any pointer may be
+ * NULL. Sizes: mean[B], whitener[B*B] (row-major, upper triangle zero),
+ * target_white[T*B], target_norm2[T], cem_weight[T*B], sum[B],
+ * sum_outer[B*B] (lower triangle). Replaces read_moments
+ * (stats_cuda.hpp:167-173) and the DetectorStatistics fields
+ * (detector.hpp:62-85). */
+hsi_status hsi_stats_read(hsi_ctx* ctx, float* mean, float* whitener,
+                          float* target_white, float* target_norm2,
+                          float* cem_weight, double* count, double* sum,
+                          double* sum_outer);
+
+/* Synchronous: install statistics computed elsewhere (host memory, same
+ * layouts as hsi_stats_read), e.g. a previous frame's, or the CPU scorer's
+ * for a parity check. target_white/target_norm2/cem_weight may be NULL when
+ * the library is empty. */
+hsi_status hsi_stats_upload(hsi_ctx* ctx, int bands, const float* mean,
+                            const float* whitener, const float* target_white,
+                            const float* target_norm2, const float* cem_weight);
+
+/* ---- scoring -------------------------------------------------------------- */
+
+/* Score every pixel of a device cube with one detector.
+ *   SAM: d_score = smallest spectral angle over the library, d_target = its
+ *        index (first wins a tie). Replaces launch_sam_best
+ *        (sam_cuda.hpp:72-96) and matches sam_best_cpu (sam_cpu.cpp:48-77).
+ *   RX:  d_score = ||L^-1 (x - mean)||^2 (rx_cpu, detectors_cpu.cpp:95-113);
+ *        d_target ignored.
+ *   ACE: d_score = best (z.u)^2/(uu zz) clamped to [0,1], d_target its index
+ *        or -1, d_aux (nullable) = the RX statistic from the same whitening
+ *        (ace_best_cpu, detectors_cpu.cpp:115-156).
+ *   CEM: d_score = best w_t . x, d_target its index (cem_best_cpu,
+ *        detectors_cpu.cpp:158-187).
+ * d_target may be NULL. PLAIN accumulates in FP64 and reads an F32 or F16
+ * cube; MIXED (RX/ACE only) whitens on FP16 matrix cores. */
+hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precision,
+                     const void* d_cube, hsi_dtype dtype, int height, int width,
+                     int bands, size_t plane_stride, float* d_score,
+                     int32_t* d_target, float* d_aux, hsi_stream stream);
+
+/* SAM and CEM from one read of the cube: the two bandwidth-bound detectors
+ * share their pixel loads. Outputs as hsi_score for each; any of the four
+ * output pointers may be NULL (a NULL score map skips that detector). */
+hsi_status hsi_score_sam_cem(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype,
+                             int height, int width, int bands, size_t plane_stride,
+                             float* d_sam_score, int32_t* d_sam_target,
+                             float* d_cem_score, int32_t* d_cem_target,
+                             hsi_stream stream);
+
+/* ---- threshold and compact ----------------------------------------------- */
+
+/* Threshold a score map, suppress non-maxima within nms_radius (a neighbour
+ * strictly better, or equal with a smaller linear index, suppresses), and
+ * compact survivors into d_out in raster order, as detect_cpu does
+ * (sam_cpu.cpp:86-138) - including polarity, which the reference kernel
+ * ignores (detect.cu:43-87). *d_count receives the number found, which may
+ * exceed max_detections; only the first max_detections are written. The
+ * count is overwritten, so it need not be pre-zeroed. d_target may be NULL
+ * (target 0). Replaces launch_detect (detect_cuda.hpp:16-19). */
+hsi_status hsi_detect(hsi_ctx* ctx, const float* d_score, const int32_t* d_target,
+                      int height, int width, hsi_detection_params params,
+                      hsi_detection* d_out, uint32_t* d_count, hsi_stream stream);
+
+/* ---- ingest helpers ---------------------------------------------------------- */
+
+/* FP32 -> FP16 narrowing of `count` elements (round to nearest even).
+ * Replaces launch_narrow_to_half (sam_cuda.hpp:107-109, sam.cu:311-316). */
+hsi_status hsi_narrow_to_half(const float* d_src, void* d_dst, size_t count,
+                              hsi_stream stream);
+
+/* BIP -> padded BSQ transpose on the device. Replaces
+ * launch_transpose_bip_to_bsq (sam_cuda.hpp:98-105, sam.cu:281-309). */
+hsi_status hsi_transpose_bip_to_bsq(const float* d_src, float* d_dst, int height,
+                                    int width, int bands, size_t plane_stride,
+                                    hsi_stream stream);
+
+/* ---- host helpers ------------------------------------------------------------ */
+
+/* RX threshold for a per-pixel false-alarm rate under the chi-squared
+ * approximation with `bands` degrees of freedom; NaN for invalid arguments
+ * (the reference throws). Replaces rx_threshold_for_false_alarm_rate
+ * (detectors.hpp:56-62, detectors_cpu.cpp:189-206). */
+double hsi_rx_threshold(int bands, double false_alarm_rate);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HSI_MI355X_H */

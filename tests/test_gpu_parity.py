@@ -1,0 +1,285 @@
+"""GPU parity against the golden fixtures (written by the compiled reference)
+and the CPU oracle. Every call goes through the C ABI.
+
+Bars (the reference-scorer parity the path is held to):
+  - SAM angles within 1e-6 rad of the oracle; winning target identical
+    unless the two best angles are within 1e-6 rad of each other.
+  - CEM given the reference's own statistics: bit-identical (the kernel forms
+    the same exact double products and sums them in the same band order).
+  - RX / ACE scores within 1e-5 relative (floor 1 for the unit-scaled ACE);
+    ACE target identical unless the two best scores are within that band.
+  - Moments within 1e-12 relative (exact products, different sum order);
+    statistics built on the device within a few float ulps.
+  - Detection lists from the same score map: identical, element by element.
+"""
+import numpy as np
+import pytest
+
+from conftest import SCENES, golden
+from gpu_util import ANGLE_TOL, FLOOR, REL, device_buffers, rel_err, same_bits, upload_golden_stats
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def H():
+    import torch
+    import hsi_mi355x
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return hsi_mi355x
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    return t
+
+
+@pytest.fixture()
+def ctx(H):
+    c = H.Context(0)
+    yield c
+    c.close()
+
+
+def load_scene(H, torch, g, dtype="f32"):
+    b, h, w = g["cube"].shape
+    return H.to_device_cube(g["cube"], h, w, dtype=dtype), b, h, w
+
+
+def sam_bars(got_s, got_t, g, pixels):
+    want_s, want_t = g["sam_score"], g["sam_target"]
+    err = np.abs(got_s.astype(np.float64) - want_s.astype(np.float64))
+    assert err.max() <= ANGLE_TOL, f"max angle error {err.max()}"
+    diff = np.nonzero(got_t != want_t)[0]
+    if diff.size:
+        allv = g["sam_all"]
+        for p in diff:
+            gap = abs(float(allv[got_t[p], p]) - float(want_s[p]))
+            assert gap <= ANGLE_TOL, f"pixel {p}: target {got_t[p]} vs {want_t[p]}, gap {gap}"
+    return float(err.max()), int((got_s.view(np.uint32) == want_s.view(np.uint32)).sum())
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_sam_matches_reference(H, torch, ctx, name):
+    g = golden(name)
+    cube, b, h, w = load_scene(H, torch, g)
+    p = h * w
+    ctx.set_library(g["library"], g["norms"])
+    buf = device_buffers(torch, p)
+    ctx.score(H.Detector.SAM, cube, h, w, b, buf["score"], buf["target"])
+    torch.cuda.synchronize()
+    s = buf["score"].cpu().numpy()
+    t = buf["target"].cpu().numpy()
+    maxerr, exact = sam_bars(s, t, g, p)
+    print(f"{name}: SAM max |d angle| {maxerr:.3g} rad, bit-identical {exact}/{p}")
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_sam_fp16_cube_matches_oracle_on_the_same_halves(H, torch, ctx, oracle_lib, name):
+    g = golden(name)
+    b, h, w = g["cube"].shape
+    halves = g["cube"].astype(np.float16).astype(np.float32)
+    cube = H.to_device_cube(halves, h, w, dtype="f16")
+    ctx.set_library(g["library"], g["norms"])
+    buf = device_buffers(torch, h * w)
+    ctx.score(H.Detector.SAM, cube, h, w, b, buf["score"], buf["target"])
+    torch.cuda.synchronize()
+    want_s, want_t = oracle_lib.sam_best(halves, g["library"], g["norms"])
+    got = buf["score"].cpu().numpy()
+    assert np.abs(got.astype(np.float64) - want_s).max() <= ANGLE_TOL
+    assert (buf["target"].cpu().numpy() == want_t).mean() > 0.999
+
+
+def test_sam_edge_cases(H, torch, ctx):
+    g = golden("edge_cases")
+    cube = H.to_device_cube(g["edge_cube"], 1, 8)
+    ctx.set_library(g["edge_library"], g["edge_norms"])
+    buf = device_buffers(torch, 8)
+    ctx.score(H.Detector.SAM, cube, 1, 8, 6, buf["score"], buf["target"])
+    torch.cuda.synchronize()
+    s = buf["score"].cpu().numpy()
+    t = buf["target"].cpu().numpy()
+    want_s, want_t = g["edge_sam_score"], g["edge_sam_target"]
+    assert np.abs(s.astype(np.float64) - want_s).max() <= ANGLE_TOL
+    assert np.array_equal(t, want_t)  # zero pixel -> target 0, duplicate -> first, zero-norm target skipped
+    assert not np.isnan(s).any()
+    assert s[0] == np.float32(np.pi / 2)
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_cem_with_reference_statistics_is_bit_identical(H, torch, ctx, name):
+    g = golden(name)
+    cube, b, h, w = load_scene(H, torch, g)
+    ctx.set_library(g["library"], g["norms"])
+    upload_golden_stats(ctx, g)
+    buf = device_buffers(torch, h * w)
+    ctx.score(H.Detector.CEM, cube, h, w, b, buf["score"], buf["target"])
+    torch.cuda.synchronize()
+    assert same_bits(buf["score"].cpu().numpy(), g["cem_score"])
+    assert np.array_equal(buf["target"].cpu().numpy(), g["cem_target"])
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_fused_sam_cem_equals_separate_launches(H, torch, ctx, name):
+    g = golden(name)
+    cube, b, h, w = load_scene(H, torch, g)
+    ctx.set_library(g["library"], g["norms"])
+    upload_golden_stats(ctx, g)
+    p = h * w
+    a = device_buffers(torch, p)
+    c = device_buffers(torch, p)
+    ctx.score_sam_cem(cube, h, w, b, a["score"], a["target"], c["score"], c["target"])
+    s = device_buffers(torch, p)
+    e = device_buffers(torch, p)
+    ctx.score(H.Detector.SAM, cube, h, w, b, s["score"], s["target"])
+    ctx.score(H.Detector.CEM, cube, h, w, b, e["score"], e["target"])
+    torch.cuda.synchronize()
+    for x, y in ((a, s), (c, e)):
+        assert same_bits(x["score"].cpu().numpy(), y["score"].cpu().numpy())
+        assert np.array_equal(x["target"].cpu().numpy(), y["target"].cpu().numpy())
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_rx_ace_with_reference_statistics(H, torch, ctx, name):
+    g = golden(name)
+    cube, b, h, w = load_scene(H, torch, g)
+    ctx.set_library(g["library"], g["norms"])
+    upload_golden_stats(ctx, g)
+    p = h * w
+    rx = device_buffers(torch, p)
+    ctx.score(H.Detector.RX, cube, h, w, b, rx["score"])
+    ace = device_buffers(torch, p)
+    ctx.score(H.Detector.ACE, cube, h, w, b, ace["score"], ace["target"], ace["aux"])
+    torch.cuda.synchronize()
+    e_rx = rel_err(rx["score"].cpu().numpy(), g["rx_score"], FLOOR["rx"])
+    e_aux = rel_err(ace["aux"].cpu().numpy(), g["ace_rx"], FLOOR["rx"])
+    e_ace = rel_err(ace["score"].cpu().numpy(), g["ace_score"], FLOOR["ace"])
+    print(f"{name}: RX max rel {e_rx.max():.3g}, ACE max {e_ace.max():.3g}")
+    assert e_rx.max() <= REL and e_aux.max() <= REL and e_ace.max() <= REL
+    got_t = ace["target"].cpu().numpy()
+    bad = np.nonzero(got_t != g["ace_target"])[0]
+    assert bad.size <= max(1, p // 10000), f"{bad.size} ACE target mismatches"
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_moments_and_statistics_built_on_device(H, torch, ctx, oracle_lib, name):
+    g = golden(name)
+    cube, b, h, w = load_scene(H, torch, g)
+    t = g["library"].shape[0]
+    ctx.set_library(g["library"], g["norms"])
+    ctx.stats_accumulate(cube, h, w, b)
+    ctx.stats_build(float(g["loading"]))
+    ctx.stats_check()
+    st = ctx.stats_read(b, t)
+    assert st["count"][0] == g["m_count"]
+    lower = np.tril_indices(b)
+    assert rel_err(st["sum"], g["m_sum"], 1e-30).max() <= 1e-12
+    assert rel_err(st["sum_outer"][lower], g["m_outer"][lower], 1e-30).max() <= 1e-12
+    # Statistics from the device moments vs the reference's: float outputs
+    # agree to a few ulps (the double internals differ only by sum order).
+    for key, want, floor in (("mean", g["st_mean"], 1e-30),
+                             ("whitener", g["st_whitener"], None),
+                             ("target_white", g["st_target_white"], None),
+                             ("target_norm2", g["st_target_norm2"], 1e-30),
+                             ("cem_weight", g["st_cem_weight"], None)):
+        fl = floor if floor is not None else float(np.abs(want).max()) * 1e-3
+        e = rel_err(st[key], want, fl).max()
+        print(f"{name}: {key} max rel err {e:.3g}")
+        assert e <= 1e-5, key
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_end_to_end_scores_with_device_statistics(H, torch, ctx, name):
+    g = golden(name)
+    cube, b, h, w = load_scene(H, torch, g)
+    p = h * w
+    ctx.set_library(g["library"], g["norms"])
+    ctx.stats_accumulate(cube, h, w, b)
+    ctx.stats_build(float(g["loading"]))
+    ctx.stats_check()
+    cem = device_buffers(torch, p)
+    ctx.score(H.Detector.CEM, cube, h, w, b, cem["score"], cem["target"])
+    rx = device_buffers(torch, p)
+    ctx.score(H.Detector.RX, cube, h, w, b, rx["score"])
+    ace = device_buffers(torch, p)
+    ctx.score(H.Detector.ACE, cube, h, w, b, ace["score"], ace["target"], ace["aux"])
+    torch.cuda.synchronize()
+    for key, got in (("cem", cem["score"]), ("rx", rx["score"]), ("ace", ace["score"])):
+        e = rel_err(got.cpu().numpy(), g[f"{key}_score"], FLOOR[key])
+        print(f"{name}: end-to-end {key} max rel {e.max():.3g}")
+        assert e.max() <= REL, key
+
+
+def test_singular_covariance_is_reported_with_loading(H, torch, ctx):
+    g = golden("edge_cases")
+    cube = H.to_device_cube(g["sing_cube"], 10, 10)
+    ctx.stats_accumulate(cube, 10, 10, 4)
+    ctx.stats_build(0.0)
+    with pytest.raises(H.HsiError) as info:
+        ctx.stats_check()
+    assert info.value.status == H.ERR_NOT_POSITIVE_DEFINITE
+    assert "loading" in str(info.value)
+    ctx.stats_build(1e-3)
+    ctx.stats_check()
+    st = ctx.stats_read(4, 0)
+    want = g["sing_l3_whitener"]
+    assert rel_err(st["whitener"], want, float(np.abs(want).max()) * 1e-3).max() <= 1e-5
+
+
+def run_detect(H, torch, ctx, score, target, h, w, thr, nms, higher, cap=None):
+    p = h * w
+    cap = p if cap is None else cap
+    s = torch.from_numpy(np.ascontiguousarray(score, np.float32).reshape(-1)).cuda()
+    t = None if target is None else torch.from_numpy(np.ascontiguousarray(target, np.int32).reshape(-1)).cuda()
+    out = torch.zeros((max(cap, 1), 4), dtype=torch.int32, device="cuda")
+    count = torch.full((1,), 12345, dtype=torch.int32, device="cuda")
+    ctx.detect(s, t, h, w, H.DetectionParams(thr, nms, cap, higher), out, count)
+    torch.cuda.synchronize()
+    found = int(count.item())
+    return H.detections_to_numpy(out, min(found, cap)), found
+
+
+@pytest.mark.parametrize("name", SCENES)
+@pytest.mark.parametrize("det", ["sam", "rx", "ace", "cem"])
+@pytest.mark.parametrize("nms", [0, 2])
+def test_detect_matches_reference_lists(H, torch, ctx, name, det, nms):
+    g = golden(name)
+    _, h, w = g["cube"].shape
+    target = None if det == "rx" else g[f"{det}_target"]
+    dets, found = run_detect(H, torch, ctx, g[f"{det}_score"], target, h, w,
+                             float(g[f"{det}_threshold"]), nms, det != "sam")
+    want = g[f"{det}_det{nms}_xyt"]
+    assert found == want.shape[0]
+    assert np.array_equal(np.stack([dets["x"], dets["y"], dets["target"]], 1).reshape(-1, 3), want)
+    assert same_bits(dets["score"], g[f"{det}_det{nms}_score"])
+
+
+@pytest.mark.parametrize("case", ["blob", "plateau", "pol_low", "pol_high", "extremum",
+                                  "ties_low", "ties_high"])
+def test_detect_edge_maps(H, torch, ctx, case):
+    g = golden("edge_cases")
+    mp = g[f"det_{case}_map"]
+    thr, nms, higher = g[f"det_{case}_args"]
+    h, w = mp.shape
+    tg = (np.arange(h * w) % 3).astype(np.int32)
+    dets, found = run_detect(H, torch, ctx, mp, tg, h, w, float(thr), int(nms), bool(higher))
+    want = g[f"det_{case}_xyt"]
+    assert found == want.shape[0]
+    assert np.array_equal(np.stack([dets["x"], dets["y"], dets["target"]], 1).reshape(-1, 3), want)
+
+
+def test_detect_saturates_without_overrun(H, torch, ctx, oracle_lib):
+    rng = np.random.default_rng(3)
+    h, w = 300, 500
+    score = rng.random((h, w), dtype=np.float32)
+    target = rng.integers(0, 4, size=h * w).astype(np.int32)
+    want, found_cpu = oracle_lib.detect(score, target, h, w, 0.5, 0, False)
+    cap = 1000
+    dets, found = run_detect(H, torch, ctx, score, target, h, w, 0.5, 0, False, cap=cap)
+    assert found == found_cpu > cap
+    assert len(dets) == cap
+    assert np.array_equal(dets["x"], want["x"][:cap]) and np.array_equal(dets["y"], want["y"][:cap])
+    # an empty result: nothing passes
+    dets, found = run_detect(H, torch, ctx, score, target, h, w, -1.0, 2, False, cap=16)
+    assert found == 0
