@@ -41,6 +41,11 @@ struct hsi_ctx {
   double* d_cem_lib = nullptr;
   float* d_white_blocks = nullptr;
   float* d_ace_u = nullptr;
+  _Float16* d_white_hl = nullptr;
+  _Float16* d_ace_hl = nullptr;
+  float* d_col_scale = nullptr;
+  float* d_row_inv = nullptr;
+  float* d_target_inv = nullptr;
 
   // Scratch sized per frame.
   double* d_mom_scratch = nullptr;
@@ -140,8 +145,9 @@ void hsi_ctx_destroy(hsi_ctx* ctx) {
   void* ptrs[] = {ctx->d_targets, ctx->d_norms, ctx->d_sam_lib, ctx->d_count, ctx->d_sum,
                   ctx->d_sum_outer, ctx->d_mean, ctx->d_whitener, ctx->d_target_white,
                   ctx->d_target_norm2, ctx->d_cem_weight, ctx->d_status, ctx->d_build_work,
-                  ctx->d_cem_lib, ctx->d_white_blocks, ctx->d_ace_u, ctx->d_mom_scratch,
-                  ctx->d_det_scratch};
+                  ctx->d_cem_lib, ctx->d_white_blocks, ctx->d_ace_u, ctx->d_white_hl,
+                  ctx->d_ace_hl, ctx->d_col_scale, ctx->d_row_inv, ctx->d_target_inv,
+                  ctx->d_mom_scratch, ctx->d_det_scratch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete ctx;
@@ -173,6 +179,11 @@ hsi_status hsi_ctx_create(int device, hsi_ctx** out) {
   if (err == hipSuccess) err = alloc(&ctx->d_cem_lib, B * T);
   if (err == hipSuccess) err = alloc(&ctx->d_white_blocks, rxace_white_floats(kMaxBands));
   if (err == hipSuccess) err = alloc(&ctx->d_ace_u, rxace_target_floats(kMaxBands, kMaxTargets));
+  if (err == hipSuccess) err = alloc(&ctx->d_white_hl, mixed_white_halves(kMaxBands));
+  if (err == hipSuccess) err = alloc(&ctx->d_ace_hl, mixed_target_halves(kMaxBands, kMaxTargets));
+  if (err == hipSuccess) err = alloc(&ctx->d_col_scale, B);
+  if (err == hipSuccess) err = alloc(&ctx->d_row_inv, B);
+  if (err == hipSuccess) err = alloc(&ctx->d_target_inv, T);
   if (err == hipSuccess) err = hipMemset(ctx->d_sum_outer, 0, B * B * sizeof(double));
   if (err == hipSuccess) err = hipMemset(ctx->d_status, 0, sizeof(unsigned));
   if (err == hipSuccess) err = hipDeviceSynchronize();
@@ -278,6 +289,19 @@ static hsi_status refresh_layouts(hsi_ctx* ctx, int bands, int targets, hipStrea
   la.white_blocks = ctx->d_white_blocks;
   la.ace_u = ctx->d_ace_u;
   HSI_HIP(ctx, launch_layouts(la, stream), "derive statistics layouts");
+  if (bands <= kMaxRxBands) {
+    MixedLayoutArgs ma;
+    ma.bands = bands;
+    ma.num_targets = targets;
+    ma.whitener = ctx->d_whitener;
+    ma.target_white = ctx->d_target_white;
+    ma.white_hl = ctx->d_white_hl;
+    ma.ace_hl = ctx->d_ace_hl;
+    ma.col_scale = ctx->d_col_scale;
+    ma.row_inv = ctx->d_row_inv;
+    ma.target_inv = ctx->d_target_inv;
+    HSI_HIP(ctx, launch_mixed_layouts(ma, stream), "derive mixed-precision layouts");
+  }
   return HSI_OK;
 }
 
@@ -486,6 +510,11 @@ hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precisio
   a.white_blocks = ctx->d_white_blocks;
   a.ace_u = ctx->d_ace_u;
   a.target_norm2 = ctx->d_target_norm2;
+  a.white_hl = ctx->d_white_hl;
+  a.ace_hl = ctx->d_ace_hl;
+  a.col_scale = ctx->d_col_scale;
+  a.row_inv = ctx->d_row_inv;
+  a.target_inv = ctx->d_target_inv;
   if (detector == HSI_DET_RX) {
     a.rx = d_score;
   } else {
@@ -495,8 +524,6 @@ hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precisio
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   const hipError_t err = precision == HSI_PLAIN ? launch_rxace_plain(a, s) : launch_rxace_mixed(a, s);
-  if (err == hipErrorNotSupported)
-    return fail(ctx, HSI_ERR_INVALID_ARG, "mixed precision RX/ACE is not available in this build");
   HSI_HIP(ctx, err, "score RX/ACE");
   return HSI_OK;
 }

@@ -77,20 +77,30 @@ struct LayoutArgs {
   double* cem_lib = nullptr;        // [bands][lib_stride]
   float* white_blocks = nullptr;    // rxace_white_floats(bands)
   float* ace_u = nullptr;           // rxace_target_floats(bands, T)
-  _Float16* white_hi = nullptr;     // rxace_white_halves(bands)
-  _Float16* white_lo = nullptr;
-  _Float16* ace_u_h = nullptr;      // rxace_target_halves(bands, T)
-  float* white_scale = nullptr;     // per 16-row block scale of the fp16 whitener
-  double* white_mean = nullptr;     // W * mean, [nb*16]
 };
 hipError_t launch_layouts(const LayoutArgs& args, hipStream_t stream);
+
+// Layouts of the mixed-precision RX/ACE kernel: the whitener and whitened
+// targets rescaled by powers of two and split into FP16 hi + lo parts.
+struct MixedLayoutArgs {
+  int bands = 0;
+  int num_targets = 0;
+  const float* whitener = nullptr;
+  const float* target_white = nullptr;
+  _Float16* white_hl = nullptr;   // mixed_white_halves(bands)
+  _Float16* ace_hl = nullptr;     // mixed_target_halves(bands, T)
+  float* col_scale = nullptr;     // [kMaxBands] y' = y * col_scale
+  float* row_inv = nullptr;       // [kMaxBands] z = z'' * row_inv
+  float* target_inv = nullptr;    // [kMaxTargets] z.u = (z.u)'' * target_inv
+};
+std::size_t mixed_white_halves(int bands);
+std::size_t mixed_target_halves(int bands, int num_targets);
+hipError_t launch_mixed_layouts(const MixedLayoutArgs& args, hipStream_t stream);
 
 // ---- RX / ACE ----------------------------------------------------------------
 __host__ __device__ inline int rx_blocks(int bands) { return (bands + 15) / 16; }
 std::size_t rxace_white_floats(int bands);
 std::size_t rxace_target_floats(int bands, int num_targets);
-std::size_t rxace_white_halves(int bands);
-std::size_t rxace_target_halves(int bands, int num_targets);
 
 struct RxAceArgs {
   const void* cube = nullptr;
@@ -103,11 +113,11 @@ struct RxAceArgs {
   const float* white_blocks = nullptr;
   const float* ace_u = nullptr;
   const float* target_norm2 = nullptr;
-  const _Float16* white_hi = nullptr;
-  const _Float16* white_lo = nullptr;
-  const _Float16* ace_u_h = nullptr;
-  const float* white_scale = nullptr;
-  const double* white_mean = nullptr;
+  const _Float16* white_hl = nullptr;
+  const _Float16* ace_hl = nullptr;
+  const float* col_scale = nullptr;
+  const float* row_inv = nullptr;
+  const float* target_inv = nullptr;
   float* rx = nullptr;
   float* ace_score = nullptr;
   int* ace_target = nullptr;

@@ -9,8 +9,10 @@
 //   RX:  sum of squares down the block's rows (||z||^2), and
 //   ACE: z . u_t for up to 32 targets, as four more MFMAs per target tile
 //        that consume the Z_I accumulator registers directly as the B
-//        operand: in K-step r lane l holds row 4*(l/16)+r of Z_I, so the
-//        target tile's A operand is laid out with the same permutation.
+//        operand: v_mfma_f64_16x16x4 leaves row l/16 + 4r of Z_I in lane l,
+//        register r, so in K-step r lane l holds exactly the B element
+//        (k = l/16) of rows {r, 4+r, 8+r, 12+r}, and the target tile's A
+//        operand is laid out with the same permutation.
 // The whitener blocks and the target tile live in LDS as floats (the
 // reference narrows them to float: detector.cpp:83-84) and are widened per
 // MFMA; each wave keeps its pixel tile's centred spectrum in registers in the
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
     for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int t = 16 * tt + 4 * g + r;
+        const int t = 16 * tt + g + 4 * r;
         uu[tt][r] = t < num_targets ? static_cast<double>(target_norm2[t]) : 0.0;
       }
   }
@@ -128,14 +130,14 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
     zz += __shfl_xor(zz, 32, 64);
 
     if constexpr (TT > 0) {
-      // Lane (g, n) holds z.u for targets 16*tt + 4*g + r of pixel n.
+      // Lane (g, n) holds z.u for targets 16*tt + g + 4*r of pixel n.
       double best = -1.0;
       int best_t = -1;
 #pragma unroll
       for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int t = 16 * tt + 4 * g + r;
+          const int t = 16 * tt + g + 4 * r;
           const double u2 = uu[tt][r];
           if (t < num_targets && u2 > 0.0 && zz > 0.0) {
             const double s = (zu[tt][r] * zu[tt][r]) / (u2 * zz);
@@ -222,7 +224,7 @@ __global__ void layouts_kernel(LayoutArgs a) {
     a.white_blocks[e] = (row < n && col < n) ? a.whitener[static_cast<std::size_t>(row) * n + col] : 0.f;
   }
   // Whitened targets in the permuted A-operand order: [tt][I][r][lane] =
-  // U[16tt + lane%16][16I + 4(lane/16) + r].
+  // U[16tt + lane%16][16I + lane/16 + 4r].
   if (T > 0 && a.ace_u) {
     for (int e = tid; e < tiles * nb * 256; e += threads) {
       const int tt = e / (nb * 256);
@@ -230,7 +232,7 @@ __global__ void layouts_kernel(LayoutArgs a) {
       const int r = (e % 256) / 64;
       const int lane = e % 64;
       const int t = 16 * tt + (lane & 15);
-      const int b = 16 * I + 4 * (lane >> 4) + r;
+      const int b = 16 * I + (lane >> 4) + 4 * r;
       a.ace_u[e] = (t < T && b < n) ? a.target_white[static_cast<std::size_t>(t) * n + b] : 0.f;
     }
   }
@@ -245,20 +247,10 @@ std::size_t rxace_white_floats(int bands) {
 std::size_t rxace_target_floats(int bands, int num_targets) {
   return static_cast<std::size_t>(num_targets > 16 ? 2 : 1) * rx_blocks(bands) * 256;
 }
-std::size_t rxace_white_halves(int bands) { return rxace_white_floats(bands); }
-std::size_t rxace_target_halves(int bands, int num_targets) {
-  return rxace_target_floats(bands, num_targets);
-}
 
 hipError_t launch_rxace_plain(const RxAceArgs& a, hipStream_t stream) {
   if (a.dtype == HSI_F16) return launch_plain_typed<_Float16>(a, stream);
   return launch_plain_typed<float>(a, stream);
-}
-
-hipError_t launch_rxace_mixed(const RxAceArgs& a, hipStream_t stream) {
-  (void)a;
-  (void)stream;
-  return hipErrorNotSupported;
 }
 
 hipError_t launch_layouts(const LayoutArgs& a, hipStream_t stream) {

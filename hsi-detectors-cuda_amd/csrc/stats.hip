@@ -31,7 +31,7 @@ namespace {
 
 constexpr int kChunk = 32;       // pixels staged per LDS chunk
 constexpr int kMomWaves = 4;
-constexpr int kMaxMomWgs = 512;
+constexpr int kMaxMomWgs = 256;
 
 __host__ __device__ constexpr int pair_row(int q) {
   int i = 0;
@@ -128,7 +128,8 @@ __global__ __launch_bounds__(256) void moments_f64_kernel(const float* __restric
 
 // Fixed-order sum of the per-workgroup partials, then scatter of each
 // accumulator element to its matrix entry. v_mfma_f64_16x16x4 leaves element
-// (row 4*(l/16)+r, column l%16) of the 16x16 block in lane l, register r.
+// (row l/16 + 4r, column l%16) of the 16x16 block in lane l, register r
+// (rows interleaved across the four lane groups).
 __global__ void moments_reduce_kernel(const double* __restrict__ partial, int wgs, int nb,
                                       int bands, double* __restrict__ count,
                                       double* __restrict__ sum, double* __restrict__ sum_outer) {
@@ -142,7 +143,7 @@ __global__ void moments_reduce_kernel(const double* __restrict__ partial, int wg
   const int r = e % 4;
   const int bi = pair_row(q);
   const int bj = pair_col(q);
-  const int row = 16 * bi + 4 * (lane >> 4) + r;
+  const int row = 16 * bi + (lane >> 4) + 4 * r;
   const int col = 16 * bj + (lane & 15);
   if (row < bands) {
     if (col <= row) sum_outer[static_cast<std::size_t>(row) * bands + col] = s;
@@ -192,13 +193,19 @@ __device__ __forceinline__ int tri_row(int e) {
 // Right-looking Cholesky of a packed lower matrix, in place. Column j is
 // finished from the diagonal, then the trailing triangle takes its rank-1
 // update. Returns false (uniformly across the block) if a pivot is not
-// positive, which also catches NaN - the test cholesky() makes
-// (linalg.cpp:113).
-__device__ bool block_cholesky(double* a, int n, double* col) {
+// positive - the test cholesky() makes (linalg.cpp:113), which also catches
+// NaN - or is below 4*n*eps of that diagonal's starting value: an exactly
+// singular matrix cancels to a pivot of +-1 ulp, whose sign depends on the
+// summation order of the moments, and a whitener built on such a pivot is
+// meaningless once narrowed to float.
+__device__ bool block_cholesky(double* a, int n, double* col, double* diag0) {
   const int tid = threadIdx.x;
+  const double tol = 4.0 * n * 2.220446049250313e-16;
+  for (int i = tid; i < n; i += kBuildThreads) diag0[i] = a[packed(i, i)];
+  __syncthreads();
   for (int j = 0; j < n; ++j) {
     const double pivot = a[packed(j, j)];
-    if (!(pivot > 0.0)) return false;
+    if (!(pivot > 0.0) || !(pivot > tol * diag0[j])) return false;
     const double d = sqrt(pivot);
     for (int i = j + 1 + tid; i < n; i += kBuildThreads) {
       const double v = a[packed(i, j)] / d;
@@ -251,6 +258,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
   __shared__ double lds_mat[IN_LDS ? 2 * kLdsPacked : 1];
   __shared__ double mu[kMaxBands];
   __shared__ double col[kMaxBands];
+  __shared__ double diag0[kMaxBands];
   __shared__ double red[kBuildThreads / kWave];
   const int n = a.bands;
   const int tid = threadIdx.x;
@@ -277,7 +285,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     m1[e] = v;
   }
   __syncthreads();
-  if (!block_cholesky(m1, n, col)) {
+  if (!block_cholesky(m1, n, col, diag0)) {
     if (tid == 0) *a.status = HSI_STATUS_COV_NOT_PD;
     return;
   }
@@ -326,7 +334,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
       m1[e] = v;
     }
     __syncthreads();
-    if (!block_cholesky(m1, n, col)) {
+    if (!block_cholesky(m1, n, col, diag0)) {
       if (tid == 0) *a.status = HSI_STATUS_CORR_NOT_PD;
       return;
     }

@@ -36,6 +36,6 @@ def upload_golden_stats(ctx, g):
 
 
 def same_bits(a, b):
-    a = np.asarray(a)
-    b = np.asarray(b)
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
     return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))

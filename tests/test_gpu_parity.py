@@ -283,3 +283,37 @@ def test_detect_saturates_without_overrun(H, torch, ctx, oracle_lib):
     # an empty result: nothing passes
     dets, found = run_detect(H, torch, ctx, score, target, h, w, -1.0, 2, False, cap=16)
     assert found == 0
+
+
+# ---- precision form ---------------------------------------------------------------
+MIXED_REL = 1e-3  # score error bar of the mixed form against the FP64 oracle
+
+
+@pytest.mark.parametrize("name", SCENES)
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+def test_mixed_rx_ace_against_fp64_oracle(H, torch, ctx, oracle_lib, name, dtype):
+    g = golden(name)
+    b, h, w = g["cube"].shape
+    p = h * w
+    src = g["cube"] if dtype == "f32" else g["cube"].astype(np.float16).astype(np.float32)
+    cube = H.to_device_cube(src, h, w, dtype=dtype)
+    ctx.set_library(g["library"], g["norms"])
+    upload_golden_stats(ctx, g)
+    st = oracle_lib.Statistics(b, g["library"].shape[0])
+    st.mean, st.whitener = g["st_mean"], g["st_whitener"]
+    st.target_white, st.target_norm2 = g["st_target_white"], g["st_target_norm2"]
+    st.cem_weight = g["st_cem_weight"]
+    want_ace, want_t, want_rx = oracle_lib.ace_best(src.reshape(b, -1), st)
+    rx = device_buffers(torch, p)
+    ctx.score(H.Detector.RX, cube, h, w, b, rx["score"], precision=H.MIXED)
+    ace = device_buffers(torch, p)
+    ctx.score(H.Detector.ACE, cube, h, w, b, ace["score"], ace["target"], ace["aux"],
+              precision=H.MIXED)
+    torch.cuda.synchronize()
+    e_rx = rel_err(rx["score"].cpu().numpy(), want_rx, FLOOR["rx"]).max()
+    e_aux = rel_err(ace["aux"].cpu().numpy(), want_rx, FLOOR["rx"]).max()
+    e_ace = rel_err(ace["score"].cpu().numpy(), want_ace, FLOOR["ace"]).max()
+    print(f"{name} {dtype}: mixed RX max rel {e_rx:.3g}, ACE max {e_ace:.3g}")
+    assert e_rx <= MIXED_REL and e_aux <= MIXED_REL and e_ace <= MIXED_REL
+    agree = (ace["target"].cpu().numpy() == want_t).mean()
+    assert agree >= 0.99
