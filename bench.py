@@ -212,6 +212,7 @@ def main():
     r = args.repeats
     ms = {
         "moments": time_on_stream(torch, lambda: stats(c0), r),
+        "moments_mixed": time_on_stream(torch, lambda: ctx.stats_accumulate(c0, H_, W_, B_, precision=H.MIXED), r),
         "build": time_on_stream(torch, build, r),
         "sam_cem_fused": time_on_stream(torch, lambda: ctx.score_sam_cem(c0, H_, W_, B_, sam_s, sam_t, cem_s, cem_t), r),
         "sam": time_on_stream(torch, lambda: ctx.score(H.Detector.SAM, c0, H_, W_, B_, sam_s, sam_t), r),
@@ -219,6 +220,8 @@ def main():
         "detect": time_on_stream(torch, lambda: ctx.detect(sam_s, sam_t, H_, W_, p_sam, sam_d, sam_n), r),
         "rx": time_on_stream(torch, lambda: ctx.score(H.Detector.RX, c0, H_, W_, B_, cem_s), r),
         "ace": time_on_stream(torch, lambda: ctx.score(H.Detector.ACE, c0, H_, W_, B_, cem_s, cem_t, sam_s), r),
+        "rx_mixed": time_on_stream(torch, lambda: ctx.score(H.Detector.RX, c0, H_, W_, B_, cem_s, precision=H.MIXED), r),
+        "ace_mixed": time_on_stream(torch, lambda: ctx.score(H.Detector.ACE, c0, H_, W_, B_, cem_s, cem_t, sam_s, precision=H.MIXED), r),
     }
     for k, v in ms.items():
         log(f"  {k:14s} {v * 1e3:9.1f} us")
@@ -231,7 +234,9 @@ def main():
                 "moments": B_ * (B_ + 1), "rx": B_ * B_ + 4 * B_,
                 "ace": B_ * B_ + 4 * B_ + 2 * B_ * T_}
     detectors = {}
-    for k in ("sam", "cem", "sam_cem_fused", "rx", "ace", "moments"):
+    bytes_px["moments_mixed"] = cube_bytes
+    flops_px["moments_mixed"] = flops_px["moments"]
+    for k in ("sam", "cem", "sam_cem_fused", "rx", "ace", "moments", "moments_mixed"):
         sec = ms[k] / 1e3
         gbs = bytes_px[k] * pixels / sec / 1e9
         tfl = flops_px[k] * pixels / sec / 1e12

@@ -1,0 +1,436 @@
+// build.hip - DetectorStatistics from the moments, on the device in FP64.
+//
+// One workgroup of 8 waves turns (count, sum, sum x x^T) into what the
+// scorers read, as build_statistics does (detector.cpp:47-152):
+//   mean = sum / N;
+//   S = M2/N - mean mean^T + loading I   (covariance, linalg.cpp:66-79)
+//   R = M2/N + loading I                 (autocorrelation, linalg.cpp:52-64)
+//   L_S L_S^T = S, whitener W = L_S^-1   (linalg.cpp:105-122, 150-162)
+//   u_t = W (d_t - mean), |u_t|^2        (detector.cpp:111-128)
+//   L_R L_R^T = R, w_t = R^-1 d_t / (d_t^T R^-1 d_t)  (detector.cpp:133-147)
+// and narrows to float. A pivot that is not positive (or is below 4*n*eps of
+// its diagonal, see block_cholesky) sets the device status word instead.
+//
+// The factorisations are blocked over 16x16 tiles so that everything but
+// the 16x16 diagonal tiles is a tile product on v_mfma_f64_16x16x4:
+//   Cholesky, per panel K: one wave factors L_KK and inverts it in registers
+//   (lane i holds row i; cross-lane values by shuffle); the panel below is
+//   L_IK = A_IK L_KK^-T; the trailing tiles take A_IJ -= L_IK L_JK^T.
+//   Inverse, per tile row I: X_IJ = -L_II^-1 sum_K L_IK X_KJ for all J < I.
+// That is three workgroup barriers per panel and one per tile row, against
+// two per column for an unblocked sweep. Tiles live in LDS (bands <= 128) or
+// in global scratch, row-major with an XOR swizzle (element (r, c) at
+// r*16 + (c ^ r)) so that reading a tile column across 16 lanes touches 16
+// different LDS banks.
+//
+// MFMA tile products: lane l = 16g + m supplies A[m][k] and B[k][m] with the
+// K index of K-step s taken as k = g + 4s for both operands, and
+// v_mfma_f64_16x16x4 leaves C[g + 4r][m] in lane l, register r - so a result
+// can be fed back as a B operand without a transpose (register s is exactly
+// B[g + 4s][m]).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace hsi_amd {
+namespace {
+
+constexpr int kThreads = 512;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kTile = 16;
+constexpr int kTileElems = kTile * kTile;
+constexpr int kLdsTiles = 8;  // up to 128 bands in LDS
+constexpr int kLdsBlocks = kLdsTiles * (kLdsTiles + 1) / 2;
+
+__device__ __forceinline__ int sw(int r, int c) { return r * kTile + (c ^ r); }
+__device__ __forceinline__ int tile_index(int I, int J) { return I * (I + 1) / 2 + J; }
+
+// Tiles are reached through a pointer whose address space is part of the
+// type: LDS-resident matrices use address_space(3) pointers so every access
+// compiles to ds_read/ds_write, not to generic (flat) loads.
+typedef __attribute__((address_space(3))) double lds_double;
+
+template <typename P>
+struct Mat {
+  P* base;
+  __device__ P* tile(int I, int J) const { return base + tile_index(I, J) * kTileElems; }
+};
+
+__device__ __forceinline__ void wave_sync() {
+  // Order this wave's LDS/global writes before its later reads of other
+  // lanes' data (a wavefront executes in lockstep; this waits for the
+  // memory operations and keeps the compiler from reordering them).
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+// C (4 regs, layout C[g+4r][m]) <- tile t
+template <typename P>
+__device__ __forceinline__ d4 load_c(const P* t, int lane) {
+  const int g = lane >> 4, m = lane & 15;
+  return d4{t[sw(g, m)], t[sw(g + 4, m)], t[sw(g + 8, m)], t[sw(g + 12, m)]};
+}
+template <typename P>
+__device__ __forceinline__ void store_c(P* t, int lane, const d4& c) {
+  const int g = lane >> 4, m = lane & 15;
+  t[sw(g, m)] = c[0];
+  t[sw(g + 4, m)] = c[1];
+  t[sw(g + 8, m)] = c[2];
+  t[sw(g + 12, m)] = c[3];
+}
+// Operand value for K-step s: M[m][g + 4s] (A of M, or B of M^T).
+template <typename P>
+__device__ __forceinline__ double op(const P* t, int lane, int s) {
+  return t[sw(lane & 15, (lane >> 4) + 4 * s)];
+}
+// Operand value for K-step s from a matrix used untransposed as B: M[g+4s][m].
+template <typename P>
+__device__ __forceinline__ double op_t(const P* t, int lane, int s) {
+  return t[sw((lane >> 4) + 4 * s, lane & 15)];
+}
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Factor the 16x16 diagonal tile and invert it, in one wave. Lane i < 16
+// holds row i; returns false if a pivot fails. Writes L_KK into `lt` and
+// L_KK^-1 into `xt` (both swizzled, upper triangle zero).
+template <typename P>
+__device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_double* diag0, int lane) {
+  double row[kTile];
+  const int i = lane & 15;
+#pragma unroll
+  for (int c = 0; c < kTile; ++c) row[c] = (lane < 16 && c <= i) ? lt[sw(i, c)] : 0.0;
+  bool ok = true;
+  const double tol = 4.0 * 256 * 2.220446049250313e-16;
+#pragma unroll
+  for (int j = 0; j < kTile; ++j) {
+    const double pivot = __shfl(row[j], j, kWave);
+    const double d0 = diag0[j];
+    if (!(pivot > 0.0) || !(pivot > tol * d0)) ok = false;
+    const double d = ok ? sqrt(pivot) : 1.0;
+    const double rd = 1.0 / d;
+    if (i == j) row[j] = d;
+    if (i > j) row[j] = row[j] * rd;
+    const double lij = row[j];
+#pragma unroll
+    for (int k = j + 1; k < kTile; ++k) {
+      const double lkj = __shfl(lij, k, kWave);
+      if (i >= k) row[k] -= lij * lkj;
+    }
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int c = 0; c < kTile; ++c) lt[sw(i, c)] = c <= i ? row[c] : 0.0;
+  }
+  wave_sync();
+  // X = L^-1, right-looking over rows: row j final, then rows below subtract.
+  // L is re-read from the tile so only one 16-element row is live at a time.
+  double x[kTile];
+#pragma unroll
+  for (int c = 0; c < kTile; ++c) x[c] = 0.0;
+#pragma unroll
+  for (int j = 0; j < kTile; ++j) {
+    // One reciprocal per pivot; row j of X is then scaled by it (a full
+    // IEEE division per element would cost sixteen per step).
+    const double rjj = 1.0 / lt[sw(j, j)];
+    if (i == j) {
+#pragma unroll
+      for (int c = 0; c < kTile; ++c) x[c] = (c == j) ? rjj : x[c] * rjj;
+    }
+    const double lij = lt[sw(i, j)];
+#pragma unroll
+    for (int c = 0; c <= j; ++c) {
+      const double xjc = __shfl(x[c], j, kWave);
+      if (i > j) x[c] -= lij * xjc;
+    }
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int c = 0; c < kTile; ++c) xt[sw(i, c)] = c <= i ? x[c] : 0.0;
+  }
+  return ok;
+}
+
+// Blocked Cholesky of A (nt x nt tiles, lower) in place; the inverses of the
+// diagonal tiles go to X's diagonal. Returns false uniformly on failure.
+template <typename P>
+__device__ __attribute__((noinline)) bool block_cholesky(Mat<P> A, Mat<P> X, int nt, const lds_double* diag0,
+                                                         __attribute__((address_space(3))) int* flag) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  for (int K = 0; K < nt; ++K) {
+    if (wave == 0) {
+      const bool ok = diag_factor(A.tile(K, K), X.tile(K, K), diag0 + K * kTile, lane);
+      if (lane == 0 && !ok) *flag = 1;
+    }
+    __syncthreads();
+    if (*flag) return false;
+    // Panel: L_IK = A_IK * Linv_KK^T.
+    const P* xkk = X.tile(K, K);
+    for (int I = K + 1 + wave; I < nt; I += kWaves) {
+      P* aik = A.tile(I, K);
+      d4 c = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) c = mfma(op(aik, lane, s), op(xkk, lane, s), c);
+      wave_sync();
+      store_c(aik, lane, c);
+    }
+    __syncthreads();
+    // Trailing: A_IJ -= L_IK L_JK^T for K < J <= I.
+    const int rest = nt - K - 1;
+    const int count = rest * (rest + 1) / 2;
+    for (int q = wave; q < count; q += kWaves) {
+      int r = 0;
+      while ((r + 1) * (r + 2) / 2 <= q) ++r;
+      const int I = K + 1 + r;
+      const int J = K + 1 + (q - r * (r + 1) / 2);
+      P* aij = A.tile(I, J);
+      const P* lik = A.tile(I, K);
+      const P* ljk = A.tile(J, K);
+      d4 c = load_c(aij, lane);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) c = mfma(-op(lik, lane, s), op(ljk, lane, s), c);
+      store_c(aij, lane, c);
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
+// X = L^-1 given L (tiles) and the diagonal tiles of X already inverted.
+template <typename P>
+__device__ __attribute__((noinline)) void block_inverse(Mat<P> L, Mat<P> X, int nt) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  for (int I = 1; I < nt; ++I) {
+    for (int J = wave; J < I; J += kWaves) {
+      d4 s = {0.0, 0.0, 0.0, 0.0};
+      for (int K = J; K < I; ++K) {
+        const P* lik = L.tile(I, K);
+        const P* xkj = X.tile(K, J);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) s = mfma(op(lik, lane, t), op_t(xkj, lane, t), s);
+      }
+      // X_IJ = -Linv_II * S, with S fed straight back as the B operand.
+      const P* xii = X.tile(I, I);
+      d4 c = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) c = mfma(-op(xii, lane, t), s[t], c);
+      store_c(X.tile(I, J), lane, c);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename P>
+__device__ __forceinline__ double tile_get(Mat<P> M, int i, int j) {
+  return M.tile(i >> 4, j >> 4)[sw(i & 15, j & 15)];
+}
+
+// Fill tiles of an n-band matrix padded to nt*16 with identity on the padded
+// diagonal: S (covariance) if centred, else R (autocorrelation). One wave per
+// tile; each lane loads its four entries at once (C layout).
+template <typename P>
+__device__ __attribute__((noinline)) void fill(Mat<P> A, int n, int nt, const double* so, double count,
+                                               const lds_double* mu, double loading, bool centred,
+                                               lds_double* diag0) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, m = lane & 15;
+  const int tiles = nt * (nt + 1) / 2;
+  for (int q = wave; q < tiles; q += kWaves) {
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= q) ++I;
+    const int J = q - I * (I + 1) / 2;
+    const int j = 16 * J + m;
+    double raw[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * I + g + 4 * r;
+      raw[r] = (i < n && j <= i) ? so[static_cast<std::size_t>(i) * n + j] : 0.0;
+    }
+    d4 c;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * I + g + 4 * r;
+      double v = 0.0;
+      if (j <= i) {
+        if (i < n) {
+          v = raw[r] / count;
+          if (centred) v = v - mu[i] * mu[j];
+          if (i == j) v += loading;
+        } else if (i == j) {
+          v = 1.0;
+        }
+      }
+      c[r] = v;
+      if (i == j) diag0[i] = v;
+    }
+    store_c(A.tile(I, J), lane, c);
+  }
+}
+
+// out[t][i] = sum_j M[i][j] V(t, j)   (TRANS false: M lower, j <= i), or
+// out[t][i] = sum_j M[j][i] V(t, j)   (TRANS true:  j >= i),
+// for t < T, i < n, as 16x16 tile products on the matrix cores: A from the
+// LDS/global tiles of M, B = V(t, j) loaded per lane (independent loads).
+template <bool TRANS, typename P, typename VF>
+__device__ __attribute__((noinline)) void tile_matvec(Mat<P> M, int nt, int n, int T, VF v_at, double* out) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  const int tt_count = (T + 15) / 16;
+  for (int item = wave; item < nt * tt_count; item += kWaves) {
+    const int I = item / tt_count;
+    const int tt = item % tt_count;
+    const int t = 16 * tt + col;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int j0 = TRANS ? I : 0;
+    const int j1 = TRANS ? nt : I + 1;
+    for (int J = j0; J < j1; ++J) {
+      const P* tile = TRANS ? M.tile(J, I) : M.tile(I, J);
+      double bv[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int j = 16 * J + g + 4 * s;
+        bv[s] = (t < T && j < n) ? v_at(t, j) : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = mfma(TRANS ? op_t(tile, lane, s) : op(tile, lane, s), bv[s], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * I + g + 4 * r;
+      if (t < T && i < n) out[static_cast<std::size_t>(t) * n + i] = acc[r];
+    }
+  }
+}
+
+// Per-target dot products over n, one wave per target: result[t] = sum_i f(t, i).
+template <typename F>
+__device__ __forceinline__ double wave_sum(int n, F f, int t) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int i = lane; i < n; i += kWave) s += f(t, i);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, kWave);
+  return __shfl(s, 0, kWave);
+}
+
+typedef __attribute__((address_space(3))) int lds_int;
+
+template <typename P>
+__device__ __forceinline__ void build_body(const BuildArgs& a, Mat<P> A, Mat<P> X, lds_double* mu,
+                                           lds_double* diag0, lds_int* flag) {
+  const int n = a.bands;
+  const int nt = (n + kTile - 1) / kTile;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int blocks = nt * (nt + 1) / 2;
+  double* vec = a.work + 2 * static_cast<std::size_t>(blocks) * kTileElems;  // [T][n]
+  double* solved = vec + static_cast<std::size_t>(kMaxTargets) * kMaxBands;  // [T][n]
+
+  const double count = *a.count;
+  if (!(count > 0.0)) {
+    if (tid == 0) *a.status = HSI_STATUS_COV_NOT_PD;
+    return;
+  }
+  if (tid == 0) *flag = 0;
+  for (int i = tid; i < n; i += kThreads) mu[i] = a.sum[i] / count;
+  __syncthreads();
+
+  fill(A, n, nt, a.sum_outer, count, mu, a.loading, true, diag0);
+  __syncthreads();
+  if (!block_cholesky(A, X, nt, diag0, flag)) {
+    if (tid == 0) *a.status = HSI_STATUS_COV_NOT_PD;
+    return;
+  }
+  block_inverse(A, X, nt);
+
+  for (int i = tid; i < n; i += kThreads) a.mean[i] = static_cast<float>(mu[i]);
+  for (int e = tid; e < n * n; e += kThreads) {
+    const int i = e / n, k = e % n;
+    a.whitener[e] = (k <= i) ? static_cast<float>(tile_get(X, i, k)) : 0.0f;
+  }
+  const int T = a.num_targets;
+  if (T > 0) {
+    const float* tg = a.targets;
+    // u_t = W (d_t - mean) with the double whitener, and |u_t|^2.
+    tile_matvec<false>(X, nt, n, T,
+                       [&](int t, int j) { return static_cast<double>(tg[static_cast<std::size_t>(t) * n + j]) - mu[j]; },
+                       vec);
+    __syncthreads();
+    for (int e = tid; e < T * n; e += kThreads) a.target_white[e] = static_cast<float>(vec[e]);
+    for (int t = wave; t < T; t += kWaves) {
+      const double s = wave_sum(n, [&](int tt, int i) { return vec[tt * n + i] * vec[tt * n + i]; }, t);
+      if (lane == 0) a.target_norm2[t] = static_cast<float>(s);
+    }
+    __syncthreads();
+
+    // CEM: R = M2/N + loading I -> L_R -> L_R^-1; R^-1 d = L_R^-T (L_R^-1 d).
+    fill(A, n, nt, a.sum_outer, count, mu, a.loading, false, diag0);
+    __syncthreads();
+    if (!block_cholesky(A, X, nt, diag0, flag)) {
+      if (tid == 0) *a.status = HSI_STATUS_CORR_NOT_PD;
+      return;
+    }
+    block_inverse(A, X, nt);
+    tile_matvec<false>(X, nt, n, T,
+                       [&](int t, int j) { return static_cast<double>(tg[static_cast<std::size_t>(t) * n + j]); },
+                       vec);
+    __syncthreads();
+    tile_matvec<true>(X, nt, n, T, [&](int t, int j) { return vec[static_cast<std::size_t>(t) * n + j]; },
+                      solved);
+    __syncthreads();
+    for (int t = wave; t < T; t += kWaves) {
+      const double gain = wave_sum(
+          n, [&](int tt, int i) { return static_cast<double>(tg[static_cast<std::size_t>(tt) * n + i]) * solved[tt * n + i]; },
+          t);
+      for (int i = lane; i < n; i += kWave) {
+        a.cem_weight[static_cast<std::size_t>(t) * n + i] =
+            gain > 0.0 ? static_cast<float>(solved[t * n + i] / gain) : 0.0f;
+      }
+    }
+  }
+  if (tid == 0) *a.status = 0u;
+}
+
+template <bool IN_LDS>
+__global__ __launch_bounds__(kThreads) void build_kernel(BuildArgs a) {
+  __shared__ double lds[IN_LDS ? 2 * kLdsBlocks * kTileElems : 1];
+  __shared__ double mu[kMaxBands];
+  __shared__ double diag0[kMaxBands];
+  __shared__ int flag;
+  lds_double* l_mu = (lds_double*)mu;
+  lds_double* l_diag0 = (lds_double*)diag0;
+  lds_int* l_flag = (lds_int*)&flag;
+  if constexpr (IN_LDS) {
+    lds_double* base = (lds_double*)lds;
+    build_body(a, Mat<lds_double>{base}, Mat<lds_double>{base + kLdsBlocks * kTileElems}, l_mu,
+               l_diag0, l_flag);
+  } else {
+    const int nt = (a.bands + kTile - 1) / kTile;
+    const std::size_t blocks = static_cast<std::size_t>(nt) * (nt + 1) / 2;
+    build_body(a, Mat<double>{a.work}, Mat<double>{a.work + blocks * kTileElems}, l_mu, l_diag0,
+               l_flag);
+  }
+}
+
+}  // namespace
+
+std::size_t build_work_doubles(int bands, int num_targets) {
+  (void)num_targets;
+  const std::size_t nt = (static_cast<std::size_t>(bands) + kTile - 1) / kTile;
+  return 2 * nt * (nt + 1) / 2 * kTileElems + 2 * static_cast<std::size_t>(kMaxTargets) * kMaxBands;
+}
+
+hipError_t launch_build(const BuildArgs& a, hipStream_t stream) {
+  if (a.bands <= kLdsTiles * kTile) {
+    hipLaunchKernelGGL(build_kernel<true>, dim3(1), dim3(kThreads), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL(build_kernel<false>, dim3(1), dim3(kThreads), 0, stream, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace hsi_amd

@@ -25,8 +25,6 @@ namespace {
 
 constexpr int kRxWaves = 4;
 
-__device__ __forceinline__ double cvt_in(float v) { return static_cast<double>(v); }
-__device__ __forceinline__ double cvt_in(_Float16 v) { return static_cast<double>(static_cast<float>(v)); }
 
 // Combine two (score, target) candidates so the larger score wins and, on
 // equal scores, the smaller target index - which is what a strict `>` scan
@@ -76,36 +74,54 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
       }
   }
 
+  // Raw inputs of a 16-pixel tile, all issued at once: band 16J + 4kk + g of
+  // pixel tile*16 + n (the B-operand layout). The next tile's loads are in
+  // flight while the matrix cores work on this one.
+  auto load_tile = [&](std::size_t t, float (&xr)[NB][4]) {
+    const std::size_t pp = t * 16 + n;
+#pragma unroll
+    for (int J = 0; J < NB; ++J)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int b = 16 * J + 4 * kk + g;
+        xr[J][kk] = (pp < pixels && b < bands) ? static_cast<float>(cube[static_cast<std::size_t>(b) * stride + pp]) : 0.f;
+      }
+  };
+  float cur[NB][4], nxt[NB][4];
+  if (wave_id < tiles) load_tile(wave_id, cur);
+
   for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
     const std::size_t p = tile * 16 + n;
     const bool live = p < pixels;
-    // Z = W Y accumulated one input band block J at a time, so only that
-    // block of the centred spectrum is live: y[kk] = x[16J+4kk+g][p] - mean.
+    if (tile + waves < tiles) load_tile(tile + waves, nxt);
+    // Z = W Y accumulated one input band block J at a time.
     d4 z[NB];
 #pragma unroll
     for (int I = 0; I < NB; ++I) z[I] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 1
+#pragma unroll
     for (int J = 0; J < NB; ++J) {
       double y[4];
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         const int b = 16 * J + 4 * kk + g;
-        double v = 0.0;
-        if (live && b < bands)
-          v = cvt_in(cube[static_cast<std::size_t>(b) * stride + p]) - static_cast<double>(mean_lds[b]);
-        y[kk] = v;
+        // padded bands and pixels read 0; their whitener columns are 0 too
+        y[kk] = (b < bands) ? static_cast<double>(cur[J][kk]) - static_cast<double>(mean_lds[b]) : 0.0;
       }
 #pragma unroll
-      for (int I = 0; I < NB; ++I) {
-        if (I >= J) {  // lower blocks only; uniform across the wave
-          const float* blk = w_lds + (I * (I + 1) / 2 + J) * 256;
+      for (int I = J; I < NB; ++I) {
+        const float* blk = w_lds + (I * (I + 1) / 2 + J) * 256;
 #pragma unroll
-          for (int kk = 0; kk < 4; ++kk)
-            z[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(blk[kk * 64 + lane]),
-                                                        y[kk], z[I], 0, 0, 0);
-        }
+        for (int kk = 0; kk < 4; ++kk)
+          z[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(blk[kk * 64 + lane]), y[kk],
+                                                      z[I], 0, 0, 0);
       }
+      // Keep each band block's whitener loads next to their MFMAs.
+      __builtin_amdgcn_sched_barrier(0);
     }
+#pragma unroll
+    for (int J = 0; J < NB; ++J)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) cur[J][kk] = nxt[J][kk];
 
     double zz = 0.0;
     d4 zu[TT > 0 ? TT : 1];

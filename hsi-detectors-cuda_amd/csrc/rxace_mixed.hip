@@ -99,23 +99,37 @@ __global__ __launch_bounds__(kMxWaves * 64) void rxace_mixed_kernel(
   const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(kMxWaves) + (threadIdx.x >> 6);
   const std::size_t waves = static_cast<std::size_t>(gridDim.x) * kMxWaves;
 
+  // Raw inputs of a 16-pixel tile, all issued at once: band 32J + 8g + i of
+  // pixel tile*16 + n. The next tile's loads are in flight while the matrix
+  // cores work on this one.
+  auto load_tile = [&](std::size_t t, float (&xr)[NKB][8]) {
+    const std::size_t pp = t * 16 + n;
+#pragma unroll
+    for (int J = 0; J < NKB; ++J)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int b = 32 * J + 8 * g + i;
+        xr[J][i] = (pp < pixels && b < bands) ? to_f(cube[static_cast<std::size_t>(b) * stride + pp]) : 0.f;
+      }
+  };
+  float cur[NKB][8], nxt[NKB][8];
+  if (wave_id < tiles) load_tile(wave_id, cur);
+
   for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
     const std::size_t p = tile * 16 + n;
     const bool live = p < pixels;
+    if (tile + waves < tiles) load_tile(tile + waves, nxt);
     f4 z[NB];
 #pragma unroll
     for (int I = 0; I < NB; ++I) z[I] = f4{0.f, 0.f, 0.f, 0.f};
 
-#pragma unroll 1
+#pragma unroll
     for (int J = 0; J < NKB; ++J) {
       float v[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const int b = 32 * J + 8 * g + i;
-        float y = 0.f;
-        if (live && b < bands)
-          y = (to_f(cube[static_cast<std::size_t>(b) * stride + p]) - mean_lds[b]) * scale_lds[b];
-        v[i] = y;
+        v[i] = (cur[J][i] - mean_lds[b]) * scale_lds[b];  // padding: scale 0
       }
       h8 yh, yl;
       split8(v, yh, yl);
@@ -130,7 +144,14 @@ __global__ __launch_bounds__(kMxWaves * 64) void rxace_mixed_kernel(
           z[I] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, yh, z[I], 0, 0, 0);
         }
       }
+      // Keep each K block's whitener loads next to their MFMAs (hoisting them
+      // all would need ~160 registers).
+      __builtin_amdgcn_sched_barrier(0);
     }
+#pragma unroll
+    for (int J = 0; J < NKB; ++J)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) cur[J][i] = nxt[J][i];
 
     double zz = 0.0;
 #pragma unroll
@@ -245,32 +266,37 @@ __global__ __launch_bounds__(1024) void mixed_layouts_kernel(MixedLayoutArgs a) 
   const int nb = rx_blocks(n);
   const int T = a.num_targets;
   const int tid = threadIdx.x;
+  // Maxima of |x| kept as the bit patterns of non-negative floats, whose
+  // unsigned order is the float order, so LDS atomicMax finds them in any
+  // order and the result is deterministic.
+  __shared__ unsigned cmax[kMaxBands], rmax[kMaxBands], tmax[kMaxTargets];
   __shared__ float cs[kMaxBands];   // column scale s_b (y' = y * s_b)
   __shared__ float rs[kMaxBands];   // row scale r_i
   __shared__ float ts[kMaxTargets]; // target scale c_t
-  // W' = W / s: column maxima of |W| mapped into [8, 16) after division.
-  for (int b = tid; b < kMaxBands; b += blockDim.x) {
-    float m = 0.f;
-    if (b < n)
-      for (int i = b; i < n; ++i) m = fmaxf(m, fabsf(a.whitener[static_cast<std::size_t>(i) * n + b]));
-    // s = 1 / pow2_for(m): W/s = W * pow2_for(m)
-    const float k = pow2_for(m);
-    cs[b] = b < n ? 1.f / k : 0.f;
+  for (int b = tid; b < kMaxBands; b += blockDim.x) cmax[b] = rmax[b] = 0u;
+  for (int t = tid; t < kMaxTargets; t += blockDim.x) tmax[t] = 0u;
+  __syncthreads();
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int i = e / n, b = e % n;
+    if (b <= i) atomicMax(&cmax[b], __float_as_uint(fabsf(a.whitener[e])));
   }
   __syncthreads();
-  for (int i = tid; i < kMaxBands; i += blockDim.x) {
-    float m = 0.f;
-    if (i < n)
-      for (int b = 0; b <= i; ++b) m = fmaxf(m, fabsf(a.whitener[static_cast<std::size_t>(i) * n + b] / cs[b]));
-    rs[i] = pow2_for(m);
+  // W' = W / s with s = 1 / pow2_for(column max): columns land in [8, 16).
+  for (int b = tid; b < kMaxBands; b += blockDim.x) cs[b] = b < n ? 1.f / pow2_for(__uint_as_float(cmax[b])) : 0.f;
+  __syncthreads();
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int i = e / n, b = e % n;
+    if (b <= i) atomicMax(&rmax[i], __float_as_uint(fabsf(a.whitener[e] / cs[b])));
   }
   __syncthreads();
-  for (int t = tid; t < kMaxTargets; t += blockDim.x) {
-    float m = 0.f;
-    if (t < T)
-      for (int i = 0; i < n; ++i) m = fmaxf(m, fabsf(a.target_white[static_cast<std::size_t>(t) * n + i] / rs[i]));
-    ts[t] = pow2_for(m);
+  for (int i = tid; i < kMaxBands; i += blockDim.x) rs[i] = pow2_for(__uint_as_float(rmax[i]));
+  __syncthreads();
+  for (int e = tid; e < T * n; e += blockDim.x) {
+    const int t = e / n, i = e % n;
+    atomicMax(&tmax[t], __float_as_uint(fabsf(a.target_white[e] / rs[i])));
   }
+  __syncthreads();
+  for (int t = tid; t < kMaxTargets; t += blockDim.x) ts[t] = pow2_for(__uint_as_float(tmax[t]));
   __syncthreads();
   for (int b = tid; b < kMaxBands; b += blockDim.x) {
     a.col_scale[b] = cs[b];

@@ -1,36 +1,36 @@
-// stats.hip - scene statistics on the device (gfx950).
+// stats.hip - scene moments on the device (gfx950).
 //
-// Two stages per frame, both FP64 where the reference is FP64:
+// sum(x) and the lower triangle of sum(x x^T) over every pixel, as
+// SpectralMoments::add accumulates them (linalg.cpp:18-29,
+// detectors_cpu.cpp:77-93). This is a rank-P symmetric update of a BxB matrix
+// - a SYRK with K = pixels - so it runs on the matrix cores over 16x16 band
+// blocks (lower blocks only). An extra "ones" band appended after the last
+// band makes the same products yield sum(x) (its row) and the pixel count
+// (its diagonal). Each workgroup owns a contiguous pixel range, stages
+// 32-pixel chunks of the cube in LDS (double-buffered: the next chunk's global
+// loads are in flight while the matrix cores work on the current one), and
+// its waves accumulate their share of the blocks in registers. Per-workgroup
+// partials are reduced in a fixed order by a second kernel, so results are
+// bit-reproducible run to run.
 //
-// 1. Moments. sum(x) and the lower triangle of sum(x x^T) over every pixel,
-//    as SpectralMoments::add accumulates them (linalg.cpp:18-29,
-//    detectors_cpu.cpp:77-93). This is a rank-P symmetric update of a BxB
-//    matrix - a SYRK with K = pixels - so it runs on the matrix cores:
-//    v_mfma_f64_16x16x4 over 16x16 band blocks (lower blocks only). Each
-//    workgroup stages 32-pixel chunks of the cube in LDS (widened to double,
-//    one extra "ones" band appended so that the same products give sum(x) and
-//    the pixel count), and its waves accumulate their share of the blocks in
-//    registers over a contiguous pixel range. Per-workgroup partials are
-//    reduced in a fixed order by a second kernel, so the result is
-//    bit-reproducible run to run. Every product of two floats is exact in
-//    double, so the only difference from the CPU oracle is summation order.
-//
-// 2. Build. One workgroup turns the moments into DetectorStatistics the way
-//    build_statistics does (detector.cpp:47-152): mean; covariance and
-//    autocorrelation with diagonal loading (linalg.cpp:45-79); Cholesky
-//    (linalg.cpp:105-122) with a device status word for a matrix that is not
-//    positive definite; the triangular inverse (linalg.cpp:150-162); the
-//    whitened centred targets and their squared norms; and the CEM filters
-//    R^-1 d / (d^T R^-1 d), with R^-1 d formed as L_R^-T (L_R^-1 d). Matrices
-//    live packed-lower in LDS (up to 128 bands) or in global scratch beyond.
+// Two arithmetic forms:
+//  PLAIN  v_mfma_f64_16x16x4 on the widened floats: every product of two
+//         floats is exact in double, so the only difference from the CPU
+//         oracle is the order of the FP64 sums.
+//  MIXED  the shifted moments of d = x - r (r: a per-band reference near the
+//         mean, from a sparse sample of the frame) on v_mfma_f32_16x16x32_f16
+//         with d split into FP16 hi + lo parts (hi*hi + hi*lo + lo*hi), FP32
+//         accumulation over 128-pixel tiles, and FP64 across tiles - the
+//         hierarchy the reference designed for its device accumulator
+//         (linalg.hpp:53-90, stats_cuda.hpp:121-135). The raw moments are then
+//         rebuilt in FP64 as moments_from_shifted does (linalg.cpp:81-103).
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace hsi_amd {
 namespace {
 
-constexpr int kChunk = 32;       // pixels staged per LDS chunk
-constexpr int kMomWaves = 4;
+constexpr int kChunk = 64;       // pixel granularity of the workgroup ranges
 constexpr int kMaxMomWgs = 256;
 
 __host__ __device__ constexpr int pair_row(int q) {
@@ -40,28 +40,77 @@ __host__ __device__ constexpr int pair_row(int q) {
 }
 __host__ __device__ constexpr int pair_col(int q) { return q - pair_row(q) * (pair_row(q) + 1) / 2; }
 
-template <int Q, int NP, int NB>
-__device__ __forceinline__ void mfma_pairs(int wave, const double (&op)[NB],
-                                           d4 (&acc)[(NP + kMomWaves - 1) / kMomWaves]) {
+// One MFMA per owned block pair; the pair index is a template parameter so the
+// operand and accumulator indices are compile-time and stay in registers.
+// Pair q belongs to owner q % OWNERS (an owner is one wave of one of the
+// workgroups that share a pixel range).
+template <int Q, int NP, int NB, int OWNERS>
+__device__ __forceinline__ void mfma_pairs_f64(int owner, const double (&op)[NB],
+                                               d4 (&acc)[(NP + OWNERS - 1) / OWNERS]) {
   if constexpr (Q < NP) {
-    if (wave == Q % kMomWaves) {
-      acc[Q / kMomWaves] = __builtin_amdgcn_mfma_f64_16x16x4f64(op[pair_row(Q)], op[pair_col(Q)],
-                                                               acc[Q / kMomWaves], 0, 0, 0);
+    if (owner == Q % OWNERS) {
+      acc[Q / OWNERS] = __builtin_amdgcn_mfma_f64_16x16x4f64(op[pair_row(Q)], op[pair_col(Q)],
+                                                            acc[Q / OWNERS], 0, 0, 0);
     }
-    mfma_pairs<Q + 1, NP, NB>(wave, op, acc);
+    mfma_pairs_f64<Q + 1, NP, NB, OWNERS>(owner, op, acc);
   }
 }
 
+template <int Q, int NP, int NB, int OWNERS>
+__device__ __forceinline__ void mfma_pairs_f16(int owner, const h8 (&hi)[NB], const h8 (&lo)[NB],
+                                               f4 (&acc)[(NP + OWNERS - 1) / OWNERS]) {
+  if constexpr (Q < NP) {
+    if (owner == Q % OWNERS) {
+      constexpr int I = pair_row(Q), J = pair_col(Q), M = Q / OWNERS;
+      acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi[I], hi[J], acc[M], 0, 0, 0);
+      acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi[I], lo[J], acc[M], 0, 0, 0);
+      acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo[I], hi[J], acc[M], 0, 0, 0);
+    }
+    mfma_pairs_f16<Q + 1, NP, NB, OWNERS>(owner, hi, lo, acc);
+  }
+}
+
+// float4 of band b at pixel px (px a multiple of 4) for the staging pass: the
+// cube values, the ones band, or zeros past the band count / pixel range.
+__device__ __forceinline__ float4 stage_load(const float* __restrict__ cube, std::size_t stride,
+                                             int bands, int b, std::size_t px, std::size_t p_end) {
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (px < p_end) {
+    if (b < bands) v = *reinterpret_cast<const float4*>(cube + static_cast<std::size_t>(b) * stride + px);
+    else if (b == bands) v = make_float4(1.f, 1.f, 1.f, 1.f);
+    if (px + 1 >= p_end) v.y = 0.f;
+    if (px + 2 >= p_end) v.z = 0.f;
+    if (px + 3 >= p_end) v.w = 0.f;
+  }
+  return v;
+}
+
+// Launch shape of each form, per block count NB. Eight waves per workgroup
+// (two per SIMD) let one wave's LDS traffic overlap another's MFMAs; chunks
+// of 64 pixels where LDS allows keep ~a chunk of loads in flight per CU.
+// Beyond 12 blocks the mixed form splits a pixel range's block pairs over two
+// workgroups (eight ids apart, so the dispatcher puts both on one XCD and the
+// second read of each chunk hits that XCD's L2) to fit its accumulators.
+constexpr int kWavesPerWg = 8;
+constexpr int kXcds = 8;
+template <int NB> constexpr int f64_chunk() { return NB <= 8 ? 64 : 32; }
+template <int NB> constexpr int f16_chunk() { return 64; }
+template <int NB> constexpr int f16_splits() { return NB > 12 ? 2 : 1; }
+
 template <int NB>
-__global__ __launch_bounds__(256) void moments_f64_kernel(const float* __restrict__ cube,
-                                                          std::size_t pixels, int bands,
-                                                          std::size_t stride,
-                                                          std::size_t px_per_wg,
-                                                          double* __restrict__ partial) {
+__global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const float* __restrict__ cube,
+                                                                      std::size_t pixels, int bands,
+                                                                      std::size_t stride,
+                                                                      std::size_t px_per_wg,
+                                                                      double* __restrict__ partial) {
+  constexpr int CH = f64_chunk<NB>();
+  constexpr int THREADS = kWavesPerWg * 64;
   constexpr int NP = NB * (NB + 1) / 2;
-  constexpr int PW = (NP + kMomWaves - 1) / kMomWaves;
-  constexpr int LDB = NB * 16 + 1;  // odd row pitch in doubles spreads LDS banks
-  __shared__ double tile[kChunk * LDB];
+  constexpr int PW = (NP + kWavesPerWg - 1) / kWavesPerWg;
+  constexpr int LDB = NB * 16 + 1;                    // odd pitch spreads LDS banks
+  constexpr int LOADS = NB * 16 * (CH / 4);           // float4 per chunk
+  constexpr int PER = (LOADS + THREADS - 1) / THREADS;
+  __shared__ double tile[2][CH * LDB];
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -72,50 +121,60 @@ __global__ __launch_bounds__(256) void moments_f64_kernel(const float* __restric
   const std::size_t p_begin = blockIdx.x * px_per_wg;
   std::size_t p_end = p_begin + px_per_wg;
   if (p_end > pixels) p_end = pixels;
+  if (p_begin >= p_end) return;
+  const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
 
-  for (std::size_t p = p_begin; p < p_end; p += kChunk) {
-    __syncthreads();
-    // Stage NB*16 band rows x 32 pixels: float4 loads along the band plane
-    // (8 lanes cover one 128-byte row), widened to double and stored
-    // transposed so a 16-band slice of one pixel is contiguous.
-    for (int e = threadIdx.x; e < NB * 16 * (kChunk / 4); e += 256) {
-      const int b = e / (kChunk / 4);
-      const int k4 = (e % (kChunk / 4)) * 4;
-      const std::size_t px = p + k4;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (px < p_end) {
-        if (b < bands) {
-          v = *reinterpret_cast<const float4*>(cube + static_cast<std::size_t>(b) * stride + px);
-        } else if (b == bands) {
-          v = make_float4(1.f, 1.f, 1.f, 1.f);
-        }
-        if (px + 1 >= p_end) v.y = 0.f;
-        if (px + 2 >= p_end) v.z = 0.f;
-        if (px + 3 >= p_end) v.w = 0.f;
-      }
-      tile[(k4 + 0) * LDB + b] = v.x;
-      tile[(k4 + 1) * LDB + b] = v.y;
-      tile[(k4 + 2) * LDB + b] = v.z;
-      tile[(k4 + 3) * LDB + b] = v.w;
+  float4 pre[PER];
+  auto load = [&](std::size_t p) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + THREADS * k;
+      const int b = e / (CH / 4);
+      pre[k] = e < LOADS ? stage_load(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_end)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    __syncthreads();
+  };
+  auto store = [&](double* t) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + THREADS * k;
+      if (e < LOADS) {
+        const int b = e / (CH / 4);
+        const int k4 = (e % (CH / 4)) * 4;
+        t[(k4 + 0) * LDB + b] = pre[k].x;
+        t[(k4 + 1) * LDB + b] = pre[k].y;
+        t[(k4 + 2) * LDB + b] = pre[k].z;
+        t[(k4 + 3) * LDB + b] = pre[k].w;
+      }
+    }
+  };
+
+  load(p_begin);
+  store(tile[0]);
+  __syncthreads();
+  for (int c = 0; c < chunks; ++c) {
+    const bool more = c + 1 < chunks;
+    if (more) load(p_begin + static_cast<std::size_t>(c + 1) * CH);
+    const double* t = tile[c & 1];
 #pragma unroll 2
-    for (int s = 0; s < kChunk / 4; ++s) {
+    for (int s = 0; s < CH / 4; ++s) {
       // v_mfma_f64_16x16x4 operands: lane l supplies row (l % 16) of the
       // 16-band block at pixel (l / 16) of this 4-pixel step, for A and B.
       double op[NB];
-      const double* row = tile + (4 * s + (lane >> 4)) * LDB + (lane & 15);
+      const double* row = t + (4 * s + (lane >> 4)) * LDB + (lane & 15);
 #pragma unroll
       for (int i = 0; i < NB; ++i) op[i] = row[16 * i];
-      mfma_pairs<0, NP, NB>(wave, op, acc);
+      mfma_pairs_f64<0, NP, NB, kWavesPerWg>(wave, op, acc);
     }
+    if (more) store(tile[(c + 1) & 1]);
+    __syncthreads();
   }
 
   // Partial layout: [wg][pair][lane][4], the accumulator registers verbatim.
   double* out = partial + static_cast<std::size_t>(blockIdx.x) * NP * 256;
 #pragma unroll
   for (int m = 0; m < PW; ++m) {
-    const int q = wave + kMomWaves * m;
+    const int q = wave + kWavesPerWg * m;
     if (q < NP) {
       double* dst = out + (static_cast<std::size_t>(q) * 64 + lane) * 4;
       dst[0] = acc[m][0];
@@ -126,292 +185,307 @@ __global__ __launch_bounds__(256) void moments_f64_kernel(const float* __restric
   }
 }
 
+// Mixed form: d = x - shift as FP16 hi + lo in LDS, [band][pixel] with a
+// pitch of CH + 8 halves (16 lanes reading 16 bytes each at rows m = 0..15
+// land on 16 disjoint bank quads).
+template <int NB>
+__global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const float* __restrict__ cube,
+                                                                        std::size_t pixels, int bands,
+                                                                        std::size_t stride,
+                                                                        std::size_t px_per_wg,
+                                                                        const float* __restrict__ shift,
+                                                                        double* __restrict__ partial) {
+  constexpr int CH = f16_chunk<NB>();
+  constexpr int NP = NB * (NB + 1) / 2;
+  constexpr int SPLITS = f16_splits<NB>();
+  constexpr int OWNERS = kWavesPerWg * SPLITS;
+  constexpr int THREADS = kWavesPerWg * 64;
+  constexpr int PW = (NP + OWNERS - 1) / OWNERS;
+  constexpr int PITCH = CH + 8;
+  constexpr int ROWS = NB * 16;
+  constexpr int LOADS = ROWS * (CH / 4);
+  constexpr int PER = (LOADS + THREADS - 1) / THREADS;
+  constexpr int KSTEPS = CH / 32;
+  __shared__ _Float16 hi_t[2][ROWS * PITCH];
+  __shared__ _Float16 lo_t[2][ROWS * PITCH];
+  __shared__ float sh[ROWS];
+
+  for (int b = threadIdx.x; b < ROWS; b += THREADS) sh[b] = b < bands ? shift[b] : 0.f;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  // Workgroup j: pixel range (j / (8 * SPLITS)) * 8 + j % 8, pair half (j / 8) % SPLITS.
+  const unsigned range = (blockIdx.x / (kXcds * SPLITS)) * kXcds + blockIdx.x % kXcds;
+  const int owner = static_cast<int>((blockIdx.x / kXcds) % SPLITS) * kWavesPerWg + wave;
+  f4 acc[PW];
+  d4 tot[PW];
+#pragma unroll
+  for (int m = 0; m < PW; ++m) {
+    acc[m] = f4{0.f, 0.f, 0.f, 0.f};
+    tot[m] = d4{0.0, 0.0, 0.0, 0.0};
+  }
+
+  const std::size_t p_begin = range * px_per_wg;
+  std::size_t p_end = p_begin + px_per_wg;
+  if (p_end > pixels) p_end = pixels;
+  if (p_begin >= p_end) return;
+  const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
+
+  float4 pre[PER];
+  auto load = [&](std::size_t p) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + THREADS * k;
+      const int b = e / (CH / 4);
+      pre[k] = e < LOADS ? stage_load(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_end)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  auto store = [&](int buf, std::size_t p) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + THREADS * k;
+      if (e < LOADS) {
+        const int b = e / (CH / 4);
+        const int k4 = (e % (CH / 4)) * 4;
+        const std::size_t px = p + k4;
+        // Shift real bands only, and only real pixels (padding stays zero).
+        const float s = b < bands ? sh[b] : 0.f;
+        float v[4] = {pre[k].x, pre[k].y, pre[k].z, pre[k].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float d = (px + i < p_end) ? v[i] - s : 0.f;
+          const _Float16 h = static_cast<_Float16>(d);
+          hi_t[buf][b * PITCH + k4 + i] = h;
+          lo_t[buf][b * PITCH + k4 + i] = static_cast<_Float16>(d - static_cast<float>(h));
+        }
+      }
+    }
+  };
+
+  load(p_begin);
+  store(0, p_begin);
+  __syncthreads();
+  for (int c = 0; c < chunks; ++c) {
+    const bool more = c + 1 < chunks;
+    const std::size_t pn = p_begin + static_cast<std::size_t>(c + 1) * CH;
+    if (more) load(pn);
+    const int buf = c & 1;
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+      // Operands: lane l = 16g + m takes eight pixels 32ks + 8g .. +7 of band
+      // row m of every 16-band block (A and B alike, so the pixel order
+      // cancels out).
+      h8 hi[NB], lo[NB];
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int off = (16 * i + (lane & 15)) * PITCH + 32 * ks + 8 * (lane >> 4);
+        hi[i] = *reinterpret_cast<const h8*>(&hi_t[buf][off]);
+        lo[i] = *reinterpret_cast<const h8*>(&lo_t[buf][off]);
+      }
+      mfma_pairs_f16<0, NP, NB, OWNERS>(owner, hi, lo, acc);
+    }
+    // FP32 tiles of 128 pixels, then FP64.
+    if ((c + 1) % (128 / CH) == 0 || !more) {
+#pragma unroll
+      for (int m = 0; m < PW; ++m) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tot[m][r] += static_cast<double>(acc[m][r]);
+        acc[m] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    if (more) store(buf ^ 1, pn);
+    __syncthreads();
+  }
+
+  double* out = partial + static_cast<std::size_t>(range) * NP * 256;
+#pragma unroll
+  for (int m = 0; m < PW; ++m) {
+    const int q = owner + OWNERS * m;
+    if (q < NP) {
+      double* dst = out + (static_cast<std::size_t>(q) * 64 + lane) * 4;
+      dst[0] = tot[m][0];
+      dst[1] = tot[m][1];
+      dst[2] = tot[m][2];
+      dst[3] = tot[m][3];
+    }
+  }
+}
+
 // Fixed-order sum of the per-workgroup partials, then scatter of each
-// accumulator element to its matrix entry. v_mfma_f64_16x16x4 leaves element
-// (row l/16 + 4r, column l%16) of the 16x16 block in lane l, register r
-// (rows interleaved across the four lane groups).
-__global__ void moments_reduce_kernel(const double* __restrict__ partial, int wgs, int nb,
-                                      int bands, double* __restrict__ count,
-                                      double* __restrict__ sum, double* __restrict__ sum_outer) {
+// accumulator element to its matrix entry. A block of 256 threads owns 16
+// consecutive elements: thread t sums partials w = t/16, t/16 + 16, ... of
+// element t%16 (16 consecutive doubles per group, so loads coalesce), and the
+// 16 group sums are added in group order - deterministic for a given
+// workgroup count. v_mfma_f64_16x16x4 leaves (row l/16 + 4r, column l%16) of
+// a block in lane l, register r (rows interleaved); v_mfma_f32_16x16x32_f16
+// leaves row 4*(l/16) + r (blocked).
+constexpr int kRedElems = 16;
+constexpr int kRedGroups = 16;
+
+__global__ __launch_bounds__(kRedElems * kRedGroups) void moments_reduce_kernel(
+    const double* __restrict__ partial, int wgs, int nb, int bands, int interleaved,
+    double* __restrict__ count, double* __restrict__ sum, double* __restrict__ sum_outer) {
+  __shared__ double part[kRedGroups][kRedElems];
   const int np = nb * (nb + 1) / 2;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= np * 256) return;
+  const int local = threadIdx.x % kRedElems;
+  const int group = threadIdx.x / kRedElems;
+  const int e = blockIdx.x * kRedElems + local;
   double s = 0.0;
-  for (int w = 0; w < wgs; ++w) s += partial[static_cast<std::size_t>(w) * np * 256 + e];
+  if (e < np * 256)
+    for (int w = group; w < wgs; w += kRedGroups) s += partial[static_cast<std::size_t>(w) * np * 256 + e];
+  part[group][local] = s;
+  __syncthreads();
+  if (group != 0 || e >= np * 256) return;
+  double total = 0.0;
+  for (int g = 0; g < kRedGroups; ++g) total += part[g][local];
   const int q = e / 256;
   const int lane = (e % 256) / 4;
   const int r = e % 4;
-  const int bi = pair_row(q);
-  const int bj = pair_col(q);
-  const int row = 16 * bi + (lane >> 4) + 4 * r;
-  const int col = 16 * bj + (lane & 15);
+  const int row = 16 * pair_row(q) + (interleaved ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
+  const int col = 16 * pair_col(q) + (lane & 15);
   if (row < bands) {
-    if (col <= row) sum_outer[static_cast<std::size_t>(row) * bands + col] = s;
+    if (col <= row) sum_outer[static_cast<std::size_t>(row) * bands + col] = total;
   } else if (row == bands) {
-    if (col < bands) sum[col] = s;
-    else if (col == bands) *count = s;
+    if (col < bands) sum[col] = total;
+    else if (col == bands) *count = total;
   }
+}
+
+// Reference point for the mixed form: the FP64 mean of a sparse sample of the
+// frame (at most 64 runs of 64 contiguous pixels), one block per band. Any
+// point near the mean protects the covariance from cancellation; the exact
+// moments are rebuilt from it afterwards.
+__global__ __launch_bounds__(256) void shift_kernel(const float* __restrict__ cube, std::size_t pixels,
+                                                    std::size_t stride, std::size_t run_stride,
+                                                    float* __restrict__ shift,
+                                                    double* __restrict__ shift_d) {
+  __shared__ double red[256];
+  __shared__ double cnt[256];
+  const int b = blockIdx.x;
+  const std::size_t runs = div_up(pixels, run_stride);
+  const std::size_t samples = runs * 64;
+  double s = 0.0, c = 0.0;
+  for (std::size_t i = threadIdx.x; i < samples; i += 256) {
+    const std::size_t p = (i / 64) * run_stride + (i % 64);
+    if (p < pixels) {
+      s += cube[static_cast<std::size_t>(b) * stride + p];
+      c += 1.0;
+    }
+  }
+  red[threadIdx.x] = s;
+  cnt[threadIdx.x] = c;
+  __syncthreads();
+  for (int span = 128; span > 0; span >>= 1) {
+    if (threadIdx.x < span) {
+      red[threadIdx.x] += red[threadIdx.x + span];
+      cnt[threadIdx.x] += cnt[threadIdx.x + span];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float r = cnt[0] > 0.0 ? static_cast<float>(red[0] / cnt[0]) : 0.f;
+    shift[b] = r;
+    shift_d[b] = static_cast<double>(r);
+  }
+}
+
+// Raw moments from shifted ones (moments_from_shifted, linalg.cpp:81-103):
+// M1 = S1 + N r, M2 = S2 + r S1^T + S1 r^T + N r r^T, lower triangle. M2 is
+// rebuilt first (it reads S1), M1 by a second launch.
+__global__ void unshift_kernel(int bands, const double* __restrict__ r, const double* __restrict__ count,
+                               const double* __restrict__ sum, double* __restrict__ sum_outer) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tri = bands * (bands + 1) / 2;
+  if (e >= tri) return;
+  int i = 0;
+  while ((i + 1) * (i + 2) / 2 <= e) ++i;
+  const int j = e - i * (i + 1) / 2;
+  const double n = *count;
+  double* m2 = sum_outer + static_cast<std::size_t>(i) * bands + j;
+  *m2 = *m2 + r[i] * sum[j] + sum[i] * r[j] + n * r[i] * r[j];
+}
+
+__global__ void unshift_sum_kernel(int bands, const double* __restrict__ r,
+                                   const double* __restrict__ count, double* __restrict__ sum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < bands) sum[i] = sum[i] + *count * r[i];
+}
+
+std::size_t mom_wgs(std::size_t pixels) {
+  const std::size_t chunks = div_up(pixels, kChunk);
+  return chunks < kMaxMomWgs ? chunks : kMaxMomWgs;
 }
 
 template <int NB>
 hipError_t launch_moments_nb(const float* cube, std::size_t pixels, int bands, std::size_t stride,
-                             double* scratch, double* count, double* sum, double* sum_outer,
-                             hipStream_t stream) {
+                             hsi_precision precision, double* scratch, double* count, double* sum,
+                             double* sum_outer, hipStream_t stream) {
   const std::size_t chunks = div_up(pixels, kChunk);
-  const std::size_t wgs = chunks < kMaxMomWgs ? chunks : kMaxMomWgs;
+  const std::size_t wgs = mom_wgs(pixels);
   const std::size_t px_per_wg = div_up(chunks, wgs) * kChunk;
   const std::size_t used = div_up(pixels, px_per_wg);
-  hipLaunchKernelGGL(moments_f64_kernel<NB>, dim3(static_cast<unsigned>(used)), dim3(256), 0,
-                     stream, cube, pixels, bands, stride, px_per_wg, scratch);
-  hipError_t err = hipGetLastError();
-  if (err != hipSuccess) return err;
   const int np = NB * (NB + 1) / 2;
-  const unsigned rblocks = static_cast<unsigned>(div_up(static_cast<std::size_t>(np) * 256, 256));
-  hipLaunchKernelGGL(moments_reduce_kernel, dim3(rblocks), dim3(256), 0, stream, scratch,
-                     static_cast<int>(used), NB, bands, count, sum, sum_outer);
+  double* partial = scratch;
+  hipError_t err;
+  if (precision == HSI_MIXED) {
+    // Shift vectors live after the partials.
+    double* shift_d = scratch + wgs * np * 256;
+    float* shift = reinterpret_cast<float*>(shift_d + kMaxBands);
+    const std::size_t run_stride = pixels > 64 * 64 ? div_up(pixels, 64) : 64;
+    hipLaunchKernelGGL(shift_kernel, dim3(bands), dim3(256), 0, stream, cube, pixels, stride,
+                       run_stride, shift, shift_d);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    // Ranges rounded up to whole groups of 8 so the range/split decode is
+    // exact; ranges past the pixel count return at once and write nothing.
+    const std::size_t ranges = f16_splits<NB>() > 1 ? round_up(used, kXcds) : used;
+    hipLaunchKernelGGL(moments_mixed_kernel<NB>, dim3(static_cast<unsigned>(ranges * f16_splits<NB>())),
+                       dim3(kWavesPerWg * 64), 0,
+                       stream, cube, pixels, bands, stride, px_per_wg, shift, partial);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    hipLaunchKernelGGL(moments_reduce_kernel, dim3(static_cast<unsigned>(np * 256 / kRedElems)),
+                       dim3(kRedElems * kRedGroups), 0, stream, partial, static_cast<int>(used), NB,
+                       bands, 0, count, sum, sum_outer);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    const int tri = bands * (bands + 1) / 2;
+    hipLaunchKernelGGL(unshift_kernel, dim3(static_cast<unsigned>(div_up(tri, 256))), dim3(256), 0,
+                       stream, bands, shift_d, count, sum, sum_outer);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    hipLaunchKernelGGL(unshift_sum_kernel, dim3(static_cast<unsigned>(div_up(bands, 256))), dim3(256),
+                       0, stream, bands, shift_d, count, sum);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(moments_f64_kernel<NB>, dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64), 0,
+                     stream,
+                     cube, pixels, bands, stride, px_per_wg, partial);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  hipLaunchKernelGGL(moments_reduce_kernel, dim3(static_cast<unsigned>(np * 256 / kRedElems)),
+                     dim3(kRedElems * kRedGroups), 0, stream, partial, static_cast<int>(used), NB,
+                     bands, 1, count, sum, sum_outer);
   return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// Build.
-
-constexpr int kBuildThreads = 1024;
-constexpr int kLdsBands = 128;
-constexpr int kLdsPacked = kLdsBands * (kLdsBands + 1) / 2;
-
-__device__ __forceinline__ int packed(int i, int k) { return i * (i + 1) / 2 + k; }
-
-// Row of the packed index e within a lower triangle (largest r with
-// r(r+1)/2 <= e), exact after the float estimate's correction.
-__device__ __forceinline__ int tri_row(int e) {
-  int r = static_cast<int>((sqrtf(8.0f * static_cast<float>(e) + 1.0f) - 1.0f) * 0.5f);
-  while ((r + 1) * (r + 2) / 2 <= e) ++r;
-  while (r > 0 && r * (r + 1) / 2 > e) --r;
-  return r;
-}
-
-// Right-looking Cholesky of a packed lower matrix, in place. Column j is
-// finished from the diagonal, then the trailing triangle takes its rank-1
-// update. Returns false (uniformly across the block) if a pivot is not
-// positive - the test cholesky() makes (linalg.cpp:113), which also catches
-// NaN - or is below 4*n*eps of that diagonal's starting value: an exactly
-// singular matrix cancels to a pivot of +-1 ulp, whose sign depends on the
-// summation order of the moments, and a whitener built on such a pivot is
-// meaningless once narrowed to float.
-__device__ bool block_cholesky(double* a, int n, double* col, double* diag0) {
-  const int tid = threadIdx.x;
-  const double tol = 4.0 * n * 2.220446049250313e-16;
-  for (int i = tid; i < n; i += kBuildThreads) diag0[i] = a[packed(i, i)];
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    const double pivot = a[packed(j, j)];
-    if (!(pivot > 0.0) || !(pivot > tol * diag0[j])) return false;
-    const double d = sqrt(pivot);
-    for (int i = j + 1 + tid; i < n; i += kBuildThreads) {
-      const double v = a[packed(i, j)] / d;
-      a[packed(i, j)] = v;
-      col[i] = v;
-    }
-    __syncthreads();
-    if (tid == 0) a[packed(j, j)] = d;
-    const int s = n - j - 1;
-    const int work = s * (s + 1) / 2;
-    for (int e = tid; e < work; e += kBuildThreads) {
-      const int r = tri_row(e);
-      const int c = e - r * (r + 1) / 2;
-      const int i = j + 1 + r;
-      const int k = j + 1 + c;
-      a[packed(i, k)] -= col[i] * col[k];
-    }
-    __syncthreads();
-  }
-  return true;
-}
-
-// X = L^-1 for packed lower L, right-looking over rows: once row j of X is
-// final, every later row subtracts L[i][j] * X[j][k].
-__device__ void block_tri_inverse(const double* l, double* x, int n) {
-  const int tid = threadIdx.x;
-  const int total = n * (n + 1) / 2;
-  for (int e = tid; e < total; e += kBuildThreads) x[e] = 0.0;
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    const double d = l[packed(j, j)];
-    for (int k = tid; k <= j; k += kBuildThreads) {
-      x[packed(j, k)] = (k == j) ? 1.0 / d : x[packed(j, k)] / d;
-    }
-    __syncthreads();
-    const int rows = n - j - 1;
-    const int cols = j + 1;
-    const int work = rows * cols;
-    for (int e = tid; e < work; e += kBuildThreads) {
-      const int i = j + 1 + e / cols;
-      const int k = e % cols;
-      x[packed(i, k)] -= l[packed(i, j)] * x[packed(j, k)];
-    }
-    __syncthreads();
-  }
-}
-
-template <bool IN_LDS>
-__global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
-  __shared__ double lds_mat[IN_LDS ? 2 * kLdsPacked : 1];
-  __shared__ double mu[kMaxBands];
-  __shared__ double col[kMaxBands];
-  __shared__ double diag0[kMaxBands];
-  __shared__ double red[kBuildThreads / kWave];
-  const int n = a.bands;
-  const int tid = threadIdx.x;
-  const int tri = n * (n + 1) / 2;
-  double* m1 = IN_LDS ? lds_mat : a.work;
-  double* m2 = IN_LDS ? lds_mat + kLdsPacked : a.work + tri;
-  double* vec = a.work + 2 * tri;  // [T][n] scratch for the target passes
-
-  const double count = *a.count;
-  if (!(count > 0.0)) {
-    if (tid == 0) *a.status = HSI_STATUS_COV_NOT_PD;
-    return;
-  }
-  for (int i = tid; i < n; i += kBuildThreads) mu[i] = a.sum[i] / count;
-  __syncthreads();
-
-  // covariance(loading): correlation(0) - mu mu^T, loading on the diagonal.
-  for (int e = tid; e < tri; e += kBuildThreads) {
-    const int i = tri_row(e);
-    const int j = e - i * (i + 1) / 2;
-    double v = a.sum_outer[static_cast<std::size_t>(i) * n + j] / count;
-    v = v - mu[i] * mu[j];
-    if (i == j) v += a.loading;
-    m1[e] = v;
-  }
-  __syncthreads();
-  if (!block_cholesky(m1, n, col, diag0)) {
-    if (tid == 0) *a.status = HSI_STATUS_COV_NOT_PD;
-    return;
-  }
-  block_tri_inverse(m1, m2, n);
-
-  for (int i = tid; i < n; i += kBuildThreads) a.mean[i] = static_cast<float>(mu[i]);
-  for (int e = tid; e < n * n; e += kBuildThreads) {
-    const int i = e / n;
-    const int k = e % n;
-    a.whitener[e] = (k <= i) ? static_cast<float>(m2[packed(i, k)]) : 0.0f;
-  }
-
-  const int T = a.num_targets;
-  if (T > 0) {
-    // ACE: u_t = L^-1 (d_t - mu), ||u_t||^2.
-    for (int e = tid; e < T * n; e += kBuildThreads) {
-      const int t = e / n;
-      const int i = e % n;
-      const float* d = a.targets + static_cast<std::size_t>(t) * n;
-      double s = 0.0;
-      for (int j = 0; j <= i; ++j) s += m2[packed(i, j)] * (static_cast<double>(d[j]) - mu[j]);
-      vec[e] = s;
-      a.target_white[e] = static_cast<float>(s);
-    }
-    __syncthreads();
-    for (int t = 0; t < T; ++t) {
-      double s = 0.0;
-      for (int i = tid; i < n; i += kBuildThreads) s += vec[t * n + i] * vec[t * n + i];
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, kWave);
-      if ((tid & 63) == 0) red[tid >> 6] = s;
-      __syncthreads();
-      if (tid == 0) {
-        double tot = 0.0;
-        for (int w = 0; w < kBuildThreads / kWave; ++w) tot += red[w];
-        a.target_norm2[t] = static_cast<float>(tot);
-      }
-      __syncthreads();
-    }
-
-    // CEM: correlation(loading) -> Cholesky -> L_R^-1.
-    for (int e = tid; e < tri; e += kBuildThreads) {
-      const int i = tri_row(e);
-      const int j = e - i * (i + 1) / 2;
-      double v = a.sum_outer[static_cast<std::size_t>(i) * n + j] / count;
-      if (i == j) v += a.loading;
-      m1[e] = v;
-    }
-    __syncthreads();
-    if (!block_cholesky(m1, n, col, diag0)) {
-      if (tid == 0) *a.status = HSI_STATUS_CORR_NOT_PD;
-      return;
-    }
-    block_tri_inverse(m1, m2, n);
-    // y_t = L_R^-1 d_t
-    for (int e = tid; e < T * n; e += kBuildThreads) {
-      const int t = e / n;
-      const int i = e % n;
-      const float* d = a.targets + static_cast<std::size_t>(t) * n;
-      double s = 0.0;
-      for (int j = 0; j <= i; ++j) s += m2[packed(i, j)] * static_cast<double>(d[j]);
-      vec[e] = s;
-    }
-    __syncthreads();
-    // R^-1 d_t = L_R^-T y_t, kept in the (now free) first matrix slot.
-    double* solved = m1;
-    for (int e = tid; e < T * n; e += kBuildThreads) {
-      const int t = e / n;
-      const int i = e % n;
-      double s = 0.0;
-      for (int k = i; k < n; ++k) s += m2[packed(k, i)] * vec[t * n + k];
-      solved[e] = s;
-    }
-    __syncthreads();
-    for (int t = 0; t < T; ++t) {
-      const float* d = a.targets + static_cast<std::size_t>(t) * n;
-      double s = 0.0;
-      for (int i = tid; i < n; i += kBuildThreads) s += static_cast<double>(d[i]) * solved[t * n + i];
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, kWave);
-      if ((tid & 63) == 0) red[tid >> 6] = s;
-      __syncthreads();
-      double gain = 0.0;
-      for (int w = 0; w < kBuildThreads / kWave; ++w) gain += red[w];
-      for (int i = tid; i < n; i += kBuildThreads) {
-        a.cem_weight[static_cast<std::size_t>(t) * n + i] =
-            gain > 0.0 ? static_cast<float>(solved[t * n + i] / gain) : 0.0f;
-      }
-      __syncthreads();
-    }
-  }
-  if (tid == 0) *a.status = 0u;
 }
 
 }  // namespace
 
 std::size_t moments_scratch_doubles(std::size_t pixels, int bands) {
-  const std::size_t chunks = div_up(pixels, kChunk);
-  const std::size_t wgs = chunks < kMaxMomWgs ? chunks : kMaxMomWgs;
-  return wgs * static_cast<std::size_t>(moment_pairs(bands)) * 256;
+  // partials, then a double and a float shift vector for the mixed form
+  return mom_wgs(pixels) * static_cast<std::size_t>(moment_pairs(bands)) * 256 + 2 * kMaxBands;
 }
 
 hipError_t launch_moments(const float* cube, std::size_t pixels, int bands, std::size_t stride,
                           hsi_precision precision, double* scratch, double* count, double* sum,
                           double* sum_outer, hipStream_t stream) {
-  (void)precision;
   switch (moment_blocks(bands)) {
-#define HSI_NB(N) \
-  case N: return launch_moments_nb<N>(cube, pixels, bands, stride, scratch, count, sum, sum_outer, stream);
+#define HSI_NB(N)                                                                                  \
+  case N:                                                                                          \
+    return launch_moments_nb<N>(cube, pixels, bands, stride, precision, scratch, count, sum,       \
+                                sum_outer, stream);
     HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8) HSI_NB(9)
     HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14) HSI_NB(15) HSI_NB(16) HSI_NB(17)
 #undef HSI_NB
   }
   return hipErrorInvalidValue;
-}
-
-std::size_t build_work_doubles(int bands, int num_targets) {
-  const std::size_t tri = static_cast<std::size_t>(bands) * (bands + 1) / 2;
-  return 2 * tri + static_cast<std::size_t>(num_targets > 0 ? num_targets : 1) * bands;
-}
-
-hipError_t launch_build(const BuildArgs& a, hipStream_t stream) {
-  if (a.bands <= kLdsBands) {
-    hipLaunchKernelGGL(build_kernel<true>, dim3(1), dim3(kBuildThreads), 0, stream, a);
-  } else {
-    hipLaunchKernelGGL(build_kernel<false>, dim3(1), dim3(kBuildThreads), 0, stream, a);
-  }
-  return hipGetLastError();
 }
 
 }  // namespace hsi_amd

@@ -317,3 +317,32 @@ def test_mixed_rx_ace_against_fp64_oracle(H, torch, ctx, oracle_lib, name, dtype
     assert e_rx <= MIXED_REL and e_aux <= MIXED_REL and e_ace <= MIXED_REL
     agree = (ace["target"].cpu().numpy() == want_t).mean()
     assert agree >= 0.99
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_mixed_moments_against_oracle(H, torch, ctx, name):
+    """FP16 hi/lo products, FP32 tiles, FP64 across tiles: the moments and the
+    scores built on them stay within the precision form's bar."""
+    g = golden(name)
+    cube, b, h, w = load_scene(H, torch, g)
+    p = h * w
+    ctx.set_library(g["library"], g["norms"])
+    ctx.stats_accumulate(cube, h, w, b, precision=H.MIXED)
+    ctx.stats_build(float(g["loading"]))
+    ctx.stats_check()
+    st = ctx.stats_read(b, g["library"].shape[0])
+    assert st["count"][0] == g["m_count"]
+    lower = np.tril_indices(b)
+    e_sum = rel_err(st["sum"], g["m_sum"], 1e-30).max()
+    e_outer = rel_err(st["sum_outer"][lower], g["m_outer"][lower], 1e-30).max()
+    scores = {}
+    for det in (H.Detector.CEM, H.Detector.RX, H.Detector.ACE):
+        buf = device_buffers(torch, p)
+        ctx.score(det, cube, h, w, b, buf["score"], buf["target"], buf["aux"])
+        scores[det] = buf["score"]
+    torch.cuda.synchronize()
+    errs = {k: rel_err(scores[d].cpu().numpy(), g[f"{k}_score"], FLOOR[k]).max()
+            for k, d in (("cem", H.Detector.CEM), ("rx", H.Detector.RX), ("ace", H.Detector.ACE))}
+    print(f"{name}: mixed moments sum {e_sum:.3g} outer {e_outer:.3g}; scores {errs}")
+    assert e_sum <= 1e-6 and e_outer <= 1e-6
+    assert max(errs.values()) <= MIXED_REL
