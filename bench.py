@@ -5,10 +5,13 @@ Workload (the plain-FP32 video configuration): frames of 1024x1024 pixels x
 113 bands, BSQ fp32, drawn by the seeded generator (Dirichlet mix of 8
 endmembers, per-frame drift of the mix, 8-target library planted at 0.5%
 each, gain U(0.3,1.2), noise N(0,0.005)). One step = one frame through the
-whole per-frame path, with the frame already resident in HBM:
-    moments (FP64, matrix cores) -> statistics build (FP64, one workgroup)
-    -> SAM + CEM from one cube read -> threshold+NMS+compact for each.
-`value` is frames x pixels / wall time of the timed steps (Mpix/s).
+whole per-frame path, with the frame already resident in HBM, on two streams:
+    stream A: moments (FP64 matrix cores) -> statistics build (FP64)
+              -> CEM -> threshold+NMS+compact
+    stream B: SAM -> threshold+NMS+compact
+SAM needs no scene statistics, so it streams the cube through HBM while the
+moments keep the matrix cores busy. `value` is frames x pixels / wall time of
+the timed steps (Mpix/s).
 
 Per-kernel rates come from HIP events on the launch stream, averaged over
 repeats, and the roofline entry is for the kernel that takes the largest
@@ -54,8 +57,24 @@ def parse():
     ap.add_argument("--frames", type=int, default=8, help="distinct resident frames cycled")
     ap.add_argument("--repeats", type=int, default=20, help="per-kernel timing repeats")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-rows", type=int, default=64, help="rows of a frame in the CPU sample")
+    ap.add_argument("--cpu-rows", type=int, default=1024, help="rows of a frame in the CPU sample")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB buffer 5 times (reference kernel for HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibrate(mib):
+    """A device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    readings give the factor that turns those counters into bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for i in range(5):
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy {i}: {mib} MiB read, {mib} MiB written")
+    print(json.dumps({"copy_mib": mib, "copies": 5}), flush=True)
 
 
 def time_on_stream(torch, fn, repeats):
@@ -90,6 +109,12 @@ def cpu_baseline(host_frame, library, rows):
                                              C.c_double, f32, f32, f32, f32, f32, C.c_char_p, C.c_int]
         ref.ref_cem_best.argtypes = [f32, C.c_int, C.c_int, C.c_int, f32, f32, f32, f32, f32,
                                      C.c_int, f32, i32]
+        ref.ref_detect.restype = C.c_int64
+        ref.ref_detect.argtypes = [f32, i32, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, i32,
+                                   f32, C.c_int64]
+        cap = 4096
+        det_xyt = np.empty(3 * cap, np.int32)
+        det_score = np.empty(cap, np.float32)
         norms = np.empty(T_, np.float32)
         ref.ref_library_norms(library, T_, B_, norms)
         mean = np.empty(B_, np.float32)
@@ -105,8 +130,10 @@ def cpu_baseline(host_frame, library, rows):
                                       tn, cw, err, 256)
         t1 = time.perf_counter()
         ref.ref_sam_best(sub, rows, W_, B_, library, norms, T_, score, target)
+        ref.ref_detect(score, target, rows, W_, SAM_THRESHOLD, 2, 0, det_xyt, det_score, cap)
         t2 = time.perf_counter()
         ref.ref_cem_best(sub, rows, W_, B_, mean, white, tw, tn, cw, T_, score, target)
+        ref.ref_detect(score, target, rows, W_, CEM_THRESHOLD, 2, 1, det_xyt, det_score, cap)
         t3 = time.perf_counter()
         if not ok:
             raise RuntimeError(err.value.decode())
@@ -117,9 +144,11 @@ def cpu_baseline(host_frame, library, rows):
         count, s, so = oracle.moments(sub)
         st = oracle.build_statistics(count, s, so, library, LOADING)
         t1 = time.perf_counter()
-        oracle.sam_best(sub, library)
+        score, target = oracle.sam_best(sub, library)
+        oracle.detect(score, target, rows, W_, SAM_THRESHOLD, 2, False, capacity=4096)
         t2 = time.perf_counter()
-        oracle.cem_best(sub, st)
+        score, target = oracle.cem_best(sub, st)
+        oracle.detect(score, target, rows, W_, CEM_THRESHOLD, 2, True, capacity=4096)
         t3 = time.perf_counter()
     total = t3 - t0
     return {
@@ -128,7 +157,8 @@ def cpu_baseline(host_frame, library, rows):
         "cores": 1,
         "kind": kind,
         "sample": f"rows 0-{rows - 1} of frame 0 ({rows}x{W_}x{B_}, {T_} targets): moments+"
-                  f"build_statistics, sam_best_cpu, cem_best_cpu, single thread",
+                  f"build_statistics, sam_best_cpu+detect_cpu, cem_best_cpu+detect_cpu, "
+                  f"single thread (the reference scorers are serial)",
         "seconds": round(total, 3),
         "per_detector_mpix_s": {"stats": round(sample_px / (t1 - t0) / 1e6, 4),
                                 "sam": round(sample_px / (t2 - t1) / 1e6, 4),
@@ -138,6 +168,9 @@ def cpu_baseline(host_frame, library, rows):
 
 def main():
     args = parse()
+    if args.copy_calibrate:
+        copy_calibrate(args.copy_calibrate)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has a single MI355X, so "
               f"--gpus {args.gpus} is not supported (the path would shard whole frames across "
@@ -165,6 +198,13 @@ def main():
     ctx = H.Context(0)
     ctx.set_library(library)
     ctx.reserve(pixels, B_)
+    # SAM runs on its own stream; it gets its own context because a context's
+    # scratch must not be shared by launches that overlap.
+    ctx_sam = H.Context(0)
+    ctx_sam.set_library(library)
+    ctx_sam.reserve(pixels, B_)
+    side = torch.cuda.Stream()
+    main_stream = torch.cuda.current_stream()
     f32 = dict(dtype=torch.float32, device="cuda")
     i32 = dict(dtype=torch.int32, device="cuda")
     sam_s, sam_t = torch.empty(pixels, **f32), torch.empty(pixels, **i32)
@@ -183,11 +223,16 @@ def main():
 
     def step(i):
         cube = frames[i % len(frames)]
+        # stream B may start only once stream A's previous frame is done with
+        # the shared score buffers of the previous step
+        side.wait_stream(main_stream)
+        ctx_sam.score(H.Detector.SAM, cube, H_, W_, B_, sam_s, sam_t, stream=side)
+        ctx_sam.detect(sam_s, sam_t, H_, W_, p_sam, sam_d, sam_n, stream=side)
         ctx.stats_accumulate(cube, H_, W_, B_)
         ctx.stats_build(LOADING)
-        ctx.score_sam_cem(cube, H_, W_, B_, sam_s, sam_t, cem_s, cem_t)
-        ctx.detect(sam_s, sam_t, H_, W_, p_sam, sam_d, sam_n)
+        ctx.score(H.Detector.CEM, cube, H_, W_, B_, cem_s, cem_t)
         ctx.detect(cem_s, cem_t, H_, W_, p_cem, cem_d, cem_n)
+        main_stream.wait_stream(side)
 
     step(0)
     ctx.stats_check()
@@ -284,7 +329,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 in, f64 accumulate",
         "data": "synthetic (seeded generator, frames resident in HBM)",
-        "config": {"workload": "plain FP32 video frame: moments+build+SAM+CEM+detect",
+        "config": {"workload": "plain FP32 video frame: moments+build+CEM+detect || SAM+detect",
                    "height": H_, "width": W_, "bands": B_, "targets": T_,
                    "resident_frames": args.frames, "loading": LOADING},
         "roofline": roof,

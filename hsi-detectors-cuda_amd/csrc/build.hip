@@ -1,7 +1,8 @@
 // build.hip - DetectorStatistics from the moments, on the device in FP64.
 //
-// One workgroup of 8 waves turns (count, sum, sum x x^T) into what the
-// scorers read, as build_statistics does (detector.cpp:47-152):
+// Two workgroups of 8 waves turn (count, sum, sum x x^T) into what the
+// scorers read, as build_statistics does (detector.cpp:47-152) - one for the
+// covariance half, one for the independent autocorrelation half:
 //   mean = sum / N;
 //   S = M2/N - mean mean^T + loading I   (covariance, linalg.cpp:66-79)
 //   R = M2/N + loading I                 (autocorrelation, linalg.cpp:52-64)
@@ -287,16 +288,26 @@ __device__ __attribute__((noinline)) void tile_matvec(Mat<P> M, int nt, int n, i
     d4 acc = {0.0, 0.0, 0.0, 0.0};
     const int j0 = TRANS ? I : 0;
     const int j1 = TRANS ? nt : I + 1;
-    for (int J = j0; J < j1; ++J) {
-      const P* tile = TRANS ? M.tile(J, I) : M.tile(I, J);
-      double bv[4];
+    // Four column tiles at a time: their B values are loaded together, so
+    // the global-memory latency is paid once per four tile products.
+    for (int J0 = j0; J0 < j1; J0 += 4) {
+      double bv[4][4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int j = 16 * J + g + 4 * s;
-        bv[s] = (t < T && j < n) ? v_at(t, j) : 0.0;
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int j = 16 * (J0 + q) + g + 4 * s;
+          bv[q][s] = (J0 + q < j1 && t < T && j < n) ? v_at(t, j) : 0.0;
+        }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (J0 + q < j1) {
+          const P* tile = TRANS ? M.tile(J0 + q, I) : M.tile(I, J0 + q);
+#pragma unroll
+          for (int s = 0; s < 4; ++s)
+            acc = mfma(TRANS ? op_t(tile, lane, s) : op(tile, lane, s), bv[q][s], acc);
+        }
       }
-#pragma unroll
-      for (int s = 0; s < 4; ++s) acc = mfma(TRANS ? op_t(tile, lane, s) : op(tile, lane, s), bv[s], acc);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -318,63 +329,62 @@ __device__ __forceinline__ double wave_sum(int n, F f, int t) {
 
 typedef __attribute__((address_space(3))) int lds_int;
 
+// One of the two independent halves of build_statistics, each on its own
+// workgroup (CU): path 0 factors the covariance (mean, whitener, whitened
+// targets and their norms - RX and ACE); path 1 factors the autocorrelation
+// (CEM filters). Each writes its own status slot.
 template <typename P>
-__device__ __forceinline__ void build_body(const BuildArgs& a, Mat<P> A, Mat<P> X, lds_double* mu,
-                                           lds_double* diag0, lds_int* flag) {
+__device__ __forceinline__ void build_path(const BuildArgs& a, int path, Mat<P> A, Mat<P> X,
+                                           lds_double* mu, lds_double* diag0, lds_int* flag,
+                                           double* vec, double* solved) {
   const int n = a.bands;
   const int nt = (n + kTile - 1) / kTile;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const int blocks = nt * (nt + 1) / 2;
-  double* vec = a.work + 2 * static_cast<std::size_t>(blocks) * kTileElems;  // [T][n]
-  double* solved = vec + static_cast<std::size_t>(kMaxTargets) * kMaxBands;  // [T][n]
-
+  const int T = a.num_targets;
+  unsigned* status = a.status + path;
   const double count = *a.count;
   if (!(count > 0.0)) {
-    if (tid == 0) *a.status = HSI_STATUS_COV_NOT_PD;
+    if (tid == 0) *status = path == 0 ? HSI_STATUS_COV_NOT_PD : HSI_STATUS_CORR_NOT_PD;
+    return;
+  }
+  if (path == 1 && T == 0) {
+    if (tid == 0) *status = 0u;
     return;
   }
   if (tid == 0) *flag = 0;
   for (int i = tid; i < n; i += kThreads) mu[i] = a.sum[i] / count;
   __syncthreads();
-
-  fill(A, n, nt, a.sum_outer, count, mu, a.loading, true, diag0);
+  fill(A, n, nt, a.sum_outer, count, mu, a.loading, path == 0, diag0);
   __syncthreads();
   if (!block_cholesky(A, X, nt, diag0, flag)) {
-    if (tid == 0) *a.status = HSI_STATUS_COV_NOT_PD;
+    if (tid == 0) *status = path == 0 ? HSI_STATUS_COV_NOT_PD : HSI_STATUS_CORR_NOT_PD;
     return;
   }
   block_inverse(A, X, nt);
+  const float* tg = a.targets;
 
-  for (int i = tid; i < n; i += kThreads) a.mean[i] = static_cast<float>(mu[i]);
-  for (int e = tid; e < n * n; e += kThreads) {
-    const int i = e / n, k = e % n;
-    a.whitener[e] = (k <= i) ? static_cast<float>(tile_get(X, i, k)) : 0.0f;
-  }
-  const int T = a.num_targets;
-  if (T > 0) {
-    const float* tg = a.targets;
-    // u_t = W (d_t - mean) with the double whitener, and |u_t|^2.
-    tile_matvec<false>(X, nt, n, T,
-                       [&](int t, int j) { return static_cast<double>(tg[static_cast<std::size_t>(t) * n + j]) - mu[j]; },
-                       vec);
-    __syncthreads();
-    for (int e = tid; e < T * n; e += kThreads) a.target_white[e] = static_cast<float>(vec[e]);
-    for (int t = wave; t < T; t += kWaves) {
-      const double s = wave_sum(n, [&](int tt, int i) { return vec[tt * n + i] * vec[tt * n + i]; }, t);
-      if (lane == 0) a.target_norm2[t] = static_cast<float>(s);
+  if (path == 0) {
+    for (int i = tid; i < n; i += kThreads) a.mean[i] = static_cast<float>(mu[i]);
+    for (int e = tid; e < n * n; e += kThreads) {
+      const int i = e / n, k = e % n;
+      a.whitener[e] = (k <= i) ? static_cast<float>(tile_get(X, i, k)) : 0.0f;
     }
-    __syncthreads();
-
-    // CEM: R = M2/N + loading I -> L_R -> L_R^-1; R^-1 d = L_R^-T (L_R^-1 d).
-    fill(A, n, nt, a.sum_outer, count, mu, a.loading, false, diag0);
-    __syncthreads();
-    if (!block_cholesky(A, X, nt, diag0, flag)) {
-      if (tid == 0) *a.status = HSI_STATUS_CORR_NOT_PD;
-      return;
+    if (T > 0) {
+      // u_t = W (d_t - mean) with the double whitener, and |u_t|^2.
+      tile_matvec<false>(X, nt, n, T,
+                         [&](int t, int j) { return static_cast<double>(tg[static_cast<std::size_t>(t) * n + j]) - mu[j]; },
+                         vec);
+      __syncthreads();
+      for (int e = tid; e < T * n; e += kThreads) a.target_white[e] = static_cast<float>(vec[e]);
+      for (int t = wave; t < T; t += kWaves) {
+        const double s = wave_sum(n, [&](int tt, int i) { return vec[tt * n + i] * vec[tt * n + i]; }, t);
+        if (lane == 0) a.target_norm2[t] = static_cast<float>(s);
+      }
     }
-    block_inverse(A, X, nt);
+  } else {
+    // R^-1 d = L_R^-T (L_R^-1 d); w = R^-1 d / (d^T R^-1 d), zero if the gain is not positive.
     tile_matvec<false>(X, nt, n, T,
                        [&](int t, int j) { return static_cast<double>(tg[static_cast<std::size_t>(t) * n + j]); },
                        vec);
@@ -392,7 +402,7 @@ __device__ __forceinline__ void build_body(const BuildArgs& a, Mat<P> A, Mat<P> 
       }
     }
   }
-  if (tid == 0) *a.status = 0u;
+  if (tid == 0) *status = 0u;
 }
 
 template <bool IN_LDS>
@@ -401,18 +411,22 @@ __global__ __launch_bounds__(kThreads) void build_kernel(BuildArgs a) {
   __shared__ double mu[kMaxBands];
   __shared__ double diag0[kMaxBands];
   __shared__ int flag;
+  const int path = blockIdx.x;
+  const int nt = (a.bands + kTile - 1) / kTile;
+  const std::size_t mat = static_cast<std::size_t>(nt) * (nt + 1) / 2 * kTileElems;
+  // work: [path 0: A, X][path 1: A, X][vec 0][vec 1][solved]
+  double* vec = a.work + 4 * mat + static_cast<std::size_t>(path) * kMaxTargets * kMaxBands;
+  double* solved = a.work + 4 * mat + 2 * static_cast<std::size_t>(kMaxTargets) * kMaxBands;
   lds_double* l_mu = (lds_double*)mu;
   lds_double* l_diag0 = (lds_double*)diag0;
   lds_int* l_flag = (lds_int*)&flag;
   if constexpr (IN_LDS) {
     lds_double* base = (lds_double*)lds;
-    build_body(a, Mat<lds_double>{base}, Mat<lds_double>{base + kLdsBlocks * kTileElems}, l_mu,
-               l_diag0, l_flag);
+    build_path(a, path, Mat<lds_double>{base}, Mat<lds_double>{base + kLdsBlocks * kTileElems}, l_mu,
+               l_diag0, l_flag, vec, solved);
   } else {
-    const int nt = (a.bands + kTile - 1) / kTile;
-    const std::size_t blocks = static_cast<std::size_t>(nt) * (nt + 1) / 2;
-    build_body(a, Mat<double>{a.work}, Mat<double>{a.work + blocks * kTileElems}, l_mu, l_diag0,
-               l_flag);
+    double* base = a.work + 2 * mat * path;
+    build_path(a, path, Mat<double>{base}, Mat<double>{base + mat}, l_mu, l_diag0, l_flag, vec, solved);
   }
 }
 
@@ -421,14 +435,14 @@ __global__ __launch_bounds__(kThreads) void build_kernel(BuildArgs a) {
 std::size_t build_work_doubles(int bands, int num_targets) {
   (void)num_targets;
   const std::size_t nt = (static_cast<std::size_t>(bands) + kTile - 1) / kTile;
-  return 2 * nt * (nt + 1) / 2 * kTileElems + 2 * static_cast<std::size_t>(kMaxTargets) * kMaxBands;
+  return 4 * nt * (nt + 1) / 2 * kTileElems + 3 * static_cast<std::size_t>(kMaxTargets) * kMaxBands;
 }
 
 hipError_t launch_build(const BuildArgs& a, hipStream_t stream) {
   if (a.bands <= kLdsTiles * kTile) {
-    hipLaunchKernelGGL(build_kernel<true>, dim3(1), dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL(build_kernel<true>, dim3(2), dim3(kThreads), 0, stream, a);
   } else {
-    hipLaunchKernelGGL(build_kernel<false>, dim3(1), dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL(build_kernel<false>, dim3(2), dim3(kThreads), 0, stream, a);
   }
   return hipGetLastError();
 }

@@ -22,6 +22,8 @@ struct hsi_ctx {
   float* d_targets = nullptr;  // [kMaxTargets][kMaxBands] capacity, [T][B] used
   float* d_norms = nullptr;    // [kMaxTargets]
   double* d_sam_lib = nullptr; // [kMaxBands][kMaxTargets] capacity
+  double* d_sam_rnorm = nullptr; // [kMaxTargets] 1/norm, for ranking
+  int lib_degenerate = 0;        // some target has norm <= 0
 
   // Moments and statistics, allocated once at the largest supported size.
   int moment_bands = 0;        // bands of the last accumulation, 0 if none
@@ -36,7 +38,7 @@ struct hsi_ctx {
   float* d_target_white = nullptr;
   float* d_target_norm2 = nullptr;
   float* d_cem_weight = nullptr;
-  unsigned* d_status = nullptr;
+  unsigned* d_status = nullptr;      // [2]: covariance half, autocorrelation half
   double* d_build_work = nullptr;
   double* d_cem_lib = nullptr;
   float* d_white_blocks = nullptr;
@@ -142,7 +144,7 @@ int hsi_max_bands(void) { return kMaxBands; }
 void hsi_ctx_destroy(hsi_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  void* ptrs[] = {ctx->d_targets, ctx->d_norms, ctx->d_sam_lib, ctx->d_count, ctx->d_sum,
+  void* ptrs[] = {ctx->d_targets, ctx->d_norms, ctx->d_sam_lib, ctx->d_sam_rnorm, ctx->d_count, ctx->d_sum,
                   ctx->d_sum_outer, ctx->d_mean, ctx->d_whitener, ctx->d_target_white,
                   ctx->d_target_norm2, ctx->d_cem_weight, ctx->d_status, ctx->d_build_work,
                   ctx->d_cem_lib, ctx->d_white_blocks, ctx->d_ace_u, ctx->d_white_hl,
@@ -166,6 +168,7 @@ hsi_status hsi_ctx_create(int device, hsi_ctx** out) {
   if (err == hipSuccess) err = alloc(&ctx->d_targets, T * B);
   if (err == hipSuccess) err = alloc(&ctx->d_norms, T);
   if (err == hipSuccess) err = alloc(&ctx->d_sam_lib, B * T);
+  if (err == hipSuccess) err = alloc(&ctx->d_sam_rnorm, T);
   if (err == hipSuccess) err = alloc(&ctx->d_count, 1);
   if (err == hipSuccess) err = alloc(&ctx->d_sum, B);
   if (err == hipSuccess) err = alloc(&ctx->d_sum_outer, B * B);
@@ -174,7 +177,7 @@ hsi_status hsi_ctx_create(int device, hsi_ctx** out) {
   if (err == hipSuccess) err = alloc(&ctx->d_target_white, T * B);
   if (err == hipSuccess) err = alloc(&ctx->d_target_norm2, T);
   if (err == hipSuccess) err = alloc(&ctx->d_cem_weight, T * B);
-  if (err == hipSuccess) err = alloc(&ctx->d_status, 1);
+  if (err == hipSuccess) err = alloc(&ctx->d_status, 2);
   if (err == hipSuccess) err = alloc(&ctx->d_build_work, build_work_doubles(kMaxBands, kMaxTargets));
   if (err == hipSuccess) err = alloc(&ctx->d_cem_lib, B * T);
   if (err == hipSuccess) err = alloc(&ctx->d_white_blocks, rxace_white_floats(kMaxBands));
@@ -185,7 +188,7 @@ hsi_status hsi_ctx_create(int device, hsi_ctx** out) {
   if (err == hipSuccess) err = alloc(&ctx->d_row_inv, B);
   if (err == hipSuccess) err = alloc(&ctx->d_target_inv, T);
   if (err == hipSuccess) err = hipMemset(ctx->d_sum_outer, 0, B * B * sizeof(double));
-  if (err == hipSuccess) err = hipMemset(ctx->d_status, 0, sizeof(unsigned));
+  if (err == hipSuccess) err = hipMemset(ctx->d_status, 0, 2 * sizeof(unsigned));
   if (err == hipSuccess) err = hipDeviceSynchronize();
   if (err != hipSuccess) {
     hsi_ctx_destroy(ctx);
@@ -249,6 +252,15 @@ hsi_status hsi_set_library(hsi_ctx* ctx, const float* values, const float* norms
           "upload norms");
   HSI_HIP(ctx, hipMemcpy(ctx->d_sam_lib, lib.data(), sizeof(double) * lib.size(), hipMemcpyHostToDevice),
           "upload library layout");
+  std::vector<double> rn(static_cast<std::size_t>(kMaxTargets), 0.0);
+  int degenerate = 0;
+  for (int t = 0; t < num_targets; ++t) {
+    if (n[t] > 0.0f) rn[t] = 1.0 / static_cast<double>(n[t]);
+    else degenerate = 1;
+  }
+  HSI_HIP(ctx, hipMemcpy(ctx->d_sam_rnorm, rn.data(), sizeof(double) * rn.size(), hipMemcpyHostToDevice),
+          "upload reciprocal norms");
+  ctx->lib_degenerate = degenerate;
   ctx->num_targets = num_targets;
   ctx->lib_bands = bands;
   ctx->lib_stride = ls;
@@ -347,9 +359,10 @@ hsi_status hsi_stats_check(hsi_ctx* ctx, hsi_stream stream) {
   if (ctx->stats_bands <= 0) return fail(ctx, HSI_ERR_NO_STATISTICS, "no statistics built");
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
   HSI_HIP(ctx, hipStreamSynchronize(static_cast<hipStream_t>(stream)), "wait for statistics");
-  unsigned status = 0;
-  HSI_HIP(ctx, hipMemcpy(&status, ctx->d_status, sizeof(unsigned), hipMemcpyDeviceToHost),
+  unsigned words[2] = {0u, 0u};
+  HSI_HIP(ctx, hipMemcpy(words, ctx->d_status, sizeof(words), hipMemcpyDeviceToHost),
           "read statistics status");
+  const unsigned status = words[0] | words[1];
   if (status == 0) return HSI_OK;
   char loading[64];
   std::snprintf(loading, sizeof loading, "%f", ctx->last_loading);
@@ -423,7 +436,7 @@ hsi_status hsi_stats_upload(hsi_ctx* ctx, int bands, const float* mean, const fl
     HSI_HIP(ctx, hipMemcpy(ctx->d_cem_weight, cem_weight, t * b * sizeof(float), hipMemcpyHostToDevice),
             "upload cem_weight");
   }
-  HSI_HIP(ctx, hipMemset(ctx->d_status, 0, sizeof(unsigned)), "reset status");
+  HSI_HIP(ctx, hipMemset(ctx->d_status, 0, 2 * sizeof(unsigned)), "reset status");
   hsi_status st = refresh_layouts(ctx, bands, targets, nullptr);
   if (st != HSI_OK) return st;
   HSI_HIP(ctx, hipDeviceSynchronize(), "upload statistics");
@@ -457,6 +470,8 @@ hsi_status hsi_score_sam_cem(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype, 
   a.lib_stride = ctx->lib_stride;
   a.sam_lib = ctx->d_sam_lib;
   a.sam_norms = ctx->d_norms;
+  a.sam_rnorms = ctx->d_sam_rnorm;
+  a.sam_exact_all = ctx->lib_degenerate;
   a.cem_lib = ctx->d_cem_lib;
   a.sam_score = d_sam_score;
   a.sam_target = d_sam_target;

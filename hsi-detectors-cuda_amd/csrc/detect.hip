@@ -23,36 +23,76 @@ constexpr int kScanThreads = 1024;
 
 __device__ __forceinline__ bool better(float a, float b, bool higher) { return higher ? a > b : a < b; }
 
+// Suppression test of detect_cpu's local_best (sam_cpu.cpp:96-112): a
+// neighbour strictly better, or equal with a smaller linear index, suppresses.
+// R > 0 is a compile-time radius: every neighbour load is issued before any
+// comparison, so a candidate pays one memory latency, not (2R+1)^2. R == 0
+// handles any radius with a loop.
+template <int R>
+__device__ __forceinline__ bool local_best(const float* __restrict__ score, std::size_t p, int width,
+                                           int height, int radius, float self, bool higher) {
+  const int x = static_cast<int>(p % static_cast<std::size_t>(width));
+  const int y = static_cast<int>(p / static_cast<std::size_t>(width));
+  if constexpr (R > 0) {
+    constexpr int D = 2 * R + 1;
+    float nb[D][D];
+#pragma unroll
+    for (int dy = -R; dy <= R; ++dy)
+#pragma unroll
+      for (int dx = -R; dx <= R; ++dx) {
+        const int ny = y + dy, nx = x + dx;
+        const bool inside = ny >= 0 && ny < height && nx >= 0 && nx < width;
+        nb[dy + R][dx + R] = inside ? score[static_cast<std::size_t>(ny) * width + nx] : self;
+      }
+    bool ok = true;
+#pragma unroll
+    for (int dy = -R; dy <= R; ++dy)
+#pragma unroll
+      for (int dx = -R; dx <= R; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        const int ny = y + dy, nx = x + dx;
+        const bool inside = ny >= 0 && ny < height && nx >= 0 && nx < width;
+        const float other = nb[dy + R][dx + R];
+        // in raster order a neighbour precedes p iff dy < 0, or dy == 0 and dx < 0
+        const bool before = dy < 0 || (dy == 0 && dx < 0);
+        if (inside && (better(other, self, higher) || (other == self && before))) ok = false;
+      }
+    return ok;
+  } else {
+    for (int dy = -radius; dy <= radius; ++dy) {
+      const int ny = y + dy;
+      if (ny < 0 || ny >= height) continue;
+      for (int dx = -radius; dx <= radius; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        const int nx = x + dx;
+        if (nx < 0 || nx >= width) continue;
+        const std::size_t q = static_cast<std::size_t>(ny) * width + nx;
+        const float other = score[q];
+        if (better(other, self, higher)) return false;
+        if (other == self && q < p) return false;
+      }
+    }
+    return true;
+  }
+}
+
+template <int R>
 __device__ __forceinline__ bool passes(const float* __restrict__ score, std::size_t p, int width,
                                        int height, float threshold, int radius, bool higher) {
   const float self = score[p];
   if (!(higher ? self >= threshold : self <= threshold)) return false;
   if (radius <= 0) return true;
-  const int x = static_cast<int>(p % static_cast<std::size_t>(width));
-  const int y = static_cast<int>(p / static_cast<std::size_t>(width));
-  for (int dy = -radius; dy <= radius; ++dy) {
-    const int ny = y + dy;
-    if (ny < 0 || ny >= height) continue;
-    for (int dx = -radius; dx <= radius; ++dx) {
-      if (dx == 0 && dy == 0) continue;
-      const int nx = x + dx;
-      if (nx < 0 || nx >= width) continue;
-      const std::size_t q = static_cast<std::size_t>(ny) * width + nx;
-      const float other = score[q];
-      if (better(other, self, higher)) return false;
-      if (other == self && q < p) return false;
-    }
-  }
-  return true;
+  return local_best<R>(score, p, width, height, radius, self, higher);
 }
 
+template <int R>
 __global__ __launch_bounds__(kDetBlock) void detect_flag_kernel(
     const float* __restrict__ score, int width, int height, float threshold, int radius,
     int higher, unsigned long long* __restrict__ masks, unsigned* __restrict__ block_counts) {
   __shared__ unsigned wave_counts[kDetBlock / kWave];
   const std::size_t pixels = static_cast<std::size_t>(width) * height;
   const std::size_t p = blockIdx.x * static_cast<std::size_t>(kDetBlock) + threadIdx.x;
-  const bool pass = p < pixels && passes(score, p, width, height, threshold, radius, higher != 0);
+  const bool pass = p < pixels && passes<R>(score, p, width, height, threshold, radius, higher != 0);
   const unsigned long long ballot = __ballot(pass);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -140,9 +180,25 @@ hipError_t launch_detect(const float* score, const int* target, int height, int 
   auto* masks = reinterpret_cast<unsigned long long*>(scratch);
   unsigned* counts = scratch + blocks * (kDetBlock / kWave) * 2;
   unsigned* offsets = counts + blocks;
-  hipLaunchKernelGGL(detect_flag_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kDetBlock), 0,
-                     stream, score, width, height, params.threshold, params.nms_radius,
-                     params.higher_is_better, masks, counts);
+  const int r = params.nms_radius;
+  const dim3 grid(static_cast<unsigned>(blocks)), block(kDetBlock);
+  switch (r) {
+    case 1:
+      hipLaunchKernelGGL(detect_flag_kernel<1>, grid, block, 0, stream, score, width, height,
+                         params.threshold, r, params.higher_is_better, masks, counts);
+      break;
+    case 2:
+      hipLaunchKernelGGL(detect_flag_kernel<2>, grid, block, 0, stream, score, width, height,
+                         params.threshold, r, params.higher_is_better, masks, counts);
+      break;
+    case 3:
+      hipLaunchKernelGGL(detect_flag_kernel<3>, grid, block, 0, stream, score, width, height,
+                         params.threshold, r, params.higher_is_better, masks, counts);
+      break;
+    default:
+      hipLaunchKernelGGL(detect_flag_kernel<0>, grid, block, 0, stream, score, width, height,
+                         params.threshold, r, params.higher_is_better, masks, counts);
+  }
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL(detect_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, counts, offsets,

@@ -20,6 +20,8 @@ struct SamCemArgs {
   int lib_stride = 1;              // sam_cem_lib_stride(num_targets)
   const double* sam_lib = nullptr;  // [bands][lib_stride] targets, zero padded
   const float* sam_norms = nullptr; // [num_targets]
+  const double* sam_rnorms = nullptr; // [lib_stride] 1/norm (0 padded), for ranking only
+  int sam_exact_all = 0;            // 1 if some target has norm <= 0: exact path throughout
   const double* cem_lib = nullptr;  // [bands][lib_stride] CEM filters, zero padded
   float* sam_score = nullptr;
   int* sam_target = nullptr;

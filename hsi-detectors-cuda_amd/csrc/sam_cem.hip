@@ -103,6 +103,7 @@ template <typename In, int NT, int PPT, bool DO_SAM, bool DO_CEM>
 __global__ __launch_bounds__(256) void sam_cem_kernel(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
     const double* __restrict__ slib, const float* __restrict__ snorm,
+    const double* __restrict__ srnorm, int exact_all,
     const double* __restrict__ clib, int num_targets, float* __restrict__ s_score,
     int* __restrict__ s_target, float* __restrict__ c_score, int* __restrict__ c_target) {
   using V = typename VecOf<In, PPT>::type;
@@ -172,10 +173,32 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
       // cosine = dot / (sqrt(|x|^2) * |t|), clamped, as angle_of
       // (sam_cpu.cpp:13-18): correctly rounded sqrt, product and quotient.
       const double root = sqrt(sq[k]);
+      // When every target has a direction and so does this pixel, rank on
+      // c'_t = dot_t * (1/root) * (1/|t|), a few ulps from the unclamped
+      // exact cosine, and take the exact quotient only for the targets that
+      // could still win: those whose clamped c' is within 1e-14 of the best
+      // clamped c'. That window holds every cosine whose acos can round to
+      // the best angle (at most ~4.4e-16 apart). Clamping ties whole groups:
+      // with a float |t| the unclamped cosine can pass 1 by ~1e-7, and every
+      // target past 1 (or, if the best is at -1, every target) scores the
+      // same angle, so the first must win. Otherwise every target goes
+      // through the exact formula.
+      const bool all_exact = exact_all || !(sq[k] > 0.0);
+      const double rroot = 1.0 / root;
+      double approx[NT];
+      double top = -__longlong_as_double(0x7ff0000000000000ll);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        approx[t] = sd[t][k] * rroot * srnorm[t];
+        if (t < num_targets && approx[t] > top) top = approx[t];
+      }
+      const double floor_c = (all_exact || top <= -1.0 + 1e-14)
+                                 ? -__longlong_as_double(0x7ff0000000000000ll)
+                                 : (top < 1.0 ? top : 1.0) - 1e-14;
       SamBest best;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        if (t < num_targets) {
+        if (t < num_targets && (all_exact || approx[t] >= floor_c)) {
           const double norm = static_cast<double>(snorm[t]);
           const bool degenerate = sq[k] <= 0.0 || norm <= 0.0;
           double c = sd[t][k] / (root * norm);
@@ -211,7 +234,8 @@ hipError_t launch_one(const SamCemArgs& a, hipStream_t stream) {
   const unsigned blocks = static_cast<unsigned>(div_up(groups, 256));
   hipLaunchKernelGGL((sam_cem_kernel<In, NT, PPT, S, Cm>), dim3(blocks), dim3(256), 0, stream,
                      static_cast<const In*>(a.cube), a.pixels, a.bands, a.stride, a.sam_lib,
-                     a.sam_norms, a.cem_lib, a.num_targets, a.sam_score, a.sam_target,
+                     a.sam_norms, a.sam_rnorms, a.sam_exact_all, a.cem_lib, a.num_targets,
+                     a.sam_score, a.sam_target,
                      a.cem_score, a.cem_target);
   return hipGetLastError();
 }

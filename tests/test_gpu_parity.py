@@ -107,6 +107,52 @@ def test_sam_edge_cases(H, torch, ctx):
     assert s[0] == np.float32(np.pi / 2)
 
 
+def near_tie_scene(bands, targets, seed):
+    """Library and pixels that crowd the SAM winner: a target repeated at
+    other scales (same direction, cosines a rounding apart), one nudged by a
+    few ulps, pixels equal to or scaled from targets (cosine clamps at 1),
+    negated targets (cosine -1), tiny pixels, and random mixtures."""
+    rng = np.random.default_rng(seed)
+    lib = rng.uniform(0.05, 0.9, (targets, bands)).astype(np.float32)
+    lib[1] = lib[0] * np.float32(3.0)
+    lib[2] = lib[0] * np.float32(0.7)
+    lib[3] = np.nextafter(lib[0], np.float32(2.0))
+    pix = []
+    for t in range(targets):
+        pix += [lib[t], lib[t] * np.float32(0.37), -lib[t], lib[t] * np.float32(1e-18)]
+    pix += list(rng.dirichlet(np.ones(targets), 200).astype(np.float32) @ lib)
+    pix += list(lib[0] + rng.normal(0, 1e-6, (40, bands)).astype(np.float32))
+    cube = np.stack(pix).astype(np.float32)  # [pixels, bands]
+    return np.ascontiguousarray(cube.T), lib
+
+
+@pytest.mark.parametrize("targets", [4, 8, 16])
+def test_sam_ranked_winner_on_near_ties(H, torch, ctx, oracle_lib, targets):
+    """Targets are ranked with reciprocal products and only the contenders take
+    the exact quotient; the winner and angle must still be the reference's."""
+    bands = 113
+    cube2, lib = near_tie_scene(bands, targets, 40 + targets)
+    p = cube2.shape[1]
+    norms = oracle_lib.library_norms(lib)
+    cube = H.to_device_cube(cube2.reshape(bands, 1, p), 1, p)
+    ctx.set_library(lib, norms)
+    buf = device_buffers(torch, p)
+    ctx.score(H.Detector.SAM, cube, 1, p, bands, buf["score"], buf["target"])
+    torch.cuda.synchronize()
+    got_s = buf["score"].cpu().numpy()
+    got_t = buf["target"].cpu().numpy()
+    want_s, want_t = oracle_lib.sam_best(cube2, lib, norms)
+    allv = oracle_lib.sam_all(cube2, lib, norms)
+    assert np.abs(got_s.astype(np.float64) - want_s).max() <= ANGLE_TOL
+    for q in np.nonzero(got_t != want_t)[0]:
+        gap = abs(float(allv[got_t[q], q]) - float(want_s[q]))
+        assert gap <= ANGLE_TOL, f"pixel {q}: target {got_t[q]} vs {want_t[q]}"
+    same = int((got_t == want_t).sum())
+    print(f"near ties, {targets} targets: winner identical {same}/{p}, "
+          f"bit-identical angles {int((got_s.view(np.uint32) == want_s.view(np.uint32)).sum())}/{p}")
+    assert same >= p - 2
+
+
 @pytest.mark.parametrize("name", SCENES)
 def test_cem_with_reference_statistics_is_bit_identical(H, torch, ctx, name):
     g = golden(name)
