@@ -29,6 +29,8 @@ struct hsi_ctx {
   int moment_bands = 0;        // bands of the last accumulation, 0 if none
   int stats_bands = 0;         // bands of the current statistics, 0 if none
   int stats_targets = 0;       // targets the statistics were built with
+  bool layouts_dirty = false;  // plain score layouts not yet derived from the statistics
+  bool mixed_dirty = false;    // mixed-precision layouts not yet derived
   double last_loading = 0.0;
   double* d_count = nullptr;
   double* d_sum = nullptr;
@@ -288,9 +290,16 @@ hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const float* d_cube, int height, i
   return HSI_OK;
 }
 
-static hsi_status refresh_layouts(hsi_ctx* ctx, int bands, int targets, hipStream_t stream) {
+// The score kernels read the statistics in derived layouts (band-major
+// doubles for CEM, MFMA-ordered blocks for RX/ACE, scaled FP16 hi/lo parts for
+// the mixed form). They are derived on the first score that needs them after
+// each build or upload, on that score's stream, so a frame that scores only
+// SAM and CEM never pays for the RX/ACE layouts.
+static hsi_status ensure_layouts(hsi_ctx* ctx, hipStream_t stream) {
+  if (!ctx->layouts_dirty) return HSI_OK;
+  const int targets = ctx->stats_targets;
   LayoutArgs la;
-  la.bands = bands;
+  la.bands = ctx->stats_bands;
   la.num_targets = targets;
   la.lib_stride = sam_cem_lib_stride(targets > 0 ? targets : 1);
   la.whitener = ctx->d_whitener;
@@ -301,19 +310,24 @@ static hsi_status refresh_layouts(hsi_ctx* ctx, int bands, int targets, hipStrea
   la.white_blocks = ctx->d_white_blocks;
   la.ace_u = ctx->d_ace_u;
   HSI_HIP(ctx, launch_layouts(la, stream), "derive statistics layouts");
-  if (bands <= kMaxRxBands) {
-    MixedLayoutArgs ma;
-    ma.bands = bands;
-    ma.num_targets = targets;
-    ma.whitener = ctx->d_whitener;
-    ma.target_white = ctx->d_target_white;
-    ma.white_hl = ctx->d_white_hl;
-    ma.ace_hl = ctx->d_ace_hl;
-    ma.col_scale = ctx->d_col_scale;
-    ma.row_inv = ctx->d_row_inv;
-    ma.target_inv = ctx->d_target_inv;
-    HSI_HIP(ctx, launch_mixed_layouts(ma, stream), "derive mixed-precision layouts");
-  }
+  ctx->layouts_dirty = false;
+  return HSI_OK;
+}
+
+static hsi_status ensure_mixed_layouts(hsi_ctx* ctx, hipStream_t stream) {
+  if (!ctx->mixed_dirty) return HSI_OK;
+  MixedLayoutArgs ma;
+  ma.bands = ctx->stats_bands;
+  ma.num_targets = ctx->stats_targets;
+  ma.whitener = ctx->d_whitener;
+  ma.target_white = ctx->d_target_white;
+  ma.white_hl = ctx->d_white_hl;
+  ma.ace_hl = ctx->d_ace_hl;
+  ma.col_scale = ctx->d_col_scale;
+  ma.row_inv = ctx->d_row_inv;
+  ma.target_inv = ctx->d_target_inv;
+  HSI_HIP(ctx, launch_mixed_layouts(ma, stream), "derive mixed-precision layouts");
+  ctx->mixed_dirty = false;
   return HSI_OK;
 }
 
@@ -346,8 +360,8 @@ hsi_status hsi_stats_build(hsi_ctx* ctx, double diagonal_loading, hsi_stream str
   a.cem_weight = ctx->d_cem_weight;
   a.status = ctx->d_status;
   HSI_HIP(ctx, launch_build(a, s), "build statistics");
-  hsi_status st = refresh_layouts(ctx, bands, targets, s);
-  if (st != HSI_OK) return st;
+  ctx->layouts_dirty = true;
+  ctx->mixed_dirty = true;
   ctx->stats_bands = bands;
   ctx->stats_targets = targets;
   ctx->last_loading = diagonal_loading;
@@ -437,9 +451,9 @@ hsi_status hsi_stats_upload(hsi_ctx* ctx, int bands, const float* mean, const fl
             "upload cem_weight");
   }
   HSI_HIP(ctx, hipMemset(ctx->d_status, 0, 2 * sizeof(unsigned)), "reset status");
-  hsi_status st = refresh_layouts(ctx, bands, targets, nullptr);
-  if (st != HSI_OK) return st;
   HSI_HIP(ctx, hipDeviceSynchronize(), "upload statistics");
+  ctx->layouts_dirty = true;
+  ctx->mixed_dirty = true;
   ctx->stats_bands = bands;
   ctx->stats_targets = targets;
   return HSI_OK;
@@ -457,9 +471,17 @@ hsi_status hsi_score_sam_cem(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype, 
     return fail(ctx, HSI_ERR_INVALID_ARG,
                 "library has " + std::to_string(ctx->lib_bands) + " bands but the cube has " +
                     std::to_string(bands));
+  for (const void* m : {static_cast<const void*>(d_sam_score), static_cast<const void*>(d_sam_target),
+                        static_cast<const void*>(d_cem_score), static_cast<const void*>(d_cem_target)})
+    if (reinterpret_cast<std::uintptr_t>(m) % 16 != 0)
+      return fail(ctx, HSI_ERR_INVALID_ARG, "SAM/CEM score and target maps must be 16-byte aligned");
   if (d_cem_score && (ctx->stats_bands != bands || ctx->stats_targets != ctx->num_targets))
     return fail(ctx, HSI_ERR_NO_STATISTICS, "CEM needs statistics built for this library and band count");
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  if (d_cem_score) {
+    const hsi_status lst = ensure_layouts(ctx, static_cast<hipStream_t>(stream));
+    if (lst != HSI_OK) return lst;
+  }
   SamCemArgs a;
   a.cube = d_cube;
   a.dtype = dtype;
@@ -514,6 +536,11 @@ hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precisio
   if (detector == HSI_DET_ACE && ctx->stats_targets <= 0)
     return fail(ctx, HSI_ERR_NO_LIBRARY, "ACE needs a target library");
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  {
+    const hipStream_t ls = static_cast<hipStream_t>(stream);
+    const hsi_status lst = precision == HSI_PLAIN ? ensure_layouts(ctx, ls) : ensure_mixed_layouts(ctx, ls);
+    if (lst != HSI_OK) return lst;
+  }
   RxAceArgs a;
   a.cube = d_cube;
   a.dtype = dtype;

@@ -55,6 +55,19 @@ __device__ __forceinline__ void widen(const V& v, double (&x)[PPT]) {
   }
 }
 
+// Stores PPT consecutive 4-byte results; the output maps are 4*PPT-byte
+// aligned at every group start (checked by the launcher).
+template <int PPT, typename T>
+__device__ __forceinline__ void store_group(T* dst, const T (&v)[PPT]) {
+  if constexpr (PPT == 4) {
+    typedef T V __attribute__((ext_vector_type(4)));
+    *reinterpret_cast<V*>(dst) = V{v[0], v[1], v[2], v[3]};
+  } else {
+    typedef T V __attribute__((ext_vector_type(2)));
+    *reinterpret_cast<V*>(dst) = V{v[0], v[1]};
+  }
+}
+
 // Best-match state for one pixel, in angle order without taking an acos per
 // target. kind 0: nothing yet (angle +inf); 1: degenerate (pixel or target
 // without direction, angle = float(pi/2) as angle_of returns); 2: a clamped
@@ -165,10 +178,10 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
     }
   }
 
+  float s_out[PPT], c_out[PPT];
+  int st_out[PPT], ct_out[PPT];
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
-    const std::size_t p = p0 + k;
-    if (p >= pixels) break;
     if constexpr (DO_SAM) {
       // cosine = dot / (sqrt(|x|^2) * |t|), clamped, as angle_of
       // (sam_cpu.cpp:13-18): correctly rounded sqrt, product and quotient.
@@ -209,8 +222,8 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
       double angle = __longlong_as_double(0x7ff0000000000000ll);  // +inf
       if (best.kind == 1) angle = kMaxSamAngle;
       else if (best.kind == 2) angle = acos(best.cosine);
-      s_score[p] = static_cast<float>(angle);
-      if (s_target) s_target[p] = best.target;
+      s_out[k] = static_cast<float>(angle);
+      st_out[k] = best.target;
     }
     if constexpr (DO_CEM) {
       double best = -__longlong_as_double(0x7ff0000000000000ll);
@@ -222,8 +235,34 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
           best_t = t;
         }
       }
-      c_score[p] = static_cast<float>(best);
-      if (c_target) c_target[p] = best_t;
+      c_out[k] = static_cast<float>(best);
+      ct_out[k] = best_t;
+    }
+  }
+  // Whole groups leave as one 8- or 16-byte store per map, so a wave writes
+  // full cache lines; the ragged tail of the frame stores pixel by pixel.
+  if (p0 + PPT <= pixels) {
+    if constexpr (DO_SAM) {
+      store_group<PPT>(s_score + p0, s_out);
+      if (s_target) store_group<PPT>(s_target + p0, st_out);
+    }
+    if constexpr (DO_CEM) {
+      store_group<PPT>(c_score + p0, c_out);
+      if (c_target) store_group<PPT>(c_target + p0, ct_out);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const std::size_t p = p0 + k;
+      if (p >= pixels) break;
+      if constexpr (DO_SAM) {
+        s_score[p] = s_out[k];
+        if (s_target) s_target[p] = st_out[k];
+      }
+      if constexpr (DO_CEM) {
+        c_score[p] = c_out[k];
+        if (c_target) c_target[p] = ct_out[k];
+      }
     }
   }
 }

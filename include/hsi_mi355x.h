@@ -185,7 +185,8 @@ hsi_status hsi_stats_upload(hsi_ctx* ctx, int bands, const float* mean,
  *   CEM: d_score = best w_t . x, d_target its index (cem_best_cpu,
  *        detectors_cpu.cpp:158-187).
  * d_target may be NULL. PLAIN accumulates in FP64 and reads an F32 or F16
- * cube; MIXED (RX/ACE only) whitens on FP16 matrix cores. */
+ * cube; MIXED (RX/ACE only) whitens on FP16 matrix cores. SAM and CEM maps
+ * must be 16-byte aligned (any device allocation is), else INVALID_ARG. */
 hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precision,
                      const void* d_cube, hsi_dtype dtype, int height, int width,
                      int bands, size_t plane_stride, float* d_score,
