@@ -40,6 +40,15 @@ FP64_PEAK_TFLOPS = 78.6
 FP16_MFMA_PEAK_TFLOPS = 2500.0
 
 H_, W_, B_, T_ = 1024, 1024, 113, 8
+
+# HBM bytes per launch of each candidate dominant kernel on this workload,
+# from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB copy (FETCH
+# KiB x 2.0, WRITE KiB x 1.0); the runs and the table are recorded in
+# profiles/hbm_traffic_1024x1024x113.md.
+MEASURED_TRAFFIC_BYTES = {
+    "moments_f64_kernel": 231471.4 * 2.0 * 1024 + 18432.0 * 1024,
+    "sam_cem_kernel": 232352.2 * 2.0 * 1024 + 16384.0 * 1024,
+}
 LOADING = 1e-6
 SAM_THRESHOLD = 0.10
 CEM_THRESHOLD = 0.5
@@ -308,7 +317,7 @@ def main():
         roof = {"kernel": dom, "bound": "latency", "achieved": None, "peak": None, "unit": None}
     if roof.get("achieved") is not None:
         roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
-    roof["traffic"] = None
+    roof["traffic"] = round(MEASURED_TRAFFIC_BYTES[roof["kernel"]]) if roof["kernel"] in MEASURED_TRAFFIC_BYTES else None
 
     cpu = None
     if not args.no_cpu:

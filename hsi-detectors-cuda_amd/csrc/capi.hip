@@ -389,6 +389,16 @@ hsi_status hsi_stats_check(hsi_ctx* ctx, hsi_stream stream) {
                   "; try a larger --loading");
 }
 
+hsi_status hsi_stats_status_copy(hsi_ctx* ctx, uint32_t* dst, hsi_stream stream) {
+  if (!ctx || !dst) return fail(ctx, HSI_ERR_INVALID_ARG, "status copy needs a destination");
+  if (ctx->stats_bands <= 0) return fail(ctx, HSI_ERR_NO_STATISTICS, "no statistics built");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, hipMemcpyAsync(dst, ctx->d_status, 2 * sizeof(unsigned), hipMemcpyDefault,
+                              static_cast<hipStream_t>(stream)),
+          "copy statistics status");
+  return HSI_OK;
+}
+
 hsi_status hsi_stats_read(hsi_ctx* ctx, float* mean, float* whitener, float* target_white,
                           float* target_norm2, float* cem_weight, double* count, double* sum,
                           double* sum_outer) {

@@ -1,0 +1,238 @@
+// hsi_detect - frame-rate target detection on one MI355X.
+//
+// Counterpart of the reference's hsi_detect (main.cpp:125-292): frames from a
+// synthetic or ENVI-replay source go through the Pipeline, and the run reports
+// throughput, latency percentiles and per-stage device times. Any of the four
+// detectors (or several) score each frame; flags keep the reference's names
+// where the meaning is the same (--source, --envi, --width/--height/--bands,
+// --targets, --noise, --library, --threshold, --nms, --max-detections,
+// --streams, --frames, --warmup, --prefill, --csv, --print-detections,
+// --dump-score, --quiet).
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "hsi_host.hpp"
+
+namespace {
+
+void usage() {
+  std::puts(R"(hsi_detect - hyperspectral target detection on an MI355X
+
+  hsi_detect [options]
+
+  --source=synthetic|envi       where frames come from               [synthetic]
+  --envi=A.hdr,B.hdr            ENVI cubes to replay (cycled)
+  --width=N --height=N          synthetic frame size                 [512x512]
+  --bands=N                     synthetic band count                     [113]
+  --targets=N                   synthetic library size                     [4]
+  --noise=F                     synthetic noise sigma                  [0.005]
+  --seed=N                      synthetic seed                      [20261203]
+  --library=FILE.csv            target spectra; the synthetic source supplies
+                                its own when this is absent
+  --detector=LIST               sam, rx, ace, cem, comma-separated       [sam]
+  --precision=plain|mixed       RX/ACE arithmetic form                 [plain]
+  --loading=F                   diagonal loading of the statistics      [1e-6]
+  --threshold=F                 detection threshold (angle for SAM)     [0.10]
+  --nms=N                       non-maximum suppression radius, 0 off      [2]
+  --max-detections=N            detection list capacity per detector   [4096]
+  --streams=N                   frames in flight                           [3]
+  --frames=N                    frames to process                        [300]
+  --warmup=N                    unmeasured frames first                   [30]
+  --prefill                     stage the ring once and replay it
+  --no-graph                    launch kernels one by one instead of replaying
+                                a captured hipGraph per frame
+  --device=N                    HIP device                                 [0]
+  --csv=FILE                    write per-frame timings
+  --print-detections=N          print the first N detections of the last frame
+  --dump-score=FILE             write the last frame's first score map (raw f32)
+  --json                        print one JSON summary line
+  --quiet
+  --help)");
+}
+
+struct Args {
+  std::map<std::string, std::string> v;
+  bool has(const std::string& k) const { return v.count(k) != 0; }
+  std::string str(const std::string& k, const std::string& d) const {
+    auto it = v.find(k);
+    return it == v.end() || it->second.empty() ? d : it->second;
+  }
+  long long integer(const std::string& k, long long d) const {
+    auto it = v.find(k);
+    return it == v.end() || it->second.empty() ? d : std::stoll(it->second);
+  }
+  double real(const std::string& k, double d) const {
+    auto it = v.find(k);
+    return it == v.end() || it->second.empty() ? d : std::stod(it->second);
+  }
+};
+
+Args parse(int argc, char** argv) {
+  Args a;
+  for (int i = 1; i < argc; ++i) {
+    std::string t = argv[i];
+    if (t.rfind("--", 0) != 0) continue;
+    t = t.substr(2);
+    const std::size_t eq = t.find('=');
+    if (eq == std::string::npos) a.v[t] = "";
+    else a.v[t.substr(0, eq)] = t.substr(eq + 1);
+  }
+  return a;
+}
+
+std::vector<std::string> commas(const std::string& s) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char c : s) {
+    if (c == ',') {
+      if (!cur.empty()) out.push_back(cur);
+      cur.clear();
+    } else {
+      cur.push_back(c);
+    }
+  }
+  if (!cur.empty()) out.push_back(cur);
+  return out;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const Args args = parse(argc, argv);
+  if (args.has("help")) {
+    usage();
+    return 0;
+  }
+  try {
+    const bool quiet = args.has("quiet");
+    // ---- source -----------------------------------------------------------
+    std::unique_ptr<hsi::FrameSource> source;
+    const std::string kind = args.str("source", "synthetic");
+    if (kind == "synthetic") {
+      hsi::SyntheticOptions so;
+      hsi_synth_default(&so.config);
+      so.config.width = static_cast<int>(args.integer("width", 512));
+      so.config.height = static_cast<int>(args.integer("height", 512));
+      so.config.bands = static_cast<int>(args.integer("bands", 113));
+      so.config.num_targets = static_cast<int>(args.integer("targets", 4));
+      so.config.noise_sigma = args.real("noise", 0.005);
+      so.config.seed = static_cast<std::uint64_t>(args.integer("seed", 20261203));
+      so.config.drift = args.real("drift", 0.0);
+      so.frames = 0;
+      source = hsi::make_synthetic_source(so);
+    } else if (kind == "envi") {
+      if (!args.has("envi")) {
+        std::fputs("--source=envi needs --envi=<header>[,<header>...]\n", stderr);
+        return 2;
+      }
+      source = hsi::make_envi_replay_source(commas(args.str("envi", "")), true);
+    } else {
+      std::fprintf(stderr, "unknown --source=%s\n", kind.c_str());
+      return 2;
+    }
+    // ---- library ----------------------------------------------------------
+    hsi::SpectralLibrary library;
+    if (args.has("library")) library = hsi::read_spectra_csv(args.str("library", ""));
+    else if (source->library()) library = *source->library();
+    // ---- pipeline ---------------------------------------------------------
+    hsi::PipelineOptions opt;
+    if (!hsi::parse_detector_list(args.str("detector", "sam"), &opt.detectors)) {
+      std::fputs("--detector must list sam, rx, ace or cem\n", stderr);
+      return 2;
+    }
+    const std::string prec = args.str("precision", "plain");
+    if (prec != "plain" && prec != "mixed") {
+      std::fputs("--precision must be plain or mixed\n", stderr);
+      return 2;
+    }
+    opt.precision = prec == "mixed" ? HSI_MIXED : HSI_PLAIN;
+    opt.loading = args.real("loading", 1e-6);
+    opt.detection.threshold = static_cast<float>(args.real("threshold", 0.10));
+    opt.detection.nms_radius = static_cast<int>(args.integer("nms", 2));
+    opt.detection.max_detections = static_cast<int>(args.integer("max-detections", 4096));
+    opt.num_streams = static_cast<int>(args.integer("streams", 3));
+    opt.prefill = args.has("prefill") ? 1 : 0;
+    opt.use_graph = !args.has("no-graph");
+    opt.device = static_cast<int>(args.integer("device", 0));
+    opt.return_score_map = args.has("dump-score");
+    std::string error;
+    auto pipeline = hsi::Pipeline::create(source.get(), library, opt, &error);
+    if (!pipeline) {
+      std::fprintf(stderr, "pipeline: %s\n", error.c_str());
+      return 1;
+    }
+    if (!quiet) std::printf("%s\n", pipeline->describe().c_str());
+    const long long warmup = args.integer("warmup", 30), frames = args.integer("frames", 300);
+    if (warmup > 0 && !pipeline->run(static_cast<std::uint64_t>(warmup), nullptr, &error)) {
+      std::fprintf(stderr, "warm-up: %s\n", error.c_str());
+      return 1;
+    }
+    pipeline->reset_stats();
+    std::ofstream csv;
+    if (args.has("csv")) {
+      csv.open(args.str("csv", ""));
+      csv << "frame,ms_wall,ms_gpu,ms_source,ms_upload,ms_stats,ms_score,ms_detect,ms_download,statistics_ok";
+      for (hsi::Detector d : opt.detectors) csv << ",found_" << hsi::to_string(d);
+      csv << "\n";
+    }
+    hsi::FrameResult last;
+    bool stats_failed = false;
+    const bool ok = pipeline->run(
+        static_cast<std::uint64_t>(frames),
+        [&](const hsi::FrameResult& r) {
+          if (csv.is_open()) {
+            csv << r.meta.index << "," << r.ms_wall << "," << r.ms_gpu << "," << r.ms_source << "," << r.ms_upload
+                << "," << r.ms_stats << "," << r.ms_score << "," << r.ms_detect << "," << r.ms_download << ","
+                << (r.statistics_ok ? 1 : 0);
+            for (const auto& dr : r.results) csv << "," << dr.detections_found;
+            csv << "\n";
+          }
+          stats_failed |= !r.statistics_ok;
+          last = r;
+        },
+        &error);
+    if (!ok) {
+      std::fprintf(stderr, "run: %s\n", error.c_str());
+      return 1;
+    }
+    const hsi::PipelineStats st = pipeline->stats();
+    if (!quiet) {
+      std::printf("frames %llu  %.1f fps  %.1f Mpix/s  %.2f GB/s of cube\n",
+                  static_cast<unsigned long long>(st.frames), st.fps, st.mpixel_per_s, st.cube_gb_per_s);
+      std::printf("latency ms  p50 %.3f  p95 %.3f  p99 %.3f  max %.3f\n", st.ms_p50, st.ms_p95, st.ms_p99, st.ms_max);
+      std::printf("mean ms  source %.3f  upload %.3f  stats %.3f  score %.3f  detect %.3f  download %.3f  gpu %.3f\n",
+                  st.mean_source, st.mean_upload, st.mean_stats, st.mean_score, st.mean_detect, st.mean_download,
+                  st.mean_gpu);
+      for (const auto& dr : last.results)
+        std::printf("last frame %s: %u found\n", hsi::to_string(dr.detector), dr.detections_found);
+      if (stats_failed) std::puts("warning: statistics failed on some frames; try a larger --loading");
+    }
+    const long long show = args.integer("print-detections", 0);
+    for (const auto& dr : last.results)
+      for (long long i = 0; i < show && i < static_cast<long long>(dr.detections.size()); ++i) {
+        const hsi::Detection& d = dr.detections[static_cast<std::size_t>(i)];
+        std::printf("%s x=%d y=%d target=%d score=%.6g\n", hsi::to_string(dr.detector), d.x, d.y, d.target, d.score);
+      }
+    if (args.has("dump-score") && !last.results.empty()) {
+      std::ofstream out(args.str("dump-score", ""), std::ios::binary);
+      const auto& m = last.results.front().score_map;
+      out.write(reinterpret_cast<const char*>(m.data()), static_cast<std::streamsize>(m.size() * sizeof(float)));
+    }
+    if (args.has("json")) {
+      std::printf("{\"frames\": %llu, \"fps\": %.3f, \"mpix_s\": %.3f, \"cube_gb_s\": %.3f, \"p50_ms\": %.4f, "
+                  "\"p99_ms\": %.4f, \"mean_gpu_ms\": %.4f, \"graph\": %s, \"detections\": %llu, "
+                  "\"statistics_ok\": %s}\n",
+                  static_cast<unsigned long long>(st.frames), st.fps, st.mpixel_per_s, st.cube_gb_per_s, st.ms_p50,
+                  st.ms_p99, st.mean_gpu, opt.use_graph ? "true" : "false",
+                  static_cast<unsigned long long>(st.total_detections), stats_failed ? "false" : "true");
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "hsi_detect: %s\n", e.what());
+    return 1;
+  }
+}

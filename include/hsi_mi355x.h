@@ -152,6 +152,14 @@ hsi_status hsi_stats_build(hsi_ctx* ctx, double diagonal_loading, hsi_stream str
  * --loading) in hsi_ctx_last_error. */
 hsi_status hsi_stats_check(hsi_ctx* ctx, hsi_stream stream);
 
+/* Asynchronous: copy the two status words of the last build (covariance
+ * path, autocorrelation path; 0 = factorised, else the factorisation failed)
+ * to `dst` (device or pinned host memory, 8 bytes) on `stream`. The
+ * non-blocking form of hsi_stats_check, for pipelines and captured graphs.
+ * No reference counterpart: build_statistics is synchronous there
+ * (detector.cpp:47-152). */
+hsi_status hsi_stats_status_copy(hsi_ctx* ctx, uint32_t* dst, hsi_stream stream);
+
 /* Synchronous copy of the statistics to host memory; any pointer may be
  * NULL. Sizes: mean[B], whitener[B*B] (row-major, upper triangle zero),
  * target_white[T*B], target_norm2[T], cem_weight[T*B], sum[B],

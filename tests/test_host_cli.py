@@ -1,0 +1,223 @@
+"""The C++ host layer: hsi_score and hsi_detect (the reference's two CLIs,
+score_main.cpp and main.cpp) over the Pipeline and the file readers.
+
+CPU tests exercise argument handling and the ENVI / CSV readers' error paths,
+all of which run before any device call. GPU tests score golden scenes
+through hsi_score (ENVI + CSV in, raw f32/i32 dumps out, compared with the
+reference's own outputs) and run the frame pipeline, eager and as replayed
+hipGraphs, against the Python path on the same seeded frames.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG, SCENES, golden
+
+BIN = os.path.join(PKG, "bin")
+
+
+def run(args, timeout=120):
+    return subprocess.run([os.path.join(BIN, args[0])] + list(args[1:]), capture_output=True, text=True,
+                          timeout=timeout)
+
+
+def write_envi(path_hdr, cube, interleave="bsq", dtype=np.float32, data_type=4, scale=None):
+    """cube: (bands, H, W) float32 -> ENVI files; integer types are stored as
+    round(value / scale)."""
+    b, h, w = cube.shape
+    if interleave == "bsq":
+        arr = cube
+    elif interleave == "bip":
+        arr = np.transpose(cube, (1, 2, 0))
+    else:  # bil: line, band, sample
+        arr = np.transpose(cube, (1, 0, 2))
+    if scale is not None:
+        arr = np.round(arr / scale)
+    arr = np.ascontiguousarray(arr).astype(dtype)
+    with open(path_hdr, "w") as f:
+        f.write(f"ENVI\nsamples = {w}\nlines = {h}\nbands = {b}\nheader offset = 0\n"
+                f"file type = ENVI Standard\ndata type = {data_type}\ninterleave = {interleave}\n"
+                f"byte order = 0\n")
+    arr.tofile(path_hdr[:-4] + ".img")
+
+
+def write_csv(path, library, names=None):
+    with open(path, "w") as f:
+        f.write("# test library\n")
+        for i, row in enumerate(library):
+            name = names[i] if names else f"t{i}"
+            f.write(name + "," + ",".join(repr(float(v)) for v in row) + "\n")
+
+
+# ---- CPU --------------------------------------------------------------------
+
+def test_cli_help_and_usage():
+    for tool in ("hsi_score", "hsi_detect"):
+        r = run([tool, "--help"])
+        assert r.returncode == 0, r.stderr
+        assert "--threshold" in r.stdout and "--nms" in r.stdout
+    r = run(["hsi_score"])
+    assert r.returncode == 2  # missing --cube / --library prints usage
+    r = run(["hsi_score", "--cube=x.hdr", "--library=y.csv", "--detector=bogus"])
+    assert r.returncode == 2 and "--detector" in r.stderr
+
+
+def test_score_reports_file_errors_before_touching_the_device(tmp_path):
+    cube = np.ones((3, 2, 4), np.float32)
+    hdr = str(tmp_path / "c.hdr")
+    write_envi(hdr, cube)
+    r = run(["hsi_score", f"--cube={tmp_path / 'missing.hdr'}", "--library=x.csv"])
+    assert r.returncode == 1 and "cannot open ENVI header" in r.stderr
+    bad = tmp_path / "bad.csv"
+    bad.write_text("a,1,2,zz\n")
+    r = run(["hsi_score", f"--cube={hdr}", f"--library={bad}"])
+    assert r.returncode == 1 and "'zz' is not a number" in r.stderr
+    ragged = tmp_path / "ragged.csv"
+    ragged.write_text("a,1,2,3\nb,1,2\n")
+    r = run(["hsi_score", f"--cube={hdr}", f"--library={ragged}"])
+    assert r.returncode == 1 and "has 2 bands, expected 3" in r.stderr
+    wrong = tmp_path / "wrong.csv"
+    wrong.write_text("wavelength,400,500,600,700\na,1,2,3,4\n")
+    r = run(["hsi_score", f"--cube={hdr}", f"--library={wrong}"])
+    assert r.returncode == 1 and "4 bands but the cube has 3" in r.stderr
+    r = run(["hsi_score", f"--cube={hdr}", f"--library={tmp_path / 'none.csv'}"])
+    assert r.returncode == 1 and "cannot open spectral library" in r.stderr
+    short = tmp_path / "s.hdr"
+    short.write_text("ENVI\nsamples = 4\nlines = 2\nbands = 30\ndata type = 4\ninterleave = bsq\n")
+    (tmp_path / "s.img").write_bytes(b"\0" * 16)
+    r = run(["hsi_score", f"--cube={short}", "--detector=rx"])
+    assert r.returncode == 1 and "header promises" in r.stderr
+    r = run(["hsi_score", f"--cube={hdr}", "--library=x.csv", "--target=nobody"])
+    assert r.returncode == 1
+
+
+def test_detect_rejects_bad_options():
+    r = run(["hsi_detect", "--detector=sam,foo"])
+    assert r.returncode == 2 and "--detector" in r.stderr
+    r = run(["hsi_detect", "--precision=half"])
+    assert r.returncode == 2
+    r = run(["hsi_detect", "--source=camera"])
+    assert r.returncode == 2 and "unknown --source" in r.stderr
+
+
+# ---- GPU --------------------------------------------------------------------
+
+def score_cli(tmp_path, g, detector, extra=(), cube=None, hdr_kw=None):
+    cube = g["cube"] if cube is None else cube
+    hdr = str(tmp_path / "cube.hdr")
+    write_envi(hdr, cube, **(hdr_kw or {}))
+    csv = str(tmp_path / "lib.csv")
+    write_csv(csv, g["library"])
+    s, t = str(tmp_path / "score.f32"), str(tmp_path / "target.i32")
+    print(f"hsi_score --detector={detector} on {cube.shape}", flush=True)
+    r = run(["hsi_score", f"--cube={hdr}", f"--library={csv}", f"--detector={detector}",
+             f"--dump-score={s}", f"--dump-target={t}", "--print=3"] + list(extra))
+    assert r.returncode == 0, r.stderr
+    return np.fromfile(s, np.float32), np.fromfile(t, np.int32), r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", SCENES)
+def test_hsi_score_sam_matches_reference_dump(tmp_path, name):
+    g = golden(name)
+    thr = repr(float(g["sam_threshold"]))
+    s, t, out = score_cli(tmp_path, g, "sam", extra=[f"--threshold={thr}", "--nms=2"])
+    assert np.abs(s.astype(np.float64) - g["sam_score"]).max() <= 1e-6
+    assert np.array_equal(t, g["sam_target"])
+    assert f" {int(g['sam_det2_score'].size)} detections" in out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("det", ["rx", "ace", "cem"])
+def test_hsi_score_statistics_detectors_match_reference(tmp_path, det):
+    g = golden("scene_32x40x113_t4")
+    s, t, out = score_cli(tmp_path, g, det)
+    want = g[f"{det}_score"].astype(np.float64)
+    err = np.abs(s - want) / np.maximum(np.abs(want), 1.0)
+    assert err.max() <= 1e-5, err.max()
+    if det != "rx":
+        agree = (t == g[f"{det}_target"]).mean()
+        assert agree > 0.999
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", [("bip", np.float32, 4, None), ("bil", np.float32, 4, None),
+                                    ("bsq", np.uint16, 12, 1 / 65535.0), ("bip", np.int16, 2, 1 / 32767.0)])
+def test_hsi_score_reads_every_envi_layout(tmp_path, oracle_lib, layout):
+    interleave, dtype, code, scale = layout
+    g = golden("scene_24x32x16_t3")
+    cube = g["cube"]
+    # the value the reader should reconstruct, in float32 like the reference
+    if scale is None:
+        want_cube = cube
+    else:
+        q = np.round(cube / scale).astype(dtype)
+        full = {np.uint16: 65535.0, np.int16: 32767.0}[dtype]
+        want_cube = q.astype(np.float32) * (np.float32(1.0) / np.float32(full))
+    s, t, _ = score_cli(tmp_path, g, "sam", cube=cube,
+                        hdr_kw=dict(interleave=interleave, dtype=dtype, data_type=code, scale=scale))
+    ws, wt = oracle_lib.sam_best(want_cube, g["library"])
+    assert np.abs(s.astype(np.float64) - ws).max() <= 1e-6
+    assert (t == wt).mean() > 0.999
+
+
+def detect_cli(tmp_path, extra):
+    csvp = str(tmp_path / f"frames_{len(os.listdir(tmp_path))}.csv")
+    args = ["hsi_detect", "--width=160", "--height=96", "--bands=113", "--targets=8", "--seed=77",
+            "--frames=6", "--warmup=2", "--streams=2", "--detector=sam,rx,ace,cem", f"--csv={csvp}",
+            "--json", "--quiet", "--threshold=0.1", "--print-detections=5"] + list(extra)
+    print(" ".join(args), flush=True)
+    r = run(args, timeout=180)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    summary = json.loads(lines[-1])
+    rows = np.genfromtxt(csvp, delimiter=",", names=True)
+    return summary, rows, lines[:-1]
+
+
+@pytest.mark.gpu
+def test_hsi_detect_graph_replay_equals_eager_and_python(tmp_path):
+    import torch
+    import hsi_mi355x as H
+    g_sum, g_rows, g_lines = detect_cli(tmp_path, [])
+    e_sum, e_rows, e_lines = detect_cli(tmp_path, ["--no-graph"])
+    assert g_sum["graph"] and not e_sum["graph"]
+    assert g_sum["frames"] == 6 and g_sum["statistics_ok"]
+    for col in ("found_sam", "found_rx", "found_ace", "found_cem", "frame"):
+        assert np.array_equal(g_rows[col], e_rows[col]), col
+    assert g_lines == e_lines  # last frame's first detections, every detector
+    # frame indices 2..7 of the same seeded stream through the Python path
+    cfg = H.synth_config(96, 160, 113, 8, seed=77)
+    lib = H.synth_library(cfg)
+    ctx = H.Context(0)
+    ctx.set_library(lib)
+    p = 96 * 160
+    s = torch.empty(p, dtype=torch.float32, device="cuda")
+    t = torch.empty(p, dtype=torch.int32, device="cuda")
+    out = torch.empty((4096, 4), dtype=torch.int32, device="cuda")
+    cnt = torch.empty(1, dtype=torch.int32, device="cuda")
+    for row in g_rows:
+        k = int(row["frame"])
+        cube = torch.from_numpy(H.synth_frame(cfg, k)).cuda()
+        ctx.score(H.Detector.SAM, cube, 96, 160, 113, s, t)
+        ctx.detect(s, t, 96, 160, H.DetectionParams(0.1, 2, 4096, False), out, cnt)
+        torch.cuda.synchronize()
+        assert int(cnt.item()) == int(row["found_sam"]), k
+        ctx.stats_accumulate(cube, 96, 160, 113)
+        ctx.stats_build(1e-6)
+        ctx.score(H.Detector.CEM, cube, 96, 160, 113, s, t)
+        ctx.detect(s, t, 96, 160, H.DetectionParams(0.1, 2, 4096, True), out, cnt)
+        torch.cuda.synchronize()
+        assert int(cnt.item()) == int(row["found_cem"]), k
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_hsi_detect_prefill_and_mixed(tmp_path):
+    summ, rows, _ = detect_cli(tmp_path, ["--prefill", "--precision=mixed"])
+    assert summ["frames"] == 6 and summ["statistics_ok"]
+    # a prefilled ring of 2 slots replays frames 0 and 1
+    assert np.array_equal(rows["found_sam"][0::2], rows["found_sam"][0::2][:1].repeat(3))
