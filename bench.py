@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--repeats", type=int, default=20, help="per-kernel timing repeats")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=1024, help="rows of a frame in the CPU sample")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 224-band precision-form leg (plain vs mixed RX/ACE)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB buffer 5 times (reference kernel for HBM byte counters)")
     return ap.parse_args()
@@ -172,6 +174,99 @@ def cpu_baseline(host_frame, library, rows):
         "per_detector_mpix_s": {"stats": round(sample_px / (t1 - t0) / 1e6, 4),
                                 "sam": round(sample_px / (t2 - t1) / 1e6, 4),
                                 "cem": round(sample_px / (t3 - t2) / 1e6, 4)},
+    }
+
+
+def precision_form(H, torch, repeats):
+    """RX+ACE in the precision form: 1024x1024 px x 224 bands, 16 targets, the
+    FP32 frame stored on the device as FP16. Per frame the RX/ACE path is
+    moments -> FP64 build -> ACE (RX as its side output).
+      plain: FP64 moments, FP64-accumulated whitening on the FP32 cube;
+      mixed: FP16 hi/lo moments with FP32 tiles reduced in FP64, FP64 build,
+             FP16 MFMA whitening of the FP16 cube (hi/lo whitener), FP64 ratios.
+    Error is reported against the plain path (FP64 throughout; the parity
+    tests hold it within ~1e-7 of the FP64 oracle), recall as the share of the
+    plain path's detections (RX at a chi-squared false-alarm rate of 1e-3, ACE
+    at 0.5, NMS radius 2) that the mixed path also reports. The tests check
+    the mixed path against the FP64 oracle itself."""
+    h, w, b, t = 1024, 1024, 224, 16
+    p = h * w
+    cfg = H.synth_config(h, w, b, t, seed=20261203 + 3000)
+    lib = H.synth_library(cfg)
+    host = H.synth_frame(cfg, 0, threads=16)
+    c32 = torch.from_numpy(host).cuda()
+    c16 = c32.half()
+    del host
+    ctx = H.Context(0)
+    ctx.set_library(lib)
+    ctx.reserve(p, b)
+    f32 = dict(dtype=torch.float32, device="cuda")
+    i32 = dict(dtype=torch.int32, device="cuda")
+    ace_s, rx_s, ace_t = torch.empty(p, **f32), torch.empty(p, **f32), torch.empty(p, **i32)
+
+    def plain():
+        ctx.stats_accumulate(c32, h, w, b)
+        ctx.stats_build(LOADING)
+        ctx.score(H.Detector.ACE, c32, h, w, b, ace_s, ace_t, rx_s)
+
+    def mixed():
+        ctx.stats_accumulate(c32, h, w, b, precision=H.MIXED)
+        ctx.stats_build(LOADING)
+        ctx.score(H.Detector.ACE, c16, h, w, b, ace_s, ace_t, rx_s, precision=H.MIXED)
+
+    def score_only(prec, cube):
+        return lambda: ctx.score(H.Detector.ACE, cube, h, w, b, ace_s, ace_t, rx_s, precision=prec)
+
+    cap = 1 << 16
+    dets = torch.empty((cap, 4), **i32)
+    cnt = torch.empty(1, **i32)
+
+    def detections(score, target, thr):
+        ctx.detect(score, target, h, w, H.DetectionParams(thr, 2, cap, True), dets, cnt)
+        torch.cuda.synchronize()
+        n = min(int(cnt.item()), cap)
+        xy = dets[:n, :2].cpu().numpy()
+        return set(map(tuple, xy.tolist()))
+
+    rx_thr = H.rx_threshold(b, 1e-3)
+
+    def outputs(fn):
+        fn()
+        ctx.stats_check()
+        torch.cuda.synchronize()
+        return (ace_s.cpu().numpy(), ace_t.cpu().numpy(), rx_s.cpu().numpy(),
+                detections(rx_s, None, rx_thr), detections(ace_s, ace_t, 0.5))
+
+    ms = {"plain_path": time_on_stream(torch, plain, repeats),
+          "mixed_path": time_on_stream(torch, mixed, repeats),
+          "plain_score": time_on_stream(torch, score_only(H.PLAIN, c32), repeats)}
+    pa, pt, pr, prx, pace = outputs(plain)
+    ms["mixed_score"] = time_on_stream(torch, score_only(H.MIXED, c16), repeats)
+    ma, mt, mr, mrx, mace = outputs(mixed)
+    log("  precision form: " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in ms.items()))
+    recall = {
+        "rx": {"plain_detections": len(prx), "mixed_detections": len(mrx),
+               "recall": round(len(prx & mrx) / max(len(prx), 1), 6)},
+        "ace": {"plain_detections": len(pace), "mixed_detections": len(mace),
+                "recall": round(len(pace & mace) / max(len(pace), 1), 6)},
+    }
+    rel_rx = np.abs(mr.astype(np.float64) - pr) / np.maximum(np.abs(pr.astype(np.float64)), 1.0)
+    ctx.close()
+    pix = p / 1e6
+    return {
+        "workload": "RX+ACE, 1024x1024x224, 16 targets, FP32 frame stored as FP16 on the device; "
+                    "path = moments + FP64 build + ACE with RX side output",
+        "plain": {"path_us": round(ms["plain_path"] * 1e3, 1), "score_us": round(ms["plain_score"] * 1e3, 1),
+                  "path_mpix_s": round(pix / (ms["plain_path"] / 1e3), 1),
+                  "note": "FP32 cube, FP64 moments and FP64-accumulated whitening"},
+        "mixed": {"path_us": round(ms["mixed_path"] * 1e3, 1), "score_us": round(ms["mixed_score"] * 1e3, 1),
+                  "path_mpix_s": round(pix / (ms["mixed_path"] / 1e3), 1)},
+        "speedup_path": round(ms["plain_path"] / ms["mixed_path"], 3),
+        "speedup_score": round(ms["plain_score"] / ms["mixed_score"], 3),
+        "recall_vs_plain": recall,
+        "error_vs_plain_fp64": {"rx_max_rel": float(f"{rel_rx.max():.3g}"),
+                                "ace_max_abs": float(f"{np.abs(ma.astype(np.float64) - pa).max():.3g}"),
+                                "ace_target_agree": round(float((mt == pt).mean()), 6)},
     }
 
 
@@ -319,6 +414,11 @@ def main():
         roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
     roof["traffic"] = round(MEASURED_TRAFFIC_BYTES[roof["kernel"]]) if roof["kernel"] in MEASURED_TRAFFIC_BYTES else None
 
+    prec = None
+    if not args.no_precision:
+        log("precision form (224 bands)")
+        prec = precision_form(H, torch, max(3, r // 4))
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline")
@@ -344,6 +444,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "detectors": detectors,
+        "precision_form": prec,
         "stage_us": {k: round(v * 1e3, 1) for k, v in ms.items()},
     }
     print(json.dumps(line), flush=True)

@@ -111,6 +111,13 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
                            f"(or `make -C hsi-detectors-cuda_amd`)")
+    # torch carries its own HIP runtime (same soname as ROCm's, a different
+    # file). Loading torch first lets this library bind to that one copy;
+    # loading it first would put a second HIP runtime in the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     sig = {
         "hsi_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
