@@ -29,14 +29,21 @@
 namespace hsi_amd {
 namespace {
 
-constexpr int kMxWaves = 4;
-
 // K blocks (32 bands) that meet the lower triangle of 16-row block I.
 __host__ __device__ constexpr int kblocks_for(int I) { return I / 2 + 1; }
 // Number of (I, J) blocks stored before row block I.
 __host__ __device__ constexpr int block_prefix(int I) {
   return (I % 2 == 0) ? (I / 2) * (I / 2 + 1) : (I / 2 + 1) * (I / 2 + 1);
 }
+
+// Launch shape by row-block count: the whitener's LDS copy grows as NB^2 and
+// above 10 blocks only one workgroup fits a CU, so those workgroups carry 16
+// waves (4 per SIMD) and issue MFMAs for 2 row blocks at a time to stay
+// within 128 registers; up to 8 blocks a wave keeps a whole tile of loads in
+// flight (4-wave workgroups, several per CU); in between, 8-wave workgroups
+// stream one K block ahead.
+__host__ __device__ constexpr int mixed_waves(int NB) { return NB > 10 ? 16 : (NB > 8 ? 8 : 4); }
+__host__ __device__ constexpr int mixed_group(int NB) { return NB > 10 ? 2 : 4; }
 
 __device__ __forceinline__ float to_f(float v) { return v; }
 __device__ __forceinline__ float to_f(_Float16 v) { return static_cast<float>(v); }
@@ -58,15 +65,48 @@ __device__ __forceinline__ void ace_merge(double& s, int& t, double s2, int t2) 
   }
 }
 
+// One K block (32 bands) of the whitening for the row blocks [I0, I0 + G):
+// the three split products are issued term by term across the group, so
+// consecutive MFMAs write different accumulators.
+template <int NB, int G>
+__device__ __forceinline__ void mixed_group_mfma(const h8* __restrict__ w_lds, int lane, int J, int I0,
+                                                 const h8& yh, const h8& yl, f4 (&z)[NB]) {
+  h8 ah[G], al[G];
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    const int I = I0 + k;
+    if (I < NB && J < kblocks_for(I)) {
+      const h8* blk = w_lds + (block_prefix(I) + J) * 128;
+      ah[k] = blk[lane];
+      al[k] = blk[64 + lane];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < G; ++k)
+    if (I0 + k < NB && J < kblocks_for(I0 + k))
+      z[I0 + k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[k], yh, z[I0 + k], 0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < G; ++k)
+    if (I0 + k < NB && J < kblocks_for(I0 + k))
+      z[I0 + k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[k], yl, z[I0 + k], 0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < G; ++k)
+    if (I0 + k < NB && J < kblocks_for(I0 + k))
+      z[I0 + k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[k], yh, z[I0 + k], 0, 0, 0);
+}
+
 template <typename In, int NB, int TT>
-__global__ __launch_bounds__(kMxWaves * 64) void rxace_mixed_kernel(
+__global__ __launch_bounds__(mixed_waves(NB) * 64) void rxace_mixed_kernel(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
     int num_targets, const float* __restrict__ mean, const _Float16* __restrict__ white_hl,
     const _Float16* __restrict__ ace_hl, const float* __restrict__ col_scale,
     const float* __restrict__ row_inv, const float* __restrict__ target_inv,
     const float* __restrict__ target_norm2, float* __restrict__ rx_out,
     float* __restrict__ ace_score, int* __restrict__ ace_target) {
+  constexpr int WAVES = mixed_waves(NB);
+  constexpr int G = mixed_group(NB);
   constexpr int NKB = (NB + 1) / 2;           // 32-band K blocks
+  constexpr int PD = NB > 8 ? 1 : NKB;        // K blocks prefetched
   constexpr int NBLK = block_prefix(NB);      // stored (I, J) blocks
   constexpr int NPAIR = (NB + 1) / 2;         // 32-row pairs for ACE
   constexpr int WH = NBLK * 1024;             // halves: hi block then lo block
@@ -96,62 +136,84 @@ __global__ __launch_bounds__(kMxWaves * 64) void rxace_mixed_kernel(
   const int g = lane >> 4;
   const int n = lane & 15;
   const std::size_t tiles = div_up(pixels, 16);
-  const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(kMxWaves) + (threadIdx.x >> 6);
-  const std::size_t waves = static_cast<std::size_t>(gridDim.x) * kMxWaves;
+  const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(WAVES) + (threadIdx.x >> 6);
+  const std::size_t waves = static_cast<std::size_t>(gridDim.x) * WAVES;
 
-  // Raw inputs of a 16-pixel tile, all issued at once: band 32J + 8g + i of
-  // pixel tile*16 + n. The next tile's loads are in flight while the matrix
-  // cores work on this one.
-  auto load_tile = [&](std::size_t t, float (&xr)[NKB][8]) {
-    const std::size_t pp = t * 16 + n;
+  // Raw inputs of one K block of a 16-pixel tile: band 32J + 8g + i of pixel
+  // tile*16 + n. The next K block (of this tile, or block 0 of the next
+  // tile) is in flight while the matrix cores work on the current one.
+  // Per-lane base (band 8g, pixel n); band and tile offsets are
+  // wave-uniform, so no per-band 64-bit address stays live across tiles.
+  const In* lane_base = cube + static_cast<std::size_t>(8 * g) * stride + n;
+  auto load_kblock = [&](std::size_t t, int J, float (&xr)[8]) {
+    const bool pix_ok = t * 16 + n < pixels;
+    const In* ptr = lane_base + t * 16 + static_cast<std::size_t>(32 * J) * stride;
 #pragma unroll
-    for (int J = 0; J < NKB; ++J)
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int b = 32 * J + 8 * g + i;
-        xr[J][i] = (pp < pixels && b < bands) ? to_f(cube[static_cast<std::size_t>(b) * stride + pp]) : 0.f;
-      }
+    for (int i = 0; i < 8; ++i) {
+      // Out-of-range elements load from the cube's first element instead of
+      // branching round the load, and are zeroed after it.
+      const bool ok = pix_ok && (32 * J + 8 * g + i < bands);
+      const In* q = ok ? ptr + static_cast<std::size_t>(i) * stride : cube;
+      const float val = to_f(*q);
+      xr[i] = ok ? val : 0.f;
+    }
   };
-  float cur[NKB][8], nxt[NKB][8];
-  if (wave_id < tiles) load_tile(wave_id, cur);
+  // One K block of the whitening: centre and scale, split, and the MFMAs of
+  // every row block that reaches it (row blocks below 2J do not).
+  auto kblock = [&](int J, const float (&raw)[8], f4 (&z)[NB]) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int b = 32 * J + 8 * g + i;
+      v[i] = (raw[i] - mean_lds[b]) * scale_lds[b];  // padding: scale 0
+    }
+    h8 yh, yl;
+    split8(v, yh, yl);
+#pragma unroll
+    for (int I0 = 2 * J; I0 < NB; I0 += G) {
+      mixed_group_mfma<NB, G>(w_lds, lane, J, I0, yh, yl, z);
+      // one group's whitener fragments live at a time
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  // Small whiteners leave registers for a whole tile in flight (PD = NKB K
+  // blocks); large ones stream one K block ahead.
+  float cur[PD][8], nxt[PD][8];
+  if (wave_id < tiles)
+#pragma unroll
+    for (int J = 0; J < PD; ++J) load_kblock(wave_id, J, cur[J]);
 
   for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
     const std::size_t p = tile * 16 + n;
     const bool live = p < pixels;
-    if (tile + waves < tiles) load_tile(tile + waves, nxt);
     f4 z[NB];
 #pragma unroll
     for (int I = 0; I < NB; ++I) z[I] = f4{0.f, 0.f, 0.f, 0.f};
 
+    if constexpr (PD == NKB) {
+      if (tile + waves < tiles)
 #pragma unroll
-    for (int J = 0; J < NKB; ++J) {
-      float v[8];
+        for (int J = 0; J < NKB; ++J) load_kblock(tile + waves, J, nxt[J]);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int b = 32 * J + 8 * g + i;
-        v[i] = (cur[J][i] - mean_lds[b]) * scale_lds[b];  // padding: scale 0
+      for (int J = 0; J < NKB; ++J) {
+        kblock(J, cur[J], z);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      h8 yh, yl;
-      split8(v, yh, yl);
 #pragma unroll
-      for (int I = 0; I < NB; ++I) {
-        if (J < kblocks_for(I)) {
-          const h8* blk = w_lds + (block_prefix(I) + J) * 128;
-          const h8 ah = blk[lane];
-          const h8 al = blk[64 + lane];
-          z[I] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, yh, z[I], 0, 0, 0);
-          z[I] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, yl, z[I], 0, 0, 0);
-          z[I] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, yh, z[I], 0, 0, 0);
-        }
+      for (int J = 0; J < NKB; ++J)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cur[J][i] = nxt[J][i];
+    } else {
+#pragma unroll
+      for (int J = 0; J < NKB; ++J) {
+        if (J + 1 < NKB) load_kblock(tile, J + 1, nxt[0]);
+        else if (tile + waves < tiles) load_kblock(tile + waves, 0, nxt[0]);
+        kblock(J, cur[0], z);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cur[0][i] = nxt[0][i];
+        __builtin_amdgcn_sched_barrier(0);
       }
-      // Keep each K block's whitener loads next to their MFMAs (hoisting them
-      // all would need ~160 registers).
-      __builtin_amdgcn_sched_barrier(0);
     }
-#pragma unroll
-    for (int J = 0; J < NKB; ++J)
-#pragma unroll
-      for (int i = 0; i < 8; ++i) cur[J][i] = nxt[J][i];
 
     double zz = 0.0;
 #pragma unroll
@@ -222,11 +284,13 @@ __global__ __launch_bounds__(kMxWaves * 64) void rxace_mixed_kernel(
 
 template <typename In, int NB, int TT>
 hipError_t launch_mixed_t(const RxAceArgs& a, hipStream_t stream) {
+  constexpr int WAVES = mixed_waves(NB);
   const std::size_t tiles = div_up(a.pixels, 16);
-  std::size_t wgs = div_up(tiles, kMxWaves);
-  if (wgs > 1024) wgs = 1024;
+  std::size_t wgs = div_up(tiles, WAVES);
+  const std::size_t cap = NB > 10 ? 256 : 1024;  // one resident workgroup per CU above 10 blocks
+  if (wgs > cap) wgs = cap;
   hipLaunchKernelGGL((rxace_mixed_kernel<In, NB, TT>), dim3(static_cast<unsigned>(wgs)),
-                     dim3(kMxWaves * 64), 0, stream, static_cast<const In*>(a.cube), a.pixels,
+                     dim3(WAVES * 64), 0, stream, static_cast<const In*>(a.cube), a.pixels,
                      a.bands, a.stride, a.num_targets, a.mean, a.white_hl, a.ace_hl, a.col_scale,
                      a.row_inv, a.target_inv, a.target_norm2, a.rx, a.ace_score, a.ace_target);
   return hipGetLastError();
