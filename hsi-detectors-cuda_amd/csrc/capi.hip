@@ -51,6 +51,20 @@ struct hsi_ctx {
   float* d_row_inv = nullptr;
   float* d_target_inv = nullptr;
 
+  // Learned pre-screen: FP32 masters, Adam moments, gradients, bf16 layouts.
+  int mlp_bands = 0;
+  long long mlp_step = 0;
+  bool mlp_dirty = false;
+  float* d_mlp_w = nullptr;
+  float* d_mlp_m = nullptr;
+  float* d_mlp_v = nullptr;
+  float* d_mlp_grad = nullptr;
+  void* d_mlp_a1 = nullptr;
+  void* d_mlp_a2 = nullptr;
+  float* d_mlp_tail = nullptr;
+  float* d_mlp_act = nullptr;
+  std::size_t mlp_act_bytes = 0;
+
   // Scratch sized per frame.
   double* d_mom_scratch = nullptr;
   std::size_t mom_scratch_bytes = 0;
@@ -151,7 +165,8 @@ void hsi_ctx_destroy(hsi_ctx* ctx) {
                   ctx->d_target_norm2, ctx->d_cem_weight, ctx->d_status, ctx->d_build_work,
                   ctx->d_cem_lib, ctx->d_white_blocks, ctx->d_ace_u, ctx->d_white_hl,
                   ctx->d_ace_hl, ctx->d_col_scale, ctx->d_row_inv, ctx->d_target_inv,
-                  ctx->d_mom_scratch, ctx->d_det_scratch};
+                  ctx->d_mom_scratch, ctx->d_det_scratch, ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v,
+                  ctx->d_mlp_grad, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, ctx->d_mlp_act};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete ctx;
@@ -615,6 +630,252 @@ hsi_status hsi_transpose_bip_to_bsq(const float* d_src, float* d_dst, int height
   if (launch_transpose_bip_to_bsq(d_src, d_dst, height, width, bands, plane_stride,
                                   static_cast<hipStream_t>(stream)) != hipSuccess)
     return HSI_ERR_DEVICE;
+  return HSI_OK;
+}
+
+}  // extern "C"
+
+// ---- learned pre-screen ------------------------------------------------------
+
+namespace {
+
+std::uint64_t splitmix(std::uint64_t& state) {
+  std::uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+hsi_status mlp_alloc(hsi_ctx* ctx) {
+  if (ctx->d_mlp_w) return HSI_OK;
+  const std::size_t P = mlp_param_count(kMaxBands);
+  HSI_HIP(ctx, alloc(&ctx->d_mlp_w, P), "allocate pre-screen weights");
+  HSI_HIP(ctx, alloc(&ctx->d_mlp_m, P), "allocate pre-screen moments");
+  HSI_HIP(ctx, alloc(&ctx->d_mlp_v, P), "allocate pre-screen moments");
+  HSI_HIP(ctx, alloc(&ctx->d_mlp_grad, P), "allocate pre-screen gradients");
+  HSI_HIP(ctx, hipMalloc(&ctx->d_mlp_a1, mlp_a1_halves(kMaxBands) * 2 + 256), "allocate pre-screen layouts");
+  HSI_HIP(ctx, hipMalloc(&ctx->d_mlp_a2, mlp_a2_halves() * 2 + 256), "allocate pre-screen layouts");
+  HSI_HIP(ctx, alloc(&ctx->d_mlp_tail, mlp_tail_floats()), "allocate pre-screen layouts");
+  return HSI_OK;
+}
+
+hsi_status mlp_install(hsi_ctx* ctx, int bands, const std::vector<float>& w) {
+  hsi_status st = mlp_alloc(ctx);
+  if (st != HSI_OK) return st;
+  const std::size_t P = w.size();
+  HSI_HIP(ctx, hipMemcpy(ctx->d_mlp_w, w.data(), P * sizeof(float), hipMemcpyHostToDevice), "upload weights");
+  HSI_HIP(ctx, hipMemset(ctx->d_mlp_m, 0, P * sizeof(float)), "reset moments");
+  HSI_HIP(ctx, hipMemset(ctx->d_mlp_v, 0, P * sizeof(float)), "reset moments");
+  HSI_HIP(ctx, hipDeviceSynchronize(), "install weights");
+  ctx->mlp_bands = bands;
+  ctx->mlp_step = 0;
+  ctx->mlp_dirty = true;
+  return HSI_OK;
+}
+
+hsi_status mlp_ready(hsi_ctx* ctx, int bands) {
+  if (ctx->mlp_bands <= 0) return fail(ctx, HSI_ERR_INVALID_ARG, "no pre-screen network: call hsi_mlp_init or hsi_mlp_set_weights");
+  if (ctx->mlp_bands != bands)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "pre-screen network takes " + std::to_string(ctx->mlp_bands) +
+                                              " bands, input has " + std::to_string(bands));
+  return HSI_OK;
+}
+
+hsi_status mlp_check_samples(hsi_ctx* ctx, const float* d_x, std::size_t x_stride, int n, const float* d_labels) {
+  if (!d_x || !d_labels || n <= 0 || x_stride < static_cast<std::size_t>(n))
+    return fail(ctx, HSI_ERR_INVALID_ARG, "training batch needs samples, labels, n > 0 and x_stride >= n");
+  return HSI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t hsi_mlp_param_count(int bands) {
+  return bands > 0 && bands <= kMaxBands ? mlp_param_count(bands) : 0;
+}
+
+hsi_status hsi_mlp_init(hsi_ctx* ctx, int bands, uint64_t seed) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (bands <= 0 || bands > kMaxBands) return fail(ctx, HSI_ERR_INVALID_ARG, "pre-screen bands out of range");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  std::vector<float> w(mlp_param_count(bands), 0.f);
+  std::uint64_t state = seed;
+  auto uniform = [&](float limit) {
+    const double u = static_cast<double>(splitmix(state) >> 11) * (1.0 / 9007199254740992.0);
+    return static_cast<float>((2.0 * u - 1.0) * limit);
+  };
+  // He-uniform for the ReLU layers, Glorot-uniform for the output; zero biases.
+  const std::size_t o_b1 = 64u * bands, o_w2 = o_b1 + 64, o_b2 = o_w2 + 32 * 64, o_w3 = o_b2 + 32;
+  for (std::size_t i = 0; i < o_b1; ++i) w[i] = uniform(std::sqrt(6.0f / bands));
+  for (std::size_t i = o_w2; i < o_b2; ++i) w[i] = uniform(std::sqrt(6.0f / 64.0f));
+  for (std::size_t i = o_w3; i < o_w3 + 32; ++i) w[i] = uniform(std::sqrt(6.0f / 33.0f));
+  return mlp_install(ctx, bands, w);
+}
+
+hsi_status hsi_mlp_set_weights(hsi_ctx* ctx, int bands, const float* weights) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (bands <= 0 || bands > kMaxBands || !weights)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "pre-screen weights need bands in range and a buffer");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  std::vector<float> w(weights, weights + mlp_param_count(bands));
+  return mlp_install(ctx, bands, w);
+}
+
+hsi_status hsi_mlp_get_weights(hsi_ctx* ctx, float* weights) {
+  if (!ctx || !weights) return fail(ctx, HSI_ERR_INVALID_ARG, "null buffer");
+  if (ctx->mlp_bands <= 0) return fail(ctx, HSI_ERR_INVALID_ARG, "no pre-screen network");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, hipDeviceSynchronize(), "wait for device");
+  HSI_HIP(ctx, hipMemcpy(weights, ctx->d_mlp_w, mlp_param_count(ctx->mlp_bands) * sizeof(float),
+                         hipMemcpyDeviceToHost),
+          "read weights");
+  return HSI_OK;
+}
+
+static hsi_status mlp_grad_common(hsi_ctx* ctx, const float* d_x, size_t x_stride, int n, const float* d_labels,
+                                  float pos_weight, float* d_grad, float* d_loss, hipStream_t s) {
+  HSI_HIP(ctx, grow(reinterpret_cast<void**>(&ctx->d_mlp_act), &ctx->mlp_act_bytes,
+                    mlp_act_floats(n) * sizeof(float)),
+          "allocate training scratch");
+  HSI_HIP(ctx, launch_mlp_gradients(d_x, x_stride, n, ctx->mlp_bands, d_labels, pos_weight, ctx->d_mlp_w,
+                                    ctx->d_mlp_act, d_grad, d_loss, s),
+          "pre-screen gradients");
+  return HSI_OK;
+}
+
+hsi_status hsi_mlp_gradients(hsi_ctx* ctx, const float* d_x, size_t x_stride, int n, const float* d_labels,
+                             float pos_weight, float* d_grad, float* d_loss, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  hsi_status st = mlp_ready(ctx, ctx->mlp_bands);
+  if (st != HSI_OK) return st;
+  if ((st = mlp_check_samples(ctx, d_x, x_stride, n, d_labels)) != HSI_OK) return st;
+  if (!d_grad) return fail(ctx, HSI_ERR_INVALID_ARG, "null gradient buffer");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  return mlp_grad_common(ctx, d_x, x_stride, n, d_labels, pos_weight, d_grad, d_loss,
+                         static_cast<hipStream_t>(stream));
+}
+
+hsi_status hsi_mlp_train_step(hsi_ctx* ctx, const float* d_x, size_t x_stride, int n, const float* d_labels,
+                              const hsi_mlp_train_params* params, float* d_loss, hsi_stream stream) {
+  if (!ctx || !params) return HSI_ERR_INVALID_ARG;
+  hsi_status st = mlp_ready(ctx, ctx->mlp_bands);
+  if (st != HSI_OK) return st;
+  if ((st = mlp_check_samples(ctx, d_x, x_stride, n, d_labels)) != HSI_OK) return st;
+  if (!(params->learning_rate > 0.f) || !(params->beta1 >= 0.f && params->beta1 < 1.f) ||
+      !(params->beta2 >= 0.f && params->beta2 < 1.f) || !(params->epsilon > 0.f))
+    return fail(ctx, HSI_ERR_INVALID_ARG, "Adam needs lr > 0, betas in [0, 1), epsilon > 0");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((st = mlp_grad_common(ctx, d_x, x_stride, n, d_labels, params->pos_weight, ctx->d_mlp_grad, d_loss, s)) != HSI_OK)
+    return st;
+  ++ctx->mlp_step;
+  const double t = static_cast<double>(ctx->mlp_step);
+  const float c1 = static_cast<float>(1.0 / (1.0 - std::pow(static_cast<double>(params->beta1), t)));
+  const float c2 = static_cast<float>(1.0 / (1.0 - std::pow(static_cast<double>(params->beta2), t)));
+  HSI_HIP(ctx, launch_adam(ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v, ctx->d_mlp_grad,
+                           mlp_param_count(ctx->mlp_bands), params->learning_rate, params->beta1, params->beta2,
+                           params->epsilon, c1, c2, s),
+          "Adam update");
+  ctx->mlp_dirty = true;
+  return HSI_OK;
+}
+
+hsi_status hsi_mlp_infer(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype, int height, int width, int bands,
+                         size_t plane_stride, int center_on_statistics_mean, float* d_logit, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, dtype);
+  if (st != HSI_OK) return st;
+  if ((st = mlp_ready(ctx, bands)) != HSI_OK) return st;
+  if (!d_logit) return fail(ctx, HSI_ERR_INVALID_ARG, "null logit map");
+  if (center_on_statistics_mean && ctx->stats_bands != bands)
+    return fail(ctx, HSI_ERR_NO_STATISTICS, "centring on the scene mean needs statistics for this band count");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (ctx->mlp_dirty) {
+    HSI_HIP(ctx, launch_mlp_layouts(ctx->d_mlp_w, bands, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, s),
+            "pre-screen layouts");
+    ctx->mlp_dirty = false;
+  }
+  MlpInferArgs a;
+  a.cube = d_cube;
+  a.dtype = dtype;
+  a.pixels = static_cast<std::size_t>(height) * width;
+  a.bands = bands;
+  a.stride = plane_stride;
+  a.center = center_on_statistics_mean ? ctx->d_mean : nullptr;
+  a.a1 = ctx->d_mlp_a1;
+  a.a2 = ctx->d_mlp_a2;
+  a.tail = ctx->d_mlp_tail;
+  a.logit = d_logit;
+  HSI_HIP(ctx, launch_mlp_infer(a, s), "pre-screen inference");
+  return HSI_OK;
+}
+
+hsi_status hsi_gather_pixels(hsi_ctx* ctx, const float* d_cube, int height, int width, int bands,
+                             size_t plane_stride, const int32_t* d_index, int n, int center_on_statistics_mean,
+                             float* d_out, size_t out_stride, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, HSI_F32);
+  if (st != HSI_OK) return st;
+  if (!d_index || !d_out || n < 0 || out_stride < static_cast<std::size_t>(n))
+    return fail(ctx, HSI_ERR_INVALID_ARG, "gather needs an index list, an output and out_stride >= n");
+  if (center_on_statistics_mean && ctx->stats_bands != bands)
+    return fail(ctx, HSI_ERR_NO_STATISTICS, "centring on the scene mean needs statistics for this band count");
+  if (n == 0) return HSI_OK;
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, launch_gather(d_cube, plane_stride, static_cast<std::size_t>(height) * width, bands, d_index, n,
+                             center_on_statistics_mean ? ctx->d_mean : nullptr,
+                             d_out, out_stride, static_cast<hipStream_t>(stream)),
+          "gather pixels");
+  return HSI_OK;
+}
+
+hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_cube, int height, int width,
+                                int bands, size_t plane_stride, const hsi_detection* d_candidates,
+                                const uint32_t* d_count, int capacity, float* d_score, int32_t* d_target,
+                                float* d_aux, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (detector != HSI_DET_RX && detector != HSI_DET_ACE)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "candidate rescoring applies to RX and ACE");
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, HSI_F32);
+  if (st != HSI_OK) return st;
+  if (!d_candidates || !d_count || capacity < 0 || !d_score)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "candidate rescoring needs a list, its count and a score map");
+  if (bands > kMaxRxBands)
+    return fail(ctx, HSI_ERR_CAPACITY, "RX/ACE support at most " + std::to_string(kMaxRxBands) + " bands");
+  if (ctx->stats_bands != bands)
+    return fail(ctx, HSI_ERR_NO_STATISTICS, "RX/ACE need statistics built for this band count");
+  if (detector == HSI_DET_ACE && ctx->stats_targets <= 0)
+    return fail(ctx, HSI_ERR_NO_LIBRARY, "ACE needs a target library");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((st = ensure_layouts(ctx, s)) != HSI_OK) return st;
+  const std::size_t pixels = static_cast<std::size_t>(height) * width;
+  HSI_HIP(ctx, launch_fill_scores(d_score, detector == HSI_DET_ACE ? d_target : nullptr,
+                                  detector == HSI_DET_ACE ? d_aux : nullptr, pixels, s),
+          "clear score maps");
+  if (capacity == 0) return HSI_OK;
+  RxAceArgs a;
+  a.cube = d_cube;
+  a.dtype = HSI_F32;
+  a.pixels = pixels;
+  a.bands = bands;
+  a.stride = plane_stride;
+  a.num_targets = detector == HSI_DET_ACE ? ctx->stats_targets : 0;
+  a.mean = ctx->d_mean;
+  a.white_blocks = ctx->d_white_blocks;
+  a.ace_u = ctx->d_ace_u;
+  a.target_norm2 = ctx->d_target_norm2;
+  if (detector == HSI_DET_RX) {
+    a.rx = d_score;
+  } else {
+    a.rx = d_aux;
+    a.ace_score = d_score;
+    a.ace_target = d_target;
+  }
+  HSI_HIP(ctx, launch_rxace_candidates(a, d_candidates, d_count, static_cast<unsigned>(capacity), width, s),
+          "rescore candidates");
   return HSI_OK;
 }
 

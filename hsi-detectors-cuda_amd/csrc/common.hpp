@@ -26,6 +26,25 @@ constexpr int kWave = 64;
 __host__ __device__ inline std::size_t div_up(std::size_t a, std::size_t b) { return (a + b - 1) / b; }
 __host__ __device__ inline std::size_t round_up(std::size_t a, std::size_t b) { return div_up(a, b) * b; }
 
+// Raw buffer view of a whole BSQ cube for buffer_load: every lane supplies a
+// 32-bit byte offset and loads at or past `bytes` return 0, so kernels that
+// read padded K blocks or ragged tiles need no guard around the load. The
+// cube must be smaller than 4 GiB (checked by the launchers).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t cube_rsrc(const void* base, std::uint32_t bytes) {
+  // 0x00020000: the gfx9 word 3 for raw (untyped) 32-bit buffer access.
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+__device__ __forceinline__ float buffer_elem(__amdgpu_buffer_rsrc_t r, std::uint32_t off, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ float buffer_elem(__amdgpu_buffer_rsrc_t r, std::uint32_t off, _Float16) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)));
+}
+// Whether a cube of `bands` planes of `stride` elements fits one buffer view.
+__host__ __device__ inline bool fits_buffer(int bands, std::size_t stride, std::size_t elem_bytes) {
+  return static_cast<std::size_t>(bands) * stride * elem_bytes < (static_cast<std::size_t>(1) << 32) - 256;
+}
+
 // float(pi/2) widened: the angle the reference reports for a pixel or target
 // with no direction (include/hsi/sam.hpp:25).
 constexpr double kMaxSamAngle = static_cast<double>(1.57079632679489661923f);

@@ -125,7 +125,41 @@ struct RxAceArgs {
   int* ace_target = nullptr;
 };
 hipError_t launch_rxace_plain(const RxAceArgs& args, hipStream_t stream);
+// The plain kernel over a candidate list (raster-order detections of a
+// pre-screen): scores only pixels (y * width + x) of the first
+// min(*count, capacity) records, written at their own positions.
+hipError_t launch_rxace_candidates(const RxAceArgs& args, const hsi_detection* cand, const unsigned* count,
+                                   unsigned capacity, int width, hipStream_t stream);
+hipError_t launch_fill_scores(float* score, int* target, float* aux, std::size_t pixels, hipStream_t stream);
 hipError_t launch_rxace_mixed(const RxAceArgs& args, hipStream_t stream);
+
+// ---- learned pre-screen (mlp.hip) --------------------------------------------
+struct MlpInferArgs {
+  const void* cube = nullptr;
+  hsi_dtype dtype = HSI_F32;
+  std::size_t pixels = 0;
+  int bands = 0;
+  std::size_t stride = 0;
+  const float* center = nullptr;  // nullable: no centring
+  const void* a1 = nullptr;       // bf16 layer-1 fragments
+  const void* a2 = nullptr;       // bf16 layer-2 fragments
+  const float* tail = nullptr;    // b1 | b2 | w3 | b3
+  float* logit = nullptr;
+};
+std::size_t mlp_param_count(int bands);
+std::size_t mlp_a1_halves(int bands);
+std::size_t mlp_a2_halves();
+std::size_t mlp_tail_floats();
+std::size_t mlp_act_floats(int n);
+hipError_t launch_mlp_layouts(const float* w, int bands, void* a1, void* a2, float* tail, hipStream_t stream);
+hipError_t launch_mlp_infer(const MlpInferArgs& args, hipStream_t stream);
+hipError_t launch_mlp_gradients(const float* x, std::size_t x_stride, int n, int bands, const float* labels,
+                                float pos_weight, const float* w, float* act, float* grad, float* loss,
+                                hipStream_t stream);
+hipError_t launch_adam(float* w, float* m, float* v, const float* g, std::size_t count, float lr, float beta1,
+                       float beta2, float eps, float c1, float c2, hipStream_t stream);
+hipError_t launch_gather(const float* cube, std::size_t stride, std::size_t pixels, int bands, const int* index,
+                         int n, const float* center, float* out, std::size_t out_stride, hipStream_t stream);
 
 // ---- detect ------------------------------------------------------------------
 std::size_t detect_scratch_words(std::size_t pixels);

@@ -37,12 +37,17 @@ __device__ __forceinline__ void ace_merge(double& s, int& t, double s2, int t2) 
   }
 }
 
-template <typename In, int NB, int TT>
+// GATHER: score only the pixels named by a candidate list (detection records
+// in raster order, pixel y * width + x), min(*cand_count, cand_cap) of them;
+// each 16-record slice of the list is one tile.
+template <typename In, int NB, int TT, bool GATHER>
 __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
     int num_targets, const float* __restrict__ mean, const float* __restrict__ white_blocks,
     const float* __restrict__ ace_u, const float* __restrict__ target_norm2,
-    float* __restrict__ rx_out, float* __restrict__ ace_score, int* __restrict__ ace_target) {
+    float* __restrict__ rx_out, float* __restrict__ ace_score, int* __restrict__ ace_target,
+    const hsi_detection* __restrict__ cand, const unsigned* __restrict__ cand_count, unsigned cand_cap,
+    int width) {
   constexpr int NPAIR = NB * (NB + 1) / 2;
   constexpr int WFLOATS = NPAIR * 256;
   constexpr int UFLOATS = (TT > 0 ? TT : 1) * NB * 256;
@@ -59,7 +64,27 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;   // lane group: K index of the A/B operands
   const int n = lane & 15;   // pixel column within the tile
-  const std::size_t tiles = div_up(pixels, 16);
+  std::size_t items = pixels;
+  if constexpr (GATHER) {
+    const unsigned c = *cand_count;
+    items = c < cand_cap ? c : cand_cap;
+  }
+  const std::size_t tiles = div_up(items, 16);
+  // Pixel of tile element t*16 + n, and whether it exists.
+  auto pixel_of = [&](std::size_t t, bool& ok) -> std::size_t {
+    const std::size_t i = t * 16 + n;
+    ok = i < items;
+    if constexpr (GATHER) {
+      if (!ok) return 0;
+      const hsi_detection d = cand[i];
+      // a record outside the frame names no pixel
+      const std::size_t rows = pixels / static_cast<std::size_t>(width);
+      ok = d.x >= 0 && d.x < width && d.y >= 0 && static_cast<std::size_t>(d.y) < rows;
+      return ok ? static_cast<std::size_t>(d.y) * static_cast<std::size_t>(width) + static_cast<std::size_t>(d.x) : 0;
+    } else {
+      return i;
+    }
+  };
   const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(kRxWaves) + (threadIdx.x >> 6);
   const std::size_t waves = static_cast<std::size_t>(gridDim.x) * kRxWaves;
 
@@ -78,21 +103,22 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
   // pixel tile*16 + n (the B-operand layout). The next tile's loads are in
   // flight while the matrix cores work on this one.
   auto load_tile = [&](std::size_t t, float (&xr)[NB][4]) {
-    const std::size_t pp = t * 16 + n;
+    bool pix_ok;
+    const std::size_t pp = pixel_of(t, pix_ok);
 #pragma unroll
     for (int J = 0; J < NB; ++J)
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         const int b = 16 * J + 4 * kk + g;
-        xr[J][kk] = (pp < pixels && b < bands) ? static_cast<float>(cube[static_cast<std::size_t>(b) * stride + pp]) : 0.f;
+        xr[J][kk] = (pix_ok && b < bands) ? static_cast<float>(cube[static_cast<std::size_t>(b) * stride + pp]) : 0.f;
       }
   };
   float cur[NB][4], nxt[NB][4];
   if (wave_id < tiles) load_tile(wave_id, cur);
 
   for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
-    const std::size_t p = tile * 16 + n;
-    const bool live = p < pixels;
+    bool live;
+    const std::size_t p = pixel_of(tile, live);
     if (tile + waves < tiles) load_tile(tile + waves, nxt);
     // Z = W Y accumulated one input band block J at a time.
     d4 z[NB];
@@ -177,29 +203,40 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
   }
 }
 
-template <typename In, int NB, int TT>
-hipError_t launch_plain_t(const RxAceArgs& a, hipStream_t stream) {
-  const std::size_t tiles = div_up(a.pixels, 16);
+struct Gather {
+  const hsi_detection* cand = nullptr;
+  const unsigned* count = nullptr;
+  unsigned capacity = 0;
+  int width = 0;
+};
+
+template <typename In, int NB, int TT, bool GATHER>
+hipError_t launch_plain_t(const RxAceArgs& a, const Gather& gth, hipStream_t stream) {
+  // A candidate list's length is known only on the device: size the grid for
+  // its capacity and let the kernel stop at the count.
+  const std::size_t tiles = div_up(GATHER ? static_cast<std::size_t>(gth.capacity) : a.pixels, 16);
   std::size_t wgs = div_up(tiles, kRxWaves);
   if (wgs > 1024) wgs = 1024;
-  hipLaunchKernelGGL((rxace_plain_kernel<In, NB, TT>), dim3(static_cast<unsigned>(wgs)),
+  if (wgs == 0) wgs = 1;
+  hipLaunchKernelGGL((rxace_plain_kernel<In, NB, TT, GATHER>), dim3(static_cast<unsigned>(wgs)),
                      dim3(kRxWaves * 64), 0, stream, static_cast<const In*>(a.cube), a.pixels,
                      a.bands, a.stride, a.num_targets, a.mean, a.white_blocks, a.ace_u,
-                     a.target_norm2, a.rx, a.ace_score, a.ace_target);
+                     a.target_norm2, a.rx, a.ace_score, a.ace_target, gth.cand, gth.count, gth.capacity,
+                     gth.width);
   return hipGetLastError();
 }
 
-template <typename In, int NB>
-hipError_t launch_plain_nb(const RxAceArgs& a, hipStream_t stream) {
-  if (a.num_targets == 0 || (!a.ace_score && !a.ace_target)) return launch_plain_t<In, NB, 0>(a, stream);
-  if (a.num_targets <= 16) return launch_plain_t<In, NB, 1>(a, stream);
-  return launch_plain_t<In, NB, 2>(a, stream);
+template <typename In, int NB, bool GATHER>
+hipError_t launch_plain_nb(const RxAceArgs& a, const Gather& gth, hipStream_t stream) {
+  if (a.num_targets == 0 || (!a.ace_score && !a.ace_target)) return launch_plain_t<In, NB, 0, GATHER>(a, gth, stream);
+  if (a.num_targets <= 16) return launch_plain_t<In, NB, 1, GATHER>(a, gth, stream);
+  return launch_plain_t<In, NB, 2, GATHER>(a, gth, stream);
 }
 
-template <typename In>
-hipError_t launch_plain_typed(const RxAceArgs& a, hipStream_t stream) {
+template <typename In, bool GATHER>
+hipError_t launch_plain_typed(const RxAceArgs& a, const Gather& gth, hipStream_t stream) {
   switch (rx_blocks(a.bands)) {
-#define HSI_NB(N) case N: return launch_plain_nb<In, N>(a, stream);
+#define HSI_NB(N) case N: return launch_plain_nb<In, N, GATHER>(a, gth, stream);
     HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8)
     HSI_NB(9) HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14)
 #undef HSI_NB
@@ -265,8 +302,37 @@ std::size_t rxace_target_floats(int bands, int num_targets) {
 }
 
 hipError_t launch_rxace_plain(const RxAceArgs& a, hipStream_t stream) {
-  if (a.dtype == HSI_F16) return launch_plain_typed<_Float16>(a, stream);
-  return launch_plain_typed<float>(a, stream);
+  const Gather none;
+  if (a.dtype == HSI_F16) return launch_plain_typed<_Float16, false>(a, none, stream);
+  return launch_plain_typed<float, false>(a, none, stream);
+}
+
+hipError_t launch_rxace_candidates(const RxAceArgs& a, const hsi_detection* cand, const unsigned* count,
+                                   unsigned capacity, int width, hipStream_t stream) {
+  if (a.dtype != HSI_F32) return hipErrorInvalidValue;  // candidates are rescored from the FP32 cube
+  Gather gth;
+  gth.cand = cand;
+  gth.count = count;
+  gth.capacity = capacity;
+  gth.width = width;
+  return launch_plain_typed<float, true>(a, gth, stream);
+}
+
+namespace {
+__global__ void fill_scores_kernel(float* __restrict__ score, int* __restrict__ target, float* __restrict__ aux,
+                                   std::size_t pixels) {
+  const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+  if (i >= pixels) return;
+  score[i] = 0.f;
+  if (target) target[i] = -1;
+  if (aux) aux[i] = 0.f;
+}
+}  // namespace
+
+hipError_t launch_fill_scores(float* score, int* target, float* aux, std::size_t pixels, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_scores_kernel, dim3(static_cast<unsigned>(div_up(pixels, 256))), dim3(256), 0, stream,
+                     score, target, aux, pixels);
+  return hipGetLastError();
 }
 
 hipError_t launch_layouts(const LayoutArgs& a, hipStream_t stream) {

@@ -142,21 +142,17 @@ __global__ __launch_bounds__(mixed_waves(NB) * 64) void rxace_mixed_kernel(
   // Raw inputs of one K block of a 16-pixel tile: band 32J + 8g + i of pixel
   // tile*16 + n. The next K block (of this tile, or block 0 of the next
   // tile) is in flight while the matrix cores work on the current one.
-  // Per-lane base (band 8g, pixel n); band and tile offsets are
-  // wave-uniform, so no per-band 64-bit address stays live across tiles.
-  const In* lane_base = cube + static_cast<std::size_t>(8 * g) * stride + n;
+  // Raw inputs of one K block of a 16-pixel tile: band 32J + 8g + i of pixel
+  // tile*16 + n, through a buffer view of the cube. Bands past the last read
+  // 0 (beyond the view); pixels past the last read padding or the next
+  // plane, which only ever reaches their own (unwritten) output column.
+  const __amdgpu_buffer_rsrc_t rsrc = cube_rsrc(cube, static_cast<std::uint32_t>(bands * stride * sizeof(In)));
+  const std::uint32_t lane_off = static_cast<std::uint32_t>((static_cast<std::size_t>(8 * g) * stride + n) * sizeof(In));
   auto load_kblock = [&](std::size_t t, int J, float (&xr)[8]) {
-    const bool pix_ok = t * 16 + n < pixels;
-    const In* ptr = lane_base + t * 16 + static_cast<std::size_t>(32 * J) * stride;
+    const std::uint32_t base = lane_off + static_cast<std::uint32_t>((t * 16 + static_cast<std::size_t>(32 * J) * stride) * sizeof(In));
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      // Out-of-range elements load from the cube's first element instead of
-      // branching round the load, and are zeroed after it.
-      const bool ok = pix_ok && (32 * J + 8 * g + i < bands);
-      const In* q = ok ? ptr + static_cast<std::size_t>(i) * stride : cube;
-      const float val = to_f(*q);
-      xr[i] = ok ? val : 0.f;
-    }
+    for (int i = 0; i < 8; ++i)
+      xr[i] = buffer_elem(rsrc, base + static_cast<std::uint32_t>(static_cast<std::size_t>(i) * stride * sizeof(In)), In{});
   };
   // One K block of the whitening: centre and scale, split, and the MFMAs of
   // every row block that reaches it (row blocks below 2J do not).
@@ -284,6 +280,7 @@ __global__ __launch_bounds__(mixed_waves(NB) * 64) void rxace_mixed_kernel(
 
 template <typename In, int NB, int TT>
 hipError_t launch_mixed_t(const RxAceArgs& a, hipStream_t stream) {
+  if (!fits_buffer(a.bands, a.stride, sizeof(In))) return hipErrorInvalidValue;
   constexpr int WAVES = mixed_waves(NB);
   const std::size_t tiles = div_up(a.pixels, 16);
   std::size_t wgs = div_up(tiles, WAVES);

@@ -92,6 +92,16 @@ class DetectionParams(C.Structure):
         super().__init__(threshold, nms_radius, max_detections, int(bool(higher_is_better)))
 
 
+class MlpTrainParams(C.Structure):
+    """hsi_mlp_train_params (Adam + positive-class weight)."""
+
+    _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("epsilon", C.c_float), ("pos_weight", C.c_float)]
+
+    def __init__(self, learning_rate=1e-3, beta1=0.9, beta2=0.999, epsilon=1e-8, pos_weight=1.0):
+        super().__init__(learning_rate, beta1, beta2, epsilon, pos_weight)
+
+
 class SynthConfig(C.Structure):
     _fields_ = [("height", C.c_int32), ("width", C.c_int32), ("bands", C.c_int32),
                 ("num_targets", C.c_int32), ("num_endmembers", C.c_int32),
@@ -144,6 +154,21 @@ def lib():
         "hsi_transpose_bip_to_bsq": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t,
                                                _vp]),
         "hsi_rx_threshold": (C.c_double, [C.c_int, C.c_double]),
+        "hsi_stats_status_copy": (C.c_int, [_vp, _vp, _vp]),
+        "hsi_mlp_param_count": (C.c_size_t, [C.c_int]),
+        "hsi_mlp_init": (C.c_int, [_vp, C.c_int, C.c_uint64]),
+        "hsi_mlp_set_weights": (C.c_int, [_vp, C.c_int, _vp]),
+        "hsi_mlp_get_weights": (C.c_int, [_vp, _vp]),
+        "hsi_mlp_gradients": (C.c_int, [_vp, _vp, C.c_size_t, C.c_int, _vp, C.c_float, _vp, _vp,
+                                        _vp]),
+        "hsi_mlp_train_step": (C.c_int, [_vp, _vp, C.c_size_t, C.c_int, _vp,
+                                         C.POINTER(MlpTrainParams), _vp, _vp]),
+        "hsi_mlp_infer": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t,
+                                    C.c_int, _vp, _vp]),
+        "hsi_gather_pixels": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t, _vp,
+                                        C.c_int, C.c_int, _vp, C.c_size_t, _vp]),
+        "hsi_score_candidates": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+                                           C.c_size_t, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
         "hsi_synth_default": (None, [C.POINTER(SynthConfig)]),
         "hsi_synth_library": (C.c_int, [C.POINTER(SynthConfig), _vp]),
         "hsi_synth_anomaly": (C.c_int, [C.POINTER(SynthConfig), _vp]),
@@ -342,6 +367,58 @@ class Context:
     def detect(self, score, target, height, width, params, out, count, stream=None):
         self._check(self._lib.hsi_detect(self.handle, _ptr(score), _ptr(target), height, width,
                                          params, _ptr(out), _ptr(count), _stream(stream)))
+
+    # learned pre-screen
+    def mlp_init(self, bands, seed=1):
+        self._check(self._lib.hsi_mlp_init(self.handle, bands, seed))
+        self.mlp_bands = bands
+
+    def mlp_set_weights(self, bands, weights):
+        w = np.ascontiguousarray(weights, np.float32)
+        assert w.size == mlp_param_count(bands)
+        self._check(self._lib.hsi_mlp_set_weights(self.handle, bands, w.ctypes.data))
+        self.mlp_bands = bands
+
+    def mlp_get_weights(self):
+        w = np.empty(mlp_param_count(self.mlp_bands), np.float32)
+        self._check(self._lib.hsi_mlp_get_weights(self.handle, w.ctypes.data))
+        return w
+
+    def mlp_gradients(self, x, labels, pos_weight, grad, loss=None, n=None, stream=None):
+        """x: device (bands, >= n) float32 tensor, band-major samples."""
+        n = labels.numel() if n is None else n
+        self._check(self._lib.hsi_mlp_gradients(self.handle, _ptr(x), x.stride(0), n, _ptr(labels),
+                                                pos_weight, _ptr(grad), _ptr(loss),
+                                                _stream(stream)))
+
+    def mlp_train_step(self, x, labels, params, loss=None, n=None, stream=None):
+        n = labels.numel() if n is None else n
+        self._check(self._lib.hsi_mlp_train_step(self.handle, _ptr(x), x.stride(0), n, _ptr(labels),
+                                                 C.byref(params), _ptr(loss), _stream(stream)))
+
+    def mlp_infer(self, cube, height, width, bands, logit, center=True, stream=None):
+        dtype = F16 if str(cube.dtype) == "torch.float16" else F32
+        self._check(self._lib.hsi_mlp_infer(self.handle, _ptr(cube), dtype, height, width, bands,
+                                            cube.stride(0), int(bool(center)), _ptr(logit),
+                                            _stream(stream)))
+
+    def gather_pixels(self, cube, height, width, bands, index, out, center=True, stream=None):
+        n = index.numel()
+        self._check(self._lib.hsi_gather_pixels(self.handle, _ptr(cube), height, width, bands,
+                                                cube.stride(0), _ptr(index), n, int(bool(center)),
+                                                _ptr(out), out.stride(0), _stream(stream)))
+
+    def score_candidates(self, detector, cube, height, width, bands, candidates, count, score,
+                         target=None, aux=None, stream=None):
+        cap = candidates.shape[0]
+        self._check(self._lib.hsi_score_candidates(self.handle, int(detector), _ptr(cube), height,
+                                                   width, bands, cube.stride(0), _ptr(candidates),
+                                                   _ptr(count), cap, _ptr(score), _ptr(target),
+                                                   _ptr(aux), _stream(stream)))
+
+
+def mlp_param_count(bands):
+    return int(lib().hsi_mlp_param_count(bands))
 
 
 def detections_to_numpy(out_tensor, count):

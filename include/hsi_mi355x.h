@@ -223,6 +223,78 @@ hsi_status hsi_detect(hsi_ctx* ctx, const float* d_score, const int32_t* d_targe
                       int height, int width, hsi_detection_params params,
                       hsi_detection* d_out, uint32_t* d_count, hsi_stream stream);
 
+/* ---- learned pre-screen --------------------------------------------------------
+ *
+ * No counterpart in the reference. A B -> 64 -> 32 -> 1 ReLU network (FP32
+ * master weights, bf16 matrix-core inference with FP32 accumulation) scores
+ * each pixel's chance of being an RX/ACE detection; pixels above a
+ * conservative cut are rescored exactly (hsi_score_candidates) and the rest
+ * are skipped. Parameters are packed W1[64][B] | b1[64] | W2[32][64] |
+ * b2[32] | w3[32] | b3. One network per context. */
+
+typedef struct hsi_mlp_train_params {
+  float learning_rate; /* Adam step size */
+  float beta1;         /* Adam first-moment decay, e.g. 0.9 */
+  float beta2;         /* Adam second-moment decay, e.g. 0.999 */
+  float epsilon;       /* Adam denominator floor, e.g. 1e-8 */
+  float pos_weight;    /* loss weight of positive samples (recall over precision) */
+} hsi_mlp_train_params;
+
+/* Number of packed parameters for `bands` inputs (0 if out of range). */
+size_t hsi_mlp_param_count(int bands);
+
+/* Synchronous: a fresh network for `bands` inputs, seeded (He-uniform hidden
+ * layers, zero biases), with Adam state cleared. */
+hsi_status hsi_mlp_init(hsi_ctx* ctx, int bands, uint64_t seed);
+
+/* Synchronous: install packed host weights; clears the Adam state. */
+hsi_status hsi_mlp_set_weights(hsi_ctx* ctx, int bands, const float* weights);
+
+/* Synchronous: copy the packed weights to host memory. */
+hsi_status hsi_mlp_get_weights(hsi_ctx* ctx, float* weights);
+
+/* Mean weighted binary cross-entropy gradient of a batch, without updating:
+ * d_x is band-major, x[b * x_stride + s] for s < n (inputs already centred),
+ * d_labels[n] in {0, 1}; d_grad receives hsi_mlp_param_count(B) floats and
+ * d_loss (nullable) the mean loss. Asynchronous. */
+hsi_status hsi_mlp_gradients(hsi_ctx* ctx, const float* d_x, size_t x_stride, int n,
+                             const float* d_labels, float pos_weight, float* d_grad,
+                             float* d_loss, hsi_stream stream);
+
+/* One Adam step on a batch (same inputs as hsi_mlp_gradients). The forward
+ * and backward passes, the gradient reductions and the update all run on
+ * the device in a fixed order. Asynchronous. */
+hsi_status hsi_mlp_train_step(hsi_ctx* ctx, const float* d_x, size_t x_stride, int n,
+                              const float* d_labels, const hsi_mlp_train_params* params,
+                              float* d_loss, hsi_stream stream);
+
+/* Logit of every pixel of a device cube (F32 or F16), the input centred on
+ * the statistics' mean when center_on_statistics_mean is non-zero (the
+ * statistics must be built for this band count). Asynchronous. */
+hsi_status hsi_mlp_infer(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype, int height,
+                         int width, int bands, size_t plane_stride,
+                         int center_on_statistics_mean, float* d_logit, hsi_stream stream);
+
+/* Copy pixels d_index[0..n) of an F32 cube into a band-major sample matrix
+ * out[b * out_stride + s], optionally centred on the statistics' mean: the
+ * training-set builder. An index outside the frame yields a zero sample.
+ * Asynchronous. */
+hsi_status hsi_gather_pixels(hsi_ctx* ctx, const float* d_cube, int height, int width,
+                             int bands, size_t plane_stride, const int32_t* d_index, int n,
+                             int center_on_statistics_mean, float* d_out, size_t out_stride,
+                             hsi_stream stream);
+
+/* Exact (PLAIN) RX or ACE for the pixels of a candidate list only: the
+ * first min(*d_count, capacity) records of d_candidates (x, y as written by
+ * hsi_detect over a logit map). Every other pixel of the score map gets 0
+ * (ACE: target -1, aux 0). Candidates' scores equal hsi_score's for the same
+ * pixels; records outside the frame are skipped. F32 cube. Asynchronous. */
+hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_cube,
+                                int height, int width, int bands, size_t plane_stride,
+                                const hsi_detection* d_candidates, const uint32_t* d_count,
+                                int capacity, float* d_score, int32_t* d_target,
+                                float* d_aux, hsi_stream stream);
+
 /* ---- ingest helpers ---------------------------------------------------------- */
 
 /* FP32 -> FP16 narrowing of `count` elements (round to nearest even).

@@ -1,0 +1,191 @@
+"""The learned pre-screen on the GPU against its CPU oracle
+(oracle/mlp_ref.py, itself pinned by finite differences): gradients of the
+device forward/backward kernels, one Adam step, bf16 inference, and the exact
+rescoring of candidate pixels against the full-frame plain path."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+BANDS = 113
+
+
+@pytest.fixture(scope="module")
+def H():
+    import torch
+    import hsi_mi355x
+    assert torch.cuda.is_available()
+    return hsi_mi355x
+
+
+def batch(n=3000, bands=BANDS, seed=5, pos=0.1):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 0.3, (bands, n)).astype(np.float32)
+    y = (rng.random(n) < pos).astype(np.float32)
+    return x, y
+
+
+def test_gradients_match_oracle(H):
+    import torch
+    ctx = H.Context(0)
+    ctx.mlp_init(BANDS, seed=9)
+    w = ctx.mlp_get_weights().astype(np.float64)
+    x, y = batch()
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    g = torch.zeros(H.mlp_param_count(BANDS), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    print("mlp gradients, 3000 samples x 113 bands", flush=True)
+    ctx.mlp_gradients(xd, yd, 4.0, g, loss)
+    torch.cuda.synchronize()
+    want_loss, want = mlp_ref.loss_and_grad(w, x.astype(np.float64), y, 4.0, BANDS)
+    got = g.cpu().numpy().astype(np.float64)
+    scale = np.abs(want).max()
+    err = np.abs(got - want).max() / scale
+    print(f"gradient max error {err:.3g} of max |g| {scale:.3g}; loss {loss.item():.6f} vs {want_loss:.6f}")
+    assert err <= 1e-4
+    assert abs(loss.item() - want_loss) <= 1e-5 * max(1.0, want_loss)
+    # every parameter block is exercised
+    offs = np.cumsum([0, 64 * BANDS, 64, 32 * 64, 32, 32, 1])
+    for a, b in zip(offs[:-1], offs[1:]):
+        assert np.abs(want[a:b]).max() > 0
+        assert np.abs(got[a:b] - want[a:b]).max() <= 1e-4 * scale
+    ctx.close()
+
+
+def test_adam_step_matches_oracle(H):
+    import torch
+    ctx = H.Context(0)
+    ctx.mlp_init(BANDS, seed=4)
+    w0 = ctx.mlp_get_weights().astype(np.float64)
+    x, y = batch(n=2048, seed=6)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    p = H.MlpTrainParams(learning_rate=1e-3, pos_weight=2.0)
+    ctx.mlp_train_step(xd, yd, p)
+    ctx.mlp_train_step(xd, yd, p)
+    torch.cuda.synchronize()
+    got = ctx.mlp_get_weights().astype(np.float64)
+    w, m, v = w0, np.zeros_like(w0), np.zeros_like(w0)
+    for t in (1, 2):
+        _, g = mlp_ref.loss_and_grad(w, x.astype(np.float64), y, 2.0, BANDS)
+        w, m, v = mlp_ref.adam_step(w, m, v, g, t, 1e-3, 0.9, 0.999, 1e-8)
+    # Adam normalises each step to ~lr, so a gradient within rounding of zero
+    # may move either way; elsewhere the update must agree closely.
+    _, g1 = mlp_ref.loss_and_grad(w0, x.astype(np.float64), y, 2.0, BANDS)
+    solid = np.abs(g1) > 1e-5 * np.abs(g1).max()
+    err = np.abs(got - w)
+    print(f"adam: max error {err[solid].max():.3g} on {solid.sum()} solid entries, "
+          f"{err.max():.3g} overall")
+    assert err[solid].max() <= 2e-6
+    assert err.max() <= 4.1e-3
+    ctx.close()
+
+
+def test_training_reduces_loss(H):
+    import torch
+    ctx = H.Context(0)
+    ctx.mlp_init(BANDS, seed=1)
+    rng = np.random.default_rng(2)
+    n = 4096
+    x = rng.normal(0, 0.3, (BANDS, n)).astype(np.float32)
+    direction = rng.normal(0, 1, BANDS).astype(np.float32)
+    y = ((direction @ x) > 1.0).astype(np.float32)  # separable by a hyperplane
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    p = H.MlpTrainParams(learning_rate=3e-3, pos_weight=1.0)
+    losses = []
+    for step in range(300):
+        ctx.mlp_train_step(xd, yd, p, loss)
+        if step % 50 == 0 or step == 299:
+            torch.cuda.synchronize()
+            losses.append(loss.item())
+    print("training loss", [round(v, 4) for v in losses])
+    assert losses[-1] < 0.25 * losses[0]
+    ctx.close()
+
+
+def test_inference_matches_bf16_oracle(H):
+    import torch
+    ctx = H.Context(0)
+    ctx.mlp_init(BANDS, seed=12)
+    w = ctx.mlp_get_weights().astype(np.float64)
+    h, wd = 13, 29
+    rng = np.random.default_rng(8)
+    cube = rng.uniform(0.0, 0.9, (BANDS, h * wd)).astype(np.float32)
+    dev = H.to_device_cube(cube, h, wd)
+    logit = torch.full((h * wd,), float("nan"), dtype=torch.float32, device="cuda")
+    print("mlp inference 13x29x113", flush=True)
+    ctx.mlp_infer(dev, h, wd, BANDS, logit, center=False)
+    torch.cuda.synchronize()
+    got = logit.cpu().numpy().astype(np.float64)
+    want = mlp_ref.forward_bf16(w, cube.astype(np.float64), BANDS)
+    exact = mlp_ref.forward(w, cube.astype(np.float64), BANDS)[0]
+    err = np.abs(got - want)
+    print(f"inference: max |logit - bf16 oracle| {err.max():.3g}, median {np.median(err):.3g}; "
+          f"vs float64 network {np.abs(got - exact).max():.3g}")
+    assert np.isfinite(got).all()
+    assert np.median(err) <= 1e-4 * (1 + np.abs(want).max())
+    assert err.max() <= 1e-2 * (1 + np.abs(want).max())
+    # the fp16 cube path reads the same values when they are halves already
+    halves = cube.astype(np.float16)
+    dev16 = H.to_device_cube(halves.astype(np.float32), h, wd, dtype="f16")
+    ctx.mlp_infer(dev16, h, wd, BANDS, logit, center=False)
+    torch.cuda.synchronize()
+    dev32 = H.to_device_cube(halves.astype(np.float32), h, wd)
+    l2 = torch.empty_like(logit)
+    ctx.mlp_infer(dev32, h, wd, BANDS, l2, center=False)
+    torch.cuda.synchronize()
+    assert torch.equal(logit, l2)
+    ctx.close()
+
+
+@pytest.mark.parametrize("det", ["rx", "ace"])
+def test_candidate_rescoring_equals_full_frame(H, det):
+    import torch
+    from conftest import golden
+    g = golden("scene_32x40x113_t4")
+    b, h, w = g["cube"].shape
+    p = h * w
+    cube = H.to_device_cube(g["cube"], h, w)
+    ctx = H.Context(0)
+    ctx.set_library(g["library"])
+    ctx.stats_accumulate(cube, h, w, b)
+    ctx.stats_build(1e-6)
+    D = H.Detector.RX if det == "rx" else H.Detector.ACE
+    f32 = dict(dtype=torch.float32, device="cuda")
+    full_s, full_a = torch.empty(p, **f32), torch.empty(p, **f32)
+    full_t = torch.empty(p, dtype=torch.int32, device="cuda")
+    ctx.score(D, cube, h, w, b, full_s, full_t if det == "ace" else None, full_a if det == "ace" else None)
+    # candidates: every third pixel, as raster-order detection records
+    idx = np.arange(0, p, 3)
+    rec = np.zeros((idx.size + 5, 4), np.int32)
+    rec[:idx.size, 0] = idx % w
+    rec[:idx.size, 1] = idx // w
+    cand = torch.from_numpy(rec).cuda()
+    count = torch.tensor([idx.size], dtype=torch.int32, device="cuda")
+    s = torch.full((p,), float("nan"), **f32)
+    a = torch.full((p,), float("nan"), **f32)
+    t = torch.full((p,), -7, dtype=torch.int32, device="cuda")
+    print(f"candidate rescoring {det}: {idx.size} of {p}", flush=True)
+    ctx.score_candidates(D, cube, h, w, b, cand, count, s, t if det == "ace" else None,
+                         a if det == "ace" else None)
+    torch.cuda.synchronize()
+    got, want = s.cpu().numpy(), full_s.cpu().numpy()
+    mask = np.zeros(p, bool)
+    mask[idx] = True
+    assert np.array_equal(got[mask].view(np.uint32), want[mask].view(np.uint32))
+    assert np.all(got[~mask] == 0)
+    if det == "ace":
+        assert np.array_equal(t.cpu().numpy()[mask], full_t.cpu().numpy()[mask])
+        assert np.all(t.cpu().numpy()[~mask] == -1)
+        assert np.array_equal(a.cpu().numpy()[mask], full_a.cpu().numpy()[mask])
+    # a count beyond the capacity is clamped to it
+    count.fill_(10 ** 6)
+    ctx.score_candidates(D, cube, h, w, b, cand[:64], count, s)
+    torch.cuda.synchronize()
+    got = s.cpu().numpy()
+    first = idx[:64]
+    assert np.array_equal(got[first].view(np.uint32), want[first].view(np.uint32))
+    assert np.count_nonzero(got) <= 64
+    ctx.close()
