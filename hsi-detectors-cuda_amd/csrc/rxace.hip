@@ -65,9 +65,13 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
   const int g = lane >> 4;   // lane group: K index of the A/B operands
   const int n = lane & 15;   // pixel column within the tile
   std::size_t items = pixels;
+  bool gather = GATHER;
   if constexpr (GATHER) {
+    // A list that overflowed its capacity means the screen flagged more than
+    // it can hold: fall back to scoring the whole frame.
     const unsigned c = *cand_count;
-    items = c < cand_cap ? c : cand_cap;
+    if (c > cand_cap) gather = false;
+    else items = c;
   }
   const std::size_t tiles = div_up(items, 16);
   // Pixel of tile element t*16 + n, and whether it exists.
@@ -75,6 +79,7 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
     const std::size_t i = t * 16 + n;
     ok = i < items;
     if constexpr (GATHER) {
+      if (!gather) return i;
       if (!ok) return 0;
       const hsi_detection d = cand[i];
       // a record outside the frame names no pixel

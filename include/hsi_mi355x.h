@@ -285,9 +285,10 @@ hsi_status hsi_gather_pixels(hsi_ctx* ctx, const float* d_cube, int height, int 
                              hsi_stream stream);
 
 /* Exact (PLAIN) RX or ACE for the pixels of a candidate list only: the
- * first min(*d_count, capacity) records of d_candidates (x, y as written by
- * hsi_detect over a logit map). Every other pixel of the score map gets 0
- * (ACE: target -1, aux 0). Candidates' scores equal hsi_score's for the same
+ * first *d_count records of d_candidates (x, y as written by hsi_detect over
+ * a logit map). Every other pixel of the score map gets 0 (ACE: target -1,
+ * aux 0). If *d_count exceeds `capacity` the list overflowed and the whole
+ * frame is scored instead (decided on the device, no host round trip). Candidates' scores equal hsi_score's for the same
  * pixels; records outside the frame are skipped. F32 cube. Asynchronous. */
 hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_cube,
                                 int height, int width, int bands, size_t plane_stride,

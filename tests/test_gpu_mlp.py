@@ -180,12 +180,9 @@ def test_candidate_rescoring_equals_full_frame(H, det):
         assert np.array_equal(t.cpu().numpy()[mask], full_t.cpu().numpy()[mask])
         assert np.all(t.cpu().numpy()[~mask] == -1)
         assert np.array_equal(a.cpu().numpy()[mask], full_a.cpu().numpy()[mask])
-    # a count beyond the capacity is clamped to it
+    # a list that overflowed its capacity scores the whole frame
     count.fill_(10 ** 6)
     ctx.score_candidates(D, cube, h, w, b, cand[:64], count, s)
     torch.cuda.synchronize()
-    got = s.cpu().numpy()
-    first = idx[:64]
-    assert np.array_equal(got[first].view(np.uint32), want[first].view(np.uint32))
-    assert np.count_nonzero(got) <= 64
+    assert np.array_equal(s.cpu().numpy().view(np.uint32), want.view(np.uint32))
     ctx.close()
