@@ -80,12 +80,16 @@ class Prescreen:
         return (rx >= self.label_rx) | (ace >= self.label_ace)
 
     # -- training ----------------------------------------------------------------
-    def add_training_frame(self, cube, h, w, negatives_per_positive=16):
-        """Statistics, plain labels, and a gathered sample set from one frame."""
+    def add_training_frame(self, cube, h, w, negatives_per_positive=16, max_positives=4096):
+        """Statistics, plain labels, and a gathered sample set from one frame
+        (at most `max_positives` positives, sampled, and
+        `negatives_per_positive` times as many negatives)."""
         t = self.torch
         self.statistics(cube, h, w)
         pos = self.labels(cube, h, w)
         pidx = np.flatnonzero(pos)
+        if pidx.size > max_positives:
+            pidx = np.sort(self.rng.choice(pidx, max_positives, replace=False))
         nidx = np.flatnonzero(~pos)
         k = min(nidx.size, max(negatives_per_positive * max(pidx.size, 1), 4096))
         nsel = self.rng.choice(nidx, k, replace=False)
@@ -98,7 +102,9 @@ class Prescreen:
         self._y.append(lab)
         return int(pidx.size), int(nsel.size)
 
-    def train(self, epochs=60, batch=1024, learning_rate=2e-3, pos_weight=8.0):
+    def train(self, epochs=60, batch=1024, learning_rate=2e-3, pos_weight=8.0, max_steps=8000):
+        """Adam over shuffled batches; epochs are cut so the run stays within
+        `max_steps` updates."""
         t = self.torch
         n = sum(x.shape[1] for x in self._x)
         stride = (n + 7) // 8 * 8
@@ -113,6 +119,7 @@ class Prescreen:
         params = MlpTrainParams(learning_rate=learning_rate, pos_weight=pos_weight)
         loss = t.zeros(1, dtype=t.float32, device="cuda")
         history = []
+        epochs = min(epochs, max(1, max_steps // max(1, n // batch))) if epochs else 0
         for epoch in range(epochs):
             perm = self.rng.permutation(n).astype(np.int32)
             for s0 in range(0, n - batch + 1, batch):
@@ -124,15 +131,16 @@ class Prescreen:
             history.append(float(loss.item()))
         return history
 
-    def calibrate(self, margin=1.0):
-        """Cut = the lowest training-positive logit, less a margin."""
+    def calibrate(self, margin=1.0, quantile=1e-4):
+        """Cut = the `quantile` of the training positives' logits (the lowest
+        one for small sets), less a margin."""
         t = self.torch
         X, Y, n = self._train_x, self._train_y, self._n
         logit = t.empty(X.shape[1], dtype=t.float32, device="cuda")
         self.ctx.mlp_infer(X, 1, X.shape[1], self.bands, logit, center=False)
         t.cuda.synchronize()
         lg = logit.cpu().numpy()[:n]
-        self.cut = float(lg[Y > 0.5].min() - margin)
+        self.cut = float(np.quantile(lg[Y > 0.5], quantile, method="lower") - margin)
         return self.cut
 
     # -- run time --------------------------------------------------------------------

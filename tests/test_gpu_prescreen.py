@@ -28,7 +28,7 @@ def run_prescreen(H, train_epochs):
     for i, cube in enumerate(train):
         npos, nneg = ps.add_training_frame(cube, H_, W_)
         print(f"training frame {i}: {npos} positives, {nneg} negatives", flush=True)
-    hist = ps.train(epochs=train_epochs) if train_epochs else []
+    hist = ps.train(epochs=train_epochs)
     cut = ps.calibrate()
     results = [ps.evaluate(c, H_, W_) for c in held]
     ctx.close()
