@@ -85,6 +85,23 @@ __device__ __forceinline__ float to_float(_Float16 v) { return static_cast<float
 
 constexpr int kInferWaves = 4;
 
+// Four consecutive pixels of one band through the cube's buffer view: one
+// 16-byte (FP32) or 8-byte (FP16) load; a band past the last reads zeros.
+__device__ __forceinline__ f4 load4(__amdgpu_buffer_rsrc_t r, std::uint32_t off, float) {
+  return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ f4 load4(__amdgpu_buffer_rsrc_t r, std::uint32_t off, _Float16) {
+  typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+  const h4 v = __builtin_bit_cast(h4, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  return f4{static_cast<float>(v[0]), static_cast<float>(v[1]), static_cast<float>(v[2]), static_cast<float>(v[3])};
+}
+
+// A wave scores 64-pixel tiles: lane (g, n) loads pixels 4n..4n+3 of bands
+// 32J + 8g + i (i < 8) as one wide load per band, which already is the MFMA
+// B operand of four 16-pixel sub-tiles (sub-tile s = pixels 4n + s), so each
+// weight fragment read from LDS serves four MFMAs and each load moves 1 KiB
+// (FP32) per wave. The next K block's loads are in flight while the matrix
+// cores work on this one.
 template <typename In, int NKB>
 __global__ __launch_bounds__(kInferWaves * 64) void mlp_infer_kernel(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
@@ -103,86 +120,101 @@ __global__ __launch_bounds__(kInferWaves * 64) void mlp_infer_kernel(
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int n = lane & 15;
-  const std::size_t tiles = div_up(pixels, 16);
+  const std::size_t tiles = div_up(pixels, 64);
   const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(kInferWaves) + (threadIdx.x >> 6);
   const std::size_t waves = static_cast<std::size_t>(gridDim.x) * kInferWaves;
-  // Band 32J + 8g + i of pixel t*16 + n through a buffer view of the cube:
-  // bands past the last read 0, pixels past the last only reach their own
-  // (unwritten) logit.
   const __amdgpu_buffer_rsrc_t rsrc = cube_rsrc(cube, static_cast<std::uint32_t>(bands * stride * sizeof(In)));
-  const std::uint32_t lane_off = static_cast<std::uint32_t>((static_cast<std::size_t>(8 * g) * stride + n) * sizeof(In));
-  auto load_tile = [&](std::size_t t, float (&xr)[NKB][8]) {
-    const std::uint32_t tb = lane_off + static_cast<std::uint32_t>(t * 16 * sizeof(In));
+  const std::uint32_t lane_off = static_cast<std::uint32_t>((static_cast<std::size_t>(8 * g) * stride + 4 * n) * sizeof(In));
+
+  auto load_kblock = [&](std::size_t t, int J, f4 (&xr)[8]) {
+    const std::uint32_t base = lane_off + static_cast<std::uint32_t>((t * 64 + static_cast<std::size_t>(32 * J) * stride) * sizeof(In));
 #pragma unroll
-    for (int J = 0; J < NKB; ++J)
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        xr[J][i] = buffer_elem(rsrc, tb + static_cast<std::uint32_t>(static_cast<std::size_t>(32 * J + i) * stride * sizeof(In)), In{});
+    for (int i = 0; i < 8; ++i)
+      xr[i] = load4(rsrc, base + static_cast<std::uint32_t>(static_cast<std::size_t>(i) * stride * sizeof(In)), In{});
   };
-  float cur[NKB][8], nxt[NKB][8];
-  if (wave_id < tiles) load_tile(wave_id, cur);
+  f4 cur[8], nxt[8];
+  if (wave_id < tiles) load_kblock(wave_id, 0, cur);
 
   for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
     // Re-read the weight fragments from LDS every tile rather than hoisting
-    // all of them into registers, which would leave one wave per SIMD.
+    // all of them into registers.
     __asm__ volatile("" ::: "memory");
-    if (tile + waves < tiles) load_tile(tile + waves, nxt);
-    f4 z1[4];
+    f4 z1[4][4];
 #pragma unroll
-    for (int I = 0; I < 4; ++I) z1[I] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int sub = 0; sub < 4; ++sub)
+#pragma unroll
+      for (int I = 0; I < 4; ++I) z1[sub][I] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int J = 0; J < NKB; ++J) {
-      bf8 xb;
+      if (J + 1 < NKB) load_kblock(tile, J + 1, nxt);
+      else if (tile + waves < tiles) load_kblock(tile + waves, 0, nxt);
+      bf8 xb[4];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const int b = 32 * J + 8 * g + i;
-        // padded bands: value 0 and weight 0
-        xb[i] = static_cast<__bf16>(b < bands ? cur[J][i] - c_lds[b] : 0.f);
+        const float c = c_lds[32 * J + 8 * g + i];  // padded bands: value 0, weight 0
+#pragma unroll
+        for (int sub = 0; sub < 4; ++sub) xb[sub][i] = static_cast<__bf16>(cur[i][sub] - c);
       }
 #pragma unroll
-      for (int I = 0; I < 4; ++I)
-        z1[I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1_lds[(J * 4 + I) * 64 + lane], xb, z1[I], 0, 0, 0);
+      for (int I = 0; I < 4; ++I) {
+        const bf8 a = a1_lds[(J * 4 + I) * 64 + lane];
+#pragma unroll
+        for (int sub = 0; sub < 4; ++sub)
+          z1[sub][I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[sub], z1[sub][I], 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) cur[i] = nxt[i];
     }
-#pragma unroll
-    for (int J = 0; J < NKB; ++J)
-#pragma unroll
-      for (int i = 0; i < 8; ++i) cur[J][i] = nxt[J][i];
 
-    f4 z2[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+    f4 res = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
-      bf8 hb;
+    for (int sub = 0; sub < 4; ++sub) {
+      f4 z2[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int I = 2 * kb + (i < 4 ? 0 : 1);
-        const int r = i & 3;
-        const float h = z1[I][r] + t_lds[16 * I + 4 * g + r];
-        hb[i] = static_cast<__bf16>(h > 0.f ? h : 0.f);
+      for (int kb = 0; kb < 2; ++kb) {
+        bf8 hb;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int I = 2 * kb + (i < 4 ? 0 : 1);
+          const int r = i & 3;
+          const float h = z1[sub][I][r] + t_lds[16 * I + 4 * g + r];
+          hb[i] = static_cast<__bf16>(h > 0.f ? h : 0.f);
+        }
+#pragma unroll
+        for (int I2 = 0; I2 < 2; ++I2)
+          z2[I2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2_lds[(kb * 2 + I2) * 64 + lane], hb, z2[I2], 0, 0, 0);
       }
+      float out = 0.f;
 #pragma unroll
       for (int I2 = 0; I2 < 2; ++I2)
-        z2[I2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2_lds[(kb * 2 + I2) * 64 + lane], hb, z2[I2], 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = 16 * I2 + 4 * g + r;
+          const float h = z2[I2][r] + t_lds[kH1 + k];
+          out += t_lds[kH1 + kH2 + k] * (h > 0.f ? h : 0.f);
+        }
+      out += __shfl_xor(out, 16, 64);
+      out += __shfl_xor(out, 32, 64);
+      res[sub] = out + t_lds[kH1 + 2 * kH2];
     }
-    float out = 0.f;
+    // lane (0, n) holds the logits of pixels 4n .. 4n + 3
+    const std::size_t p = tile * 64 + 4 * static_cast<std::size_t>(n);
+    if (g == 0) {
+      if (p + 4 <= pixels) {
+        *reinterpret_cast<f4*>(logit + p) = res;
+      } else {
 #pragma unroll
-    for (int I2 = 0; I2 < 2; ++I2)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = 16 * I2 + 4 * g + r;
-        const float h = z2[I2][r] + t_lds[kH1 + k];
-        out += t_lds[kH1 + kH2 + k] * (h > 0.f ? h : 0.f);
+        for (int sub = 0; sub < 4; ++sub)
+          if (p + sub < pixels) logit[p + sub] = res[sub];
       }
-    out += __shfl_xor(out, 16, 64);
-    out += __shfl_xor(out, 32, 64);
-    const std::size_t p = tile * 16 + n;
-    if (g == 0 && p < pixels) logit[p] = out + t_lds[kH1 + 2 * kH2];
+    }
   }
 }
 
 template <typename In>
 hipError_t launch_infer_typed(const MlpInferArgs& a, hipStream_t stream) {
   if (!fits_buffer(a.bands, a.stride, sizeof(In))) return hipErrorInvalidValue;
-  const std::size_t tiles = div_up(a.pixels, 16);
+  const std::size_t tiles = div_up(a.pixels, 64);
   std::size_t wgs = div_up(tiles, kInferWaves);
   if (wgs > 2048) wgs = 2048;
   const dim3 grid(static_cast<unsigned>(wgs)), block(kInferWaves * 64);
