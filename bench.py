@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--repeats", type=int, default=20, help="per-kernel timing repeats")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=1024, help="rows of a frame in the CPU sample")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-scene batched leg")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned pre-screen leg (2048x2048 frames)")
     ap.add_argument("--no-precision", action="store_true",
@@ -338,6 +340,87 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
     }
 
 
+def batched_form(H, torch, repeats, seed=20261203 + 4000):
+    """64 uneven scenes in one batch: heights U{128..2048}, widths U{128..1024},
+    bands from {16, 113, 224}, libraries of U{1..32} targets; 16 of them are
+    4x4 snapshot-mosaic frames of 2048x1088 (u16) demosaiced on the device to
+    512x272x16. All four detectors run. Timed: the batch's persistent
+    launches (one per shape class, shapes from the autotuner) against the
+    same scoring issued scene by scene through the single-scene entry points."""
+    rng = np.random.default_rng(seed)
+    f32 = dict(dtype=torch.float32, device="cuda")
+    i32 = dict(dtype=torch.int32, device="cuda")
+    scenes = []
+    total_px = 0
+    ctx0 = H.Context(0)
+    demosaic_us = 0.0
+    for i in range(64):
+        t = int(rng.integers(1, 33))
+        if i % 4 == 3:   # a snapshot-mosaic frame
+            h, w, b = 272, 512, 16
+        else:
+            h, w = int(rng.integers(128, 2049)), int(rng.integers(128, 1025))
+            b = int(rng.choice([16, 113, 224]))
+        cfg = H.synth_config(h, w, b, min(t, 8), seed=seed + i, anomaly_fraction=0.005)
+        lib = H.synth_library(H.synth_config(h, w, b, t, seed=seed + i))
+        stride = H.plane_stride(h, w)
+        if i % 4 == 3:
+            host = H.synth_frame(cfg, 0, threads=16)[:, :h * w].reshape(b, h, w)
+            raw = np.zeros((h * 4, w * 4), np.uint16)
+            for band in range(b):
+                oy, ox = divmod(band, 4)
+                raw[oy::4, ox::4] = np.clip(np.round(host[band] * 65535.0), 0, 65535).astype(np.uint16)
+            raw_d = torch.from_numpy(raw.view(np.int16)).cuda()
+            cube = torch.zeros((b, stride), **f32)
+            demosaic_us += time_on_stream(torch, lambda: H.demosaic(ctx0, raw_d, H.RAW_U16, h * 4, w * 4, 4, b, cube), 3) * 1e3
+            del raw_d
+        else:
+            cube = torch.from_numpy(H.synth_frame(cfg, 0, threads=16)).cuda()
+        p = h * w
+        total_px += p
+        scenes.append(dict(cube=cube, height=h, width=w, library=lib,
+                           sam_score=torch.empty(p, **f32), sam_target=torch.empty(p, **i32),
+                           cem_score=torch.empty(p, **f32), cem_target=torch.empty(p, **i32),
+                           rx=torch.empty(p, **f32), ace_score=torch.empty(p, **f32),
+                           ace_target=torch.empty(p, **i32)))
+    batch = H.Batch(0)
+    batch.set_scenes(scenes)
+    batch.statistics(LOADING)
+    batch.check()
+    report = batch.autotune()
+    log("  batch autotuner:\n    " + report.strip().replace("\n", "\n    "))
+    ms_batch = time_on_stream(torch, batch.score, repeats)
+    ctxs = [batch.context(i) for i in range(len(scenes))]
+
+    def per_scene():
+        for c, sc in zip(ctxs, scenes):
+            b = sc["cube"].shape[0]
+            c.score_sam_cem(sc["cube"], sc["height"], sc["width"], b, sc["sam_score"], sc["sam_target"],
+                            sc["cem_score"], sc["cem_target"])
+            c.score(H.Detector.ACE, sc["cube"], sc["height"], sc["width"], b, sc["ace_score"],
+                    sc["ace_target"], sc["rx"])
+
+    ms_single = time_on_stream(torch, per_scene, repeats)
+    ms_stats = time_on_stream(torch, lambda: batch.statistics(LOADING), repeats)
+    classes = report.strip().count("\n") + 1
+    ctx0.close()
+    batch.close()
+    log(f"  batched: {ms_batch * 1e3:.1f} us for {total_px / 1e6:.1f} Mpix, scene by scene {ms_single * 1e3:.1f} us")
+    return {
+        "workload": "64 uneven scenes (H 128-2048, W 128-1024, bands 16/113/224, 1-32 targets; 16 demosaiced "
+                    "2048x1088 4x4 mosaics), SAM+CEM+RX+ACE",
+        "pixels": total_px,
+        "launches": classes,
+        "batched_us": round(ms_batch * 1e3, 1),
+        "scene_by_scene_us": round(ms_single * 1e3, 1),
+        "batched_mpix_s": round(total_px / (ms_batch / 1e3) / 1e6, 1),
+        "speedup_vs_scene_by_scene": round(ms_single / ms_batch, 3),
+        "statistics_us": round(ms_stats * 1e3, 1),
+        "demosaic_us_total": round(demosaic_us, 1),
+        "autotuner": report.strip().split("\n"),
+    }
+
+
 def main():
     args = parse()
     if args.copy_calibrate:
@@ -487,6 +570,11 @@ def main():
         log("precision form (224 bands)")
         prec = precision_form(H, torch, max(3, r // 4))
 
+    batched = None
+    if not args.no_batch:
+        log("batched uneven scenes (64)")
+        batched = batched_form(H, torch, max(3, r // 4))
+
     learned = None
     if not args.no_learned:
         log("learned pre-screen (2048x2048x113)")
@@ -519,6 +607,7 @@ def main():
         "detectors": detectors,
         "precision_form": prec,
         "learned_form": learned,
+        "batched_form": batched,
         "stage_us": {k: round(v * 1e3, 1) for k, v in ms.items()},
     }
     print(json.dumps(line), flush=True)

@@ -65,6 +65,10 @@ struct hsi_ctx {
   float* d_mlp_act = nullptr;
   std::size_t mlp_act_bytes = 0;
 
+  // Demosaic band positions (uploaded when they change).
+  int* d_positions = nullptr;
+  std::vector<int> positions;
+
   // Scratch sized per frame.
   double* d_mom_scratch = nullptr;
   std::size_t mom_scratch_bytes = 0;
@@ -166,7 +170,8 @@ void hsi_ctx_destroy(hsi_ctx* ctx) {
                   ctx->d_cem_lib, ctx->d_white_blocks, ctx->d_ace_u, ctx->d_white_hl,
                   ctx->d_ace_hl, ctx->d_col_scale, ctx->d_row_inv, ctx->d_target_inv,
                   ctx->d_mom_scratch, ctx->d_det_scratch, ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v,
-                  ctx->d_mlp_grad, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, ctx->d_mlp_act};
+                  ctx->d_mlp_grad, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, ctx->d_mlp_act,
+                  ctx->d_positions};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete ctx;
@@ -633,6 +638,37 @@ hsi_status hsi_transpose_bip_to_bsq(const float* d_src, float* d_dst, int height
   return HSI_OK;
 }
 
+hsi_status hsi_demosaic(hsi_ctx* ctx, const void* d_raw, hsi_raw_type raw_type, int raw_height, int raw_width,
+                        int period, int bands, const int32_t* band_order, float* d_cube, size_t plane_stride,
+                        hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (!d_raw || !d_cube || period <= 0 || raw_height < period || raw_width < period)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "demosaic: image is smaller than one mosaic tile");
+  if (bands <= 0 || bands > period * period || bands > kMaxBands)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "demosaic: bands must be in 1.." + std::to_string(period * period));
+  if (raw_type != HSI_RAW_U8 && raw_type != HSI_RAW_U16 && raw_type != HSI_RAW_F32)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "demosaic: unknown raw sample type");
+  const std::size_t pixels = static_cast<std::size_t>(raw_height / period) * (raw_width / period);
+  if (plane_stride < pixels) return fail(ctx, HSI_ERR_INVALID_ARG, "demosaic: plane stride below the pixel count");
+  std::vector<int> pos(static_cast<std::size_t>(bands));
+  for (int b = 0; b < bands; ++b) {
+    pos[static_cast<std::size_t>(b)] = band_order ? band_order[b] : b;
+    if (pos[static_cast<std::size_t>(b)] < 0 || pos[static_cast<std::size_t>(b)] >= period * period)
+      return fail(ctx, HSI_ERR_INVALID_ARG, "demosaic: band_order entries must be in 0..period^2-1");
+  }
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  if (!ctx->d_positions) HSI_HIP(ctx, alloc(&ctx->d_positions, kMaxBands), "allocate band positions");
+  if (pos != ctx->positions) {
+    HSI_HIP(ctx, hipMemcpy(ctx->d_positions, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice),
+            "upload band positions");
+    ctx->positions = pos;
+  }
+  HSI_HIP(ctx, launch_demosaic(d_raw, static_cast<int>(raw_type), raw_height, raw_width, period, bands,
+                               ctx->d_positions, d_cube, plane_stride, static_cast<hipStream_t>(stream)),
+          "demosaic");
+  return HSI_OK;
+}
+
 }  // extern "C"
 
 // ---- learned pre-screen ------------------------------------------------------
@@ -876,6 +912,359 @@ hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float
   }
   HSI_HIP(ctx, launch_rxace_candidates(a, d_candidates, d_count, static_cast<unsigned>(capacity), width, s),
           "rescore candidates");
+  return HSI_OK;
+}
+
+}  // extern "C"
+
+// ---- batched scenes -------------------------------------------------------------
+//
+// Scenes of different sizes, band counts and library sizes scored together:
+// one hsi_ctx per scene holds its library and statistics, and the per-pixel
+// scoring runs as one persistent launch per kernel shape class - SAM/CEM
+// grouped by library stride (1..32 targets -> 1, 2, 4, 8, 16, 32), RX/ACE by
+// 16-band row blocks and target tiles - over device tables of the scenes.
+// Launch shapes (pixels per thread, grid size, tiles per work item) come from
+// an autotuner that times the candidates on the batch itself.
+
+struct hsi_batch {
+  int device = 0;
+  std::string error;
+  std::vector<hsi_ctx*> ctx;
+  std::vector<hsi_scene> scenes;
+  struct SamClass {
+    int lib_stride = 1;
+    std::vector<SamSceneDev> table;
+    unsigned blocks[2] = {0, 0};  // total 256*PPT-pixel blocks for PPT = 4, 2
+    SamSceneDev* d_table[2] = {nullptr, nullptr};
+    int* d_block_scene[2] = {nullptr, nullptr};
+    int ppt = 4;
+    int workgroups = 0;           // 0: one workgroup per block
+  };
+  struct RxClass {
+    int nb = 0, tt = 0;
+    std::vector<RxSceneDev> table;
+    RxSceneDev* d_table = nullptr;
+    std::vector<RxWorkItem> items[3];  // tiles per item: 16, 64, 256
+    RxWorkItem* d_items[3] = {nullptr, nullptr, nullptr};
+    int chunk = 1;                     // index into items
+    int workgroups = 0;                // 0: one workgroup per item
+  };
+  std::vector<SamClass> sam;
+  std::vector<RxClass> rx;
+  bool tuned = false;
+};
+
+namespace {
+
+hsi_status bfail(hsi_batch* b, hsi_status st, const std::string& msg) {
+  if (b) b->error = msg;
+  return st;
+}
+
+const int kRxChunks[3] = {16, 64, 256};
+
+void batch_release_tables(hsi_batch* b) {
+  for (auto& c : b->sam) {
+    for (auto*& p : c.d_table)
+      if (p) (void)hipFree(p), p = nullptr;
+    for (auto*& p : c.d_block_scene)
+      if (p) (void)hipFree(p), p = nullptr;
+  }
+  for (auto& c : b->rx) {
+    if (c.d_table) (void)hipFree(c.d_table), c.d_table = nullptr;
+    for (auto*& p : c.d_items)
+      if (p) (void)hipFree(p), p = nullptr;
+  }
+  b->sam.clear();
+  b->rx.clear();
+}
+
+template <typename T>
+hipError_t upload_vec(T** dst, const std::vector<T>& v) {
+  if (v.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+hipError_t run_sam_class(const hsi_batch::SamClass& c, int ppt, int wgs, hipStream_t s) {
+  const int k = ppt == 4 ? 0 : 1;
+  return launch_sam_cem_batch(c.lib_stride, ppt, c.d_table[k], c.d_block_scene[k], c.blocks[k], wgs, s);
+}
+
+hipError_t run_rx_class(const hsi_batch::RxClass& c, int chunk, int wgs, hipStream_t s) {
+  return launch_rxace_batch(c.nb, c.tt, c.d_table, c.d_items[chunk], static_cast<int>(c.items[chunk].size()), wgs, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+hsi_status hsi_batch_create(int device, hsi_batch** out) {
+  if (!out) return HSI_ERR_INVALID_ARG;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return HSI_ERR_DEVICE;
+  *out = new hsi_batch();
+  (*out)->device = device;
+  return HSI_OK;
+}
+
+void hsi_batch_destroy(hsi_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  batch_release_tables(b);
+  for (hsi_ctx* c : b->ctx) hsi_ctx_destroy(c);
+  delete b;
+}
+
+const char* hsi_batch_last_error(const hsi_batch* b) { return b ? b->error.c_str() : ""; }
+
+hsi_ctx* hsi_batch_context(hsi_batch* b, int scene) {
+  if (!b || scene < 0 || scene >= static_cast<int>(b->ctx.size())) return nullptr;
+  return b->ctx[static_cast<std::size_t>(scene)];
+}
+
+hsi_status hsi_batch_set_scenes(hsi_batch* b, const hsi_scene* scenes, int count) {
+  if (!b) return HSI_ERR_INVALID_ARG;
+  if (!scenes || count <= 0) return bfail(b, HSI_ERR_INVALID_ARG, "a batch needs at least one scene");
+  if (hipSetDevice(b->device) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "select device");
+  batch_release_tables(b);
+  while (static_cast<int>(b->ctx.size()) < count) {
+    hsi_ctx* c = nullptr;
+    const hsi_status st = hsi_ctx_create(b->device, &c);
+    if (st != HSI_OK) return bfail(b, st, "cannot create a scene context");
+    b->ctx.push_back(c);
+  }
+  while (static_cast<int>(b->ctx.size()) > count) {
+    hsi_ctx_destroy(b->ctx.back());
+    b->ctx.pop_back();
+  }
+  b->scenes.assign(scenes, scenes + count);
+  b->tuned = false;
+  for (int i = 0; i < count; ++i) {
+    const hsi_scene& sc = scenes[i];
+    hsi_ctx* c = b->ctx[static_cast<std::size_t>(i)];
+    hsi_status st = check_cube(c, sc.d_cube, sc.height, sc.width, sc.bands, sc.plane_stride, HSI_F32);
+    if (st != HSI_OK) return bfail(b, st, "scene " + std::to_string(i) + ": " + c->error);
+    if (sc.bands > kMaxRxBands && (sc.d_rx_score || sc.d_ace_score))
+      return bfail(b, HSI_ERR_CAPACITY, "scene " + std::to_string(i) + ": RX/ACE support at most " +
+                                            std::to_string(kMaxRxBands) + " bands");
+    const bool wants_lib = sc.d_sam_score || sc.d_cem_score || sc.d_ace_score;
+    if (wants_lib && (!sc.library || sc.num_targets <= 0))
+      return bfail(b, HSI_ERR_NO_LIBRARY, "scene " + std::to_string(i) + ": SAM/CEM/ACE need a library");
+    for (const void* m : {static_cast<const void*>(sc.d_sam_score), static_cast<const void*>(sc.d_sam_target),
+                          static_cast<const void*>(sc.d_cem_score), static_cast<const void*>(sc.d_cem_target)})
+      if (reinterpret_cast<std::uintptr_t>(m) % 16 != 0)
+        return bfail(b, HSI_ERR_INVALID_ARG, "scene " + std::to_string(i) + ": SAM/CEM maps must be 16-byte aligned");
+    if (sc.library && sc.num_targets > 0) {
+      st = hsi_set_library(c, sc.library, nullptr, sc.num_targets, sc.bands);
+      if (st != HSI_OK) return bfail(b, st, "scene " + std::to_string(i) + ": " + c->error);
+    }
+    st = hsi_ctx_reserve(c, static_cast<std::size_t>(sc.height) * sc.width, sc.bands);
+    if (st != HSI_OK) return bfail(b, st, "scene " + std::to_string(i) + ": " + c->error);
+  }
+  // SAM/CEM classes by library stride.
+  for (int i = 0; i < count; ++i) {
+    const hsi_scene& sc = scenes[i];
+    if (!sc.d_sam_score && !sc.d_cem_score) continue;
+    hsi_ctx* c = b->ctx[static_cast<std::size_t>(i)];
+    hsi_batch::SamClass* cls = nullptr;
+    for (auto& k : b->sam)
+      if (k.lib_stride == c->lib_stride) cls = &k;
+    if (!cls) {
+      b->sam.emplace_back();
+      cls = &b->sam.back();
+      cls->lib_stride = c->lib_stride;
+      cls->ppt = c->lib_stride <= 8 ? 4 : 2;
+    }
+    SamSceneDev d{};
+    d.cube = sc.d_cube;
+    d.pixels = static_cast<std::size_t>(sc.height) * sc.width;
+    d.bands = sc.bands;
+    d.stride = sc.plane_stride;
+    d.num_targets = c->num_targets;
+    d.exact_all = c->lib_degenerate;
+    d.sam_lib = c->d_sam_lib;
+    d.sam_norms = c->d_norms;
+    d.sam_rnorms = c->d_sam_rnorm;
+    d.cem_lib = c->d_cem_lib;
+    d.sam_score = sc.d_sam_score;
+    d.sam_target = sc.d_sam_target;
+    d.cem_score = sc.d_cem_score;
+    d.cem_target = sc.d_cem_target;
+    cls->table.push_back(d);
+  }
+  for (auto& cls : b->sam) {
+    const int ppts[2] = {4, 2};
+    for (int k = 0; k < 2; ++k) {
+      std::vector<SamSceneDev> t = cls.table;
+      std::vector<int> owner;
+      unsigned total = 0;
+      for (std::size_t j = 0; j < t.size(); ++j) {
+        t[j].block_begin = total;
+        const unsigned nblk = static_cast<unsigned>(div_up(t[j].pixels, static_cast<std::size_t>(256 * ppts[k])));
+        owner.insert(owner.end(), nblk, static_cast<int>(j));
+        total += nblk;
+      }
+      cls.blocks[k] = total;
+      if (upload_vec(&cls.d_table[k], t) != hipSuccess || upload_vec(&cls.d_block_scene[k], owner) != hipSuccess)
+        return bfail(b, HSI_ERR_DEVICE, "upload scene table");
+    }
+  }
+  // RX/ACE classes by row blocks and target tiles.
+  for (int i = 0; i < count; ++i) {
+    const hsi_scene& sc = scenes[i];
+    if (!sc.d_rx_score && !sc.d_ace_score) continue;
+    hsi_ctx* c = b->ctx[static_cast<std::size_t>(i)];
+    const int nb = rx_blocks(sc.bands);
+    const int T = sc.d_ace_score ? c->num_targets : 0;
+    const int tt = T == 0 ? 0 : (T <= 16 ? 1 : 2);
+    hsi_batch::RxClass* cls = nullptr;
+    for (auto& k : b->rx)
+      if (k.nb == nb && k.tt == tt) cls = &k;
+    if (!cls) {
+      b->rx.emplace_back();
+      cls = &b->rx.back();
+      cls->nb = nb;
+      cls->tt = tt;
+    }
+    RxSceneDev d{};
+    d.cube = sc.d_cube;
+    d.pixels = static_cast<std::size_t>(sc.height) * sc.width;
+    d.bands = sc.bands;
+    d.stride = sc.plane_stride;
+    d.num_targets = T;
+    d.mean = c->d_mean;
+    d.white_blocks = c->d_white_blocks;
+    d.ace_u = c->d_ace_u;
+    d.target_norm2 = c->d_target_norm2;
+    d.rx = sc.d_rx_score;
+    d.ace_score = sc.d_ace_score;
+    d.ace_target = sc.d_ace_target;
+    const int scene_index = static_cast<int>(cls->table.size());
+    cls->table.push_back(d);
+    const std::size_t tiles = div_up(d.pixels, 16);
+    for (int k = 0; k < 3; ++k)
+      for (std::size_t t0 = 0; t0 < tiles; t0 += static_cast<std::size_t>(kRxChunks[k]))
+        cls->items[k].push_back(RxWorkItem{scene_index, t0, std::min(tiles, t0 + static_cast<std::size_t>(kRxChunks[k]))});
+  }
+  for (auto& cls : b->rx) {
+    if (upload_vec(&cls.d_table, cls.table) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "upload scene table");
+    for (int k = 0; k < 3; ++k)
+      if (upload_vec(&cls.d_items[k], cls.items[k]) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "upload work items");
+  }
+  return HSI_OK;
+}
+
+hsi_status hsi_batch_statistics(hsi_batch* b, double loading, hsi_stream stream) {
+  if (!b) return HSI_ERR_INVALID_ARG;
+  for (std::size_t i = 0; i < b->scenes.size(); ++i) {
+    const hsi_scene& sc = b->scenes[i];
+    if (!sc.d_rx_score && !sc.d_ace_score && !sc.d_cem_score) continue;
+    hsi_ctx* c = b->ctx[i];
+    hsi_status st = hsi_stats_accumulate(c, sc.d_cube, sc.height, sc.width, sc.bands, sc.plane_stride, HSI_PLAIN, stream);
+    if (st == HSI_OK) st = hsi_stats_build(c, loading, stream);
+    if (st == HSI_OK) st = ensure_layouts(c, static_cast<hipStream_t>(stream));
+    if (st != HSI_OK) return bfail(b, st, "scene " + std::to_string(i) + ": " + c->error);
+  }
+  return HSI_OK;
+}
+
+hsi_status hsi_batch_check(hsi_batch* b, int* failed_scene) {
+  if (!b) return HSI_ERR_INVALID_ARG;
+  if (failed_scene) *failed_scene = -1;
+  for (std::size_t i = 0; i < b->scenes.size(); ++i) {
+    hsi_ctx* c = b->ctx[i];
+    if (c->stats_bands <= 0) continue;
+    const hsi_status st = hsi_stats_check(c, nullptr);
+    if (st != HSI_OK) {
+      if (failed_scene) *failed_scene = static_cast<int>(i);
+      return bfail(b, st, "scene " + std::to_string(i) + ": " + c->error);
+    }
+  }
+  return HSI_OK;
+}
+
+hsi_status hsi_batch_score(hsi_batch* b, hsi_stream stream) {
+  if (!b) return HSI_ERR_INVALID_ARG;
+  if (hipSetDevice(b->device) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "select device");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  for (std::size_t i = 0; i < b->scenes.size(); ++i) {
+    const hsi_scene& sc = b->scenes[i];
+    if ((sc.d_cem_score || sc.d_rx_score || sc.d_ace_score) && b->ctx[i]->stats_bands != sc.bands)
+      return bfail(b, HSI_ERR_NO_STATISTICS, "scene " + std::to_string(i) + ": run hsi_batch_statistics first");
+  }
+  for (const auto& c : b->sam) {
+    const hipError_t e = run_sam_class(c, c.ppt, c.workgroups, s);
+    if (e != hipSuccess) return bfail(b, HSI_ERR_DEVICE, std::string("batched SAM/CEM: ") + hipGetErrorString(e));
+  }
+  for (const auto& c : b->rx) {
+    const hipError_t e = run_rx_class(c, c.chunk, c.workgroups, s);
+    if (e != hipSuccess) return bfail(b, HSI_ERR_DEVICE, std::string("batched RX/ACE: ") + hipGetErrorString(e));
+  }
+  return HSI_OK;
+}
+
+hsi_status hsi_batch_autotune(hsi_batch* b, hsi_stream stream, char* report, size_t report_len) {
+  if (!b) return HSI_ERR_INVALID_ARG;
+  if (hipSetDevice(b->device) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "select device");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return bfail(b, HSI_ERR_DEVICE, "create events");
+  std::string text;
+  auto time_ms = [&](auto&& launch) -> float {
+    if (launch() != hipSuccess) return -1.f;  // warm-up; an invalid shape is skipped
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < 3; ++r) (void)launch();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return ms / 3.f;
+  };
+  for (auto& c : b->sam) {
+    float best = -1.f;
+    for (int ppt : {4, 2}) {
+      if (c.lib_stride > 8 && ppt == 4) continue;
+      for (int wgs : {0, 2048, 8192}) {
+        const float ms = time_ms([&] { return run_sam_class(c, ppt, wgs, s); });
+        if (ms >= 0.f && (best < 0.f || ms < best)) {
+          best = ms;
+          c.ppt = ppt;
+          c.workgroups = wgs;
+        }
+      }
+    }
+    text += "sam/cem targets<=" + std::to_string(c.lib_stride) + " scenes=" + std::to_string(c.table.size()) +
+            ": ppt " + std::to_string(c.ppt) + ", workgroups " +
+            (c.workgroups > 0 ? std::to_string(c.workgroups) + " persistent" : std::string("one per block")) + ", " +
+            std::to_string(best * 1e3f) + " us\n";
+  }
+  for (auto& c : b->rx) {
+    float best = -1.f;
+    for (int chunk = 0; chunk < 3; ++chunk)
+      for (int wgs : {0, 256, 512, 1024}) {
+        const float ms = time_ms([&] { return run_rx_class(c, chunk, wgs, s); });
+        if (ms >= 0.f && (best < 0.f || ms < best)) {
+          best = ms;
+          c.chunk = chunk;
+          c.workgroups = wgs;
+        }
+      }
+    text += "rx/ace bands<=" + std::to_string(16 * c.nb) + " target tiles=" + std::to_string(c.tt) +
+            " scenes=" + std::to_string(c.table.size()) + ": tiles/item " + std::to_string(kRxChunks[c.chunk]) +
+            ", workgroups " + (c.workgroups > 0 ? std::to_string(c.workgroups) + " persistent" : std::string("one per item")) +
+            ", " + std::to_string(best * 1e3f) + " us\n";
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tuned = true;
+  if (report && report_len > 0) {
+    std::strncpy(report, text.c_str(), report_len - 1);
+    report[report_len - 1] = '\0';
+  }
   return HSI_OK;
 }
 

@@ -46,7 +46,51 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
   }
 }
 
+// Snapshot-mosaic demosaic: band b of output pixel (y, x) is raw sample
+// (y * period + pos / period, x * period + pos % period), pos = the band's
+// mosaic position; integer samples scaled to [0, 1] in float as the
+// reference's PNG loader does (value * (1 / full scale)). One thread per
+// output pixel per band; reads of one band plane stride by `period`.
+template <typename Raw>
+__global__ void demosaic_kernel(const Raw* __restrict__ raw, int raw_width, int out_h, int out_w, int period,
+                                int bands, const int* __restrict__ position, float scale, float* __restrict__ cube,
+                                std::size_t stride) {
+  const std::size_t pixels = static_cast<std::size_t>(out_h) * out_w;
+  const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= pixels || b >= bands) return;
+  const int y = static_cast<int>(i / out_w), x = static_cast<int>(i % out_w);
+  const int pos = position[b];
+  const std::size_t r = static_cast<std::size_t>(y * period + pos / period) * raw_width +
+                        static_cast<std::size_t>(x) * period + pos % period;
+  cube[static_cast<std::size_t>(b) * stride + i] = static_cast<float>(raw[r]) * scale;
+}
+
 }  // namespace
+
+hipError_t launch_demosaic(const void* raw, int raw_type, int raw_height, int raw_width, int period, int bands,
+                           const int* d_position, float* cube, std::size_t stride, hipStream_t stream) {
+  (void)raw_height;
+  const int out_h = raw_height / period, out_w = raw_width / period;
+  const std::size_t pixels = static_cast<std::size_t>(out_h) * out_w;
+  const dim3 grid(static_cast<unsigned>(div_up(pixels, 256)), static_cast<unsigned>(bands));
+  switch (raw_type) {
+    case 0:
+      hipLaunchKernelGGL((demosaic_kernel<unsigned char>), grid, dim3(256), 0, stream,
+                         static_cast<const unsigned char*>(raw), raw_width, out_h, out_w, period, bands, d_position,
+                         1.0f / 255.0f, cube, stride);
+      break;
+    case 1:
+      hipLaunchKernelGGL((demosaic_kernel<unsigned short>), grid, dim3(256), 0, stream,
+                         static_cast<const unsigned short*>(raw), raw_width, out_h, out_w, period, bands, d_position,
+                         1.0f / 65535.0f, cube, stride);
+      break;
+    default:
+      hipLaunchKernelGGL((demosaic_kernel<float>), grid, dim3(256), 0, stream, static_cast<const float*>(raw),
+                         raw_width, out_h, out_w, period, bands, d_position, 1.0f, cube, stride);
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_narrow_to_half(const float* src, _Float16* dst, std::size_t count,
                                  hipStream_t stream) {

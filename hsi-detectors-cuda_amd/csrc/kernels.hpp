@@ -131,6 +131,56 @@ hipError_t launch_rxace_plain(const RxAceArgs& args, hipStream_t stream);
 hipError_t launch_rxace_candidates(const RxAceArgs& args, const hsi_detection* cand, const unsigned* count,
                                    unsigned capacity, int width, hipStream_t stream);
 hipError_t launch_fill_scores(float* score, int* target, float* aux, std::size_t pixels, hipStream_t stream);
+
+// Batched plain RX/ACE: scenes of one row-block count NB (= rx_blocks(bands))
+// and target-tile count TT (0: RX only, 1: <= 16 targets, 2: <= 32), FP32
+// cubes, statistics layouts already derived.
+struct RxSceneDev {
+  const void* cube;
+  std::size_t pixels;
+  int bands;
+  std::size_t stride;
+  int num_targets;
+  const float* mean;
+  const float* white_blocks;
+  const float* ace_u;
+  const float* target_norm2;
+  float* rx;
+  float* ace_score;
+  int* ace_target;
+};
+struct RxWorkItem {
+  int scene;
+  std::size_t tile_begin, tile_end;  // 16-pixel tiles of that scene
+};
+// Batched SAM/CEM over scenes sharing a library stride (next power of two of
+// the target count); FP32 cubes. block_begin numbers the scenes' 256*PPT
+// pixel blocks in order; a NULL score map skips that detector for the scene.
+struct SamSceneDev {
+  const void* cube;
+  std::size_t pixels;
+  int bands;
+  std::size_t stride;
+  int num_targets;
+  int exact_all;
+  const double* sam_lib;
+  const float* sam_norms;
+  const double* sam_rnorms;
+  const double* cem_lib;
+  float* sam_score;
+  int* sam_target;
+  float* cem_score;
+  int* cem_target;
+  unsigned block_begin;
+};
+// block_scene[b] = scene of pixel block b. SAM and CEM run fused up to 16
+// targets, at 32 targets as two launches. workgroups > 0: a persistent grid
+// of that size; <= 0: one workgroup per block.
+hipError_t launch_sam_cem_batch(int lib_stride, int ppt, const SamSceneDev* scenes, const int* block_scene,
+                                unsigned total_blocks, int workgroups, hipStream_t stream);
+// workgroups > 0: a persistent grid of that size; <= 0: one workgroup per item.
+hipError_t launch_rxace_batch(int nb, int tt, const RxSceneDev* scenes, const RxWorkItem* work, int num_items,
+                              int workgroups, hipStream_t stream);
 hipError_t launch_rxace_mixed(const RxAceArgs& args, hipStream_t stream);
 
 // ---- learned pre-screen (mlp.hip) --------------------------------------------
@@ -160,6 +210,11 @@ hipError_t launch_adam(float* w, float* m, float* v, const float* g, std::size_t
                        float beta2, float eps, float c1, float c2, hipStream_t stream);
 hipError_t launch_gather(const float* cube, std::size_t stride, std::size_t pixels, int bands, const int* index,
                          int n, const float* center, float* out, std::size_t out_stride, hipStream_t stream);
+
+// ---- ingest (ingest.hip) -------------------------------------------------------
+// raw_type: 0 = u8, 1 = u16, 2 = f32; d_position[bands] = mosaic positions.
+hipError_t launch_demosaic(const void* raw, int raw_type, int raw_height, int raw_width, int period, int bands,
+                           const int* d_position, float* cube, std::size_t stride, hipStream_t stream);
 
 // ---- detect ------------------------------------------------------------------
 std::size_t detect_scratch_words(std::size_t pixels);

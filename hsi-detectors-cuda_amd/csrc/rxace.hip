@@ -37,30 +37,19 @@ __device__ __forceinline__ void ace_merge(double& s, int& t, double s2, int t2) 
   }
 }
 
+// Scores tiles first, first + step, ... below the tile count of one scene
+// whose whitener blocks, target tiles and mean are already in LDS.
 // GATHER: score only the pixels named by a candidate list (detection records
-// in raster order, pixel y * width + x), min(*cand_count, cand_cap) of them;
-// each 16-record slice of the list is one tile.
+// in raster order, pixel y * width + x), *cand_count of them (the whole frame
+// if that exceeds cand_cap); each 16-record slice of the list is one tile.
 template <typename In, int NB, int TT, bool GATHER>
-__global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
-    const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
-    int num_targets, const float* __restrict__ mean, const float* __restrict__ white_blocks,
-    const float* __restrict__ ace_u, const float* __restrict__ target_norm2,
-    float* __restrict__ rx_out, float* __restrict__ ace_score, int* __restrict__ ace_target,
+__device__ __forceinline__ void rxace_tiles(
+    const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride, int num_targets,
+    const float* __restrict__ target_norm2, float* __restrict__ rx_out, float* __restrict__ ace_score,
+    int* __restrict__ ace_target, const float* w_lds, const float* u_lds,
+    const float* mean_lds, std::size_t first, std::size_t step, std::size_t last,
     const hsi_detection* __restrict__ cand, const unsigned* __restrict__ cand_count, unsigned cand_cap,
     int width) {
-  constexpr int NPAIR = NB * (NB + 1) / 2;
-  constexpr int WFLOATS = NPAIR * 256;
-  constexpr int UFLOATS = (TT > 0 ? TT : 1) * NB * 256;
-  __shared__ float w_lds[WFLOATS];
-  __shared__ float u_lds[UFLOATS];
-  __shared__ float mean_lds[NB * 16];
-
-  for (int e = threadIdx.x; e < WFLOATS; e += blockDim.x) w_lds[e] = white_blocks[e];
-  if constexpr (TT > 0)
-    for (int e = threadIdx.x; e < UFLOATS; e += blockDim.x) u_lds[e] = ace_u[e];
-  for (int e = threadIdx.x; e < NB * 16; e += blockDim.x) mean_lds[e] = e < bands ? mean[e] : 0.f;
-  __syncthreads();
-
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;   // lane group: K index of the A/B operands
   const int n = lane & 15;   // pixel column within the tile
@@ -73,7 +62,8 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
     if (c > cand_cap) gather = false;
     else items = c;
   }
-  const std::size_t tiles = div_up(items, 16);
+  std::size_t tiles = div_up(items, 16);
+  if (last < tiles) tiles = last;
   // Pixel of tile element t*16 + n, and whether it exists.
   auto pixel_of = [&](std::size_t t, bool& ok) -> std::size_t {
     const std::size_t i = t * 16 + n;
@@ -90,8 +80,8 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
       return i;
     }
   };
-  const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(kRxWaves) + (threadIdx.x >> 6);
-  const std::size_t waves = static_cast<std::size_t>(gridDim.x) * kRxWaves;
+  const std::size_t wave_id = first;
+  const std::size_t waves = step;
 
   double uu[TT > 0 ? TT : 1][4];
   if constexpr (TT > 0) {
@@ -122,6 +112,9 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
   if (wave_id < tiles) load_tile(wave_id, cur);
 
   for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
+    // Read the whitener blocks from LDS at each use rather than hoisting them
+    // out of the tile loop into registers.
+    __asm__ volatile("" ::: "memory");
     bool live;
     const std::size_t p = pixel_of(tile, live);
     if (tile + waves < tiles) load_tile(tile + waves, nxt);
@@ -205,6 +198,87 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
       }
     }
     if (live && g == 0 && rx_out) rx_out[p] = static_cast<float>(zz);
+  }
+}
+
+
+template <int NB, int TT>
+struct RxLds {
+  static constexpr int WFLOATS = NB * (NB + 1) / 2 * 256;
+  static constexpr int UFLOATS = (TT > 0 ? TT : 1) * NB * 256;
+};
+
+template <int NB, int TT>
+__device__ __forceinline__ void rxace_stage(const float* __restrict__ white_blocks, const float* __restrict__ ace_u,
+                                            const float* __restrict__ mean, int bands, float* w_lds, float* u_lds,
+                                            float* mean_lds) {
+  for (int e = threadIdx.x; e < RxLds<NB, TT>::WFLOATS; e += blockDim.x) w_lds[e] = white_blocks[e];
+  if constexpr (TT > 0)
+    for (int e = threadIdx.x; e < RxLds<NB, TT>::UFLOATS; e += blockDim.x) u_lds[e] = ace_u[e];
+  for (int e = threadIdx.x; e < NB * 16; e += blockDim.x) mean_lds[e] = e < bands ? mean[e] : 0.f;
+}
+
+template <typename In, int NB, int TT, bool GATHER>
+__global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
+    const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
+    int num_targets, const float* __restrict__ mean, const float* __restrict__ white_blocks,
+    const float* __restrict__ ace_u, const float* __restrict__ target_norm2,
+    float* __restrict__ rx_out, float* __restrict__ ace_score, int* __restrict__ ace_target,
+    const hsi_detection* __restrict__ cand, const unsigned* __restrict__ cand_count, unsigned cand_cap,
+    int width) {
+  __shared__ float w_lds[RxLds<NB, TT>::WFLOATS];
+  __shared__ float u_lds[RxLds<NB, TT>::UFLOATS];
+  __shared__ float mean_lds[NB * 16];
+  rxace_stage<NB, TT>(white_blocks, ace_u, mean, bands, w_lds, u_lds, mean_lds);
+  __syncthreads();
+  const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(kRxWaves) + (threadIdx.x >> 6);
+  rxace_tiles<In, NB, TT, GATHER>(cube, pixels, bands, stride, num_targets, target_norm2, rx_out, ace_score,
+                                  ace_target, w_lds, u_lds, mean_lds, wave_id,
+                                  static_cast<std::size_t>(gridDim.x) * kRxWaves, ~static_cast<std::size_t>(0),
+                                  cand, cand_count, cand_cap, width);
+}
+
+// Batched scenes of one shape class in one launch over work items (a scene
+// and a range of its tiles). PERSIST: a grid of fixed size where each
+// workgroup walks a contiguous run of items, restaging the scene's whitener
+// and targets in LDS when its run crosses into the next scene; otherwise one
+// workgroup per item, which keeps the single-scene kernel's register budget
+// (the item loop costs registers the largest whiteners do not have).
+template <int NB, int TT, bool PERSIST>
+__global__ __launch_bounds__(kRxWaves * 64) void rxace_batch_kernel(const RxSceneDev* __restrict__ scenes,
+                                                                     const RxWorkItem* __restrict__ work,
+                                                                     int num_items) {
+  __shared__ float w_lds[RxLds<NB, TT>::WFLOATS];
+  __shared__ float u_lds[RxLds<NB, TT>::UFLOATS];
+  __shared__ float mean_lds[NB * 16];
+  if constexpr (PERSIST) {
+    const int per = (num_items + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x);
+    const int it_end = min(num_items, (static_cast<int>(blockIdx.x) + 1) * per);
+    int staged = -1;
+    for (int it = static_cast<int>(blockIdx.x) * per; it < it_end; ++it) {
+      const RxWorkItem w = work[it];
+      const RxSceneDev& sc = scenes[w.scene];
+      if (w.scene != staged) {
+        __syncthreads();
+        rxace_stage<NB, TT>(sc.white_blocks, sc.ace_u, sc.mean, sc.bands, w_lds, u_lds, mean_lds);
+        __syncthreads();
+        staged = w.scene;
+      }
+      rxace_tiles<float, NB, TT, false>(static_cast<const float*>(sc.cube), sc.pixels, sc.bands, sc.stride,
+                                        sc.num_targets, sc.target_norm2, sc.rx, sc.ace_score, sc.ace_target, w_lds,
+                                        u_lds, mean_lds, w.tile_begin + (threadIdx.x >> 6), kRxWaves, w.tile_end,
+                                        nullptr, nullptr, 0, 0);
+    }
+  } else {
+    if (static_cast<int>(blockIdx.x) >= num_items) return;
+    const RxWorkItem w = work[blockIdx.x];
+    const RxSceneDev& sc = scenes[w.scene];
+    rxace_stage<NB, TT>(sc.white_blocks, sc.ace_u, sc.mean, sc.bands, w_lds, u_lds, mean_lds);
+    __syncthreads();
+    rxace_tiles<float, NB, TT, false>(static_cast<const float*>(sc.cube), sc.pixels, sc.bands, sc.stride,
+                                      sc.num_targets, sc.target_norm2, sc.rx, sc.ace_score, sc.ace_target, w_lds,
+                                      u_lds, mean_lds, w.tile_begin + (threadIdx.x >> 6), kRxWaves, w.tile_end,
+                                      nullptr, nullptr, 0, 0);
   }
 }
 
@@ -304,6 +378,37 @@ std::size_t rxace_white_floats(int bands) {
 }
 std::size_t rxace_target_floats(int bands, int num_targets) {
   return static_cast<std::size_t>(num_targets > 16 ? 2 : 1) * rx_blocks(bands) * 256;
+}
+
+namespace {
+template <int NB, bool PERSIST>
+hipError_t launch_batch_p(int tt, const RxSceneDev* scenes, const RxWorkItem* work, int n, int wgs,
+                          hipStream_t stream) {
+  const dim3 grid(static_cast<unsigned>(wgs)), block(kRxWaves * 64);
+  if (tt == 0) hipLaunchKernelGGL((rxace_batch_kernel<NB, 0, PERSIST>), grid, block, 0, stream, scenes, work, n);
+  else if (tt == 1) hipLaunchKernelGGL((rxace_batch_kernel<NB, 1, PERSIST>), grid, block, 0, stream, scenes, work, n);
+  else hipLaunchKernelGGL((rxace_batch_kernel<NB, 2, PERSIST>), grid, block, 0, stream, scenes, work, n);
+  return hipGetLastError();
+}
+template <int NB>
+hipError_t launch_batch_nb(int tt, const RxSceneDev* scenes, const RxWorkItem* work, int n, int wgs,
+                           hipStream_t stream) {
+  // workgroups <= 0: one workgroup per item
+  if (wgs <= 0) return launch_batch_p<NB, false>(tt, scenes, work, n, n, stream);
+  return launch_batch_p<NB, true>(tt, scenes, work, n, wgs, stream);
+}
+}  // namespace
+
+hipError_t launch_rxace_batch(int nb, int tt, const RxSceneDev* scenes, const RxWorkItem* work, int num_items,
+                              int workgroups, hipStream_t stream) {
+  if (num_items <= 0) return hipSuccess;
+  switch (nb) {
+#define HSI_NB(N) case N: return launch_batch_nb<N>(tt, scenes, work, num_items, workgroups, stream);
+    HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8)
+    HSI_NB(9) HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14)
+#undef HSI_NB
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_rxace_plain(const RxAceArgs& a, hipStream_t stream) {

@@ -112,16 +112,15 @@ __device__ __forceinline__ void sam_offer(SamBest& best, bool degenerate, double
   }
 }
 
+// One thread's PPT consecutive pixels starting at p0 (< pixels).
 template <typename In, int NT, int PPT, bool DO_SAM, bool DO_CEM>
-__global__ __launch_bounds__(256) void sam_cem_kernel(
+__device__ __forceinline__ void sam_cem_group(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
     const double* __restrict__ slib, const float* __restrict__ snorm,
     const double* __restrict__ srnorm, int exact_all,
     const double* __restrict__ clib, int num_targets, float* __restrict__ s_score,
-    int* __restrict__ s_target, float* __restrict__ c_score, int* __restrict__ c_target) {
+    int* __restrict__ s_target, float* __restrict__ c_score, int* __restrict__ c_target, std::size_t p0) {
   using V = typename VecOf<In, PPT>::type;
-  const std::size_t p0 = (blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x) * PPT;
-  if (p0 >= pixels) return;
 
   double sq[PPT];
   double sd[DO_SAM ? NT : 1][PPT];
@@ -209,15 +208,37 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
                                  ? -__longlong_as_double(0x7ff0000000000000ll)
                                  : (top < 1.0 ? top : 1.0) - 1e-14;
       SamBest best;
+      if constexpr (NT <= 16) {
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        if (t < num_targets && (all_exact || approx[t] >= floor_c)) {
-          const double norm = static_cast<double>(snorm[t]);
-          const bool degenerate = sq[k] <= 0.0 || norm <= 0.0;
-          double c = sd[t][k] / (root * norm);
-          c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
-          sam_offer(best, degenerate, c, t);
+        for (int t = 0; t < NT; ++t) {
+          if (t < num_targets && (all_exact || approx[t] >= floor_c)) {
+            const double norm = static_cast<double>(snorm[t]);
+            const bool degenerate = sq[k] <= 0.0 || norm <= 0.0;
+            double c = sd[t][k] / (root * norm);
+            c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+            sam_offer(best, degenerate, c, t);
+          }
         }
+      } else {
+        // 32 targets: the exact quotients are formed in an unrolled pass
+        // (so the accumulators stay in registers) and offered in a rolled
+        // one, which keeps the acos-carrying tie logic out of the unrolled
+        // code; only these per-pixel candidate values may live in scratch.
+        double cex[NT];
+        unsigned long long take = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          cex[t] = 0.0;
+          if (t < num_targets && (all_exact || approx[t] >= floor_c)) {
+            double c = sd[t][k] / (root * static_cast<double>(snorm[t]));
+            cex[t] = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+            take |= 1ull << t;
+          }
+        }
+#pragma unroll 1
+        for (int t = 0; t < NT; ++t)
+          if ((take >> t) & 1ull)
+            sam_offer(best, sq[k] <= 0.0 || static_cast<double>(snorm[t]) <= 0.0, cex[t], t);
       }
       double angle = __longlong_as_double(0x7ff0000000000000ll);  // +inf
       if (best.kind == 1) angle = kMaxSamAngle;
@@ -243,11 +264,11 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
   // full cache lines; the ragged tail of the frame stores pixel by pixel.
   if (p0 + PPT <= pixels) {
     if constexpr (DO_SAM) {
-      store_group<PPT>(s_score + p0, s_out);
+      if (s_score) store_group<PPT>(s_score + p0, s_out);
       if (s_target) store_group<PPT>(s_target + p0, st_out);
     }
     if constexpr (DO_CEM) {
-      store_group<PPT>(c_score + p0, c_out);
+      if (c_score) store_group<PPT>(c_score + p0, c_out);
       if (c_target) store_group<PPT>(c_target + p0, ct_out);
     }
   } else {
@@ -256,14 +277,53 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
       const std::size_t p = p0 + k;
       if (p >= pixels) break;
       if constexpr (DO_SAM) {
-        s_score[p] = s_out[k];
+        if (s_score) s_score[p] = s_out[k];
         if (s_target) s_target[p] = st_out[k];
       }
       if constexpr (DO_CEM) {
-        c_score[p] = c_out[k];
+        if (c_score) c_score[p] = c_out[k];
         if (c_target) c_target[p] = ct_out[k];
       }
     }
+  }
+}
+
+template <typename In, int NT, int PPT, bool DO_SAM, bool DO_CEM>
+__global__ __launch_bounds__(256) void sam_cem_kernel(
+    const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
+    const double* __restrict__ slib, const float* __restrict__ snorm,
+    const double* __restrict__ srnorm, int exact_all,
+    const double* __restrict__ clib, int num_targets, float* __restrict__ s_score,
+    int* __restrict__ s_target, float* __restrict__ c_score, int* __restrict__ c_target) {
+  const std::size_t p0 = (blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x) * PPT;
+  if (p0 >= pixels) return;
+  sam_cem_group<In, NT, PPT, DO_SAM, DO_CEM>(cube, pixels, bands, stride, slib, snorm, srnorm, exact_all, clib,
+                                            num_targets, s_score, s_target, c_score, c_target, p0);
+}
+
+// Batched scenes of one target-count class (library stride NT) in one
+// persistent launch: the scenes' 256*PPT-pixel blocks are numbered in scene
+// order (block_begin prefix), and each workgroup takes every gridDim.x-th
+// block, finding its scene by a binary search over the prefix (uniform
+// across the workgroup, so it runs on the scalar unit).
+// block_scene[b] names the scene of block b. PERSIST: a fixed grid walks the
+// blocks with a stride; otherwise one workgroup per block, which keeps the
+// single-scene kernel's register budget.
+template <typename In, int NT, int PPT, bool DO_SAM, bool DO_CEM, bool PERSIST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void sam_cem_batch_kernel(const SamSceneDev* __restrict__ scenes,
+                                                            const int* __restrict__ block_scene,
+                                                            unsigned total_blocks) {
+  for (unsigned blk = blockIdx.x; blk < total_blocks; blk += gridDim.x) {
+    const SamSceneDev& sc = scenes[block_scene[blk]];
+    const std::size_t p0 = (static_cast<std::size_t>(blk - sc.block_begin) * 256 + threadIdx.x) * PPT;
+    if (p0 < sc.pixels)
+      // A scene without one of the maps still runs the shared loop; its
+      // results for the missing detector are not stored.
+      sam_cem_group<In, NT, PPT, DO_SAM, DO_CEM>(static_cast<const In*>(sc.cube), sc.pixels, sc.bands, sc.stride,
+                                                sc.sam_lib, sc.sam_norms, sc.sam_rnorms, sc.exact_all, sc.cem_lib,
+                                                sc.num_targets, sc.sam_score, sc.sam_target, sc.cem_score,
+                                                sc.cem_target, p0);
+    if constexpr (!PERSIST) break;
   }
 }
 
@@ -283,6 +343,10 @@ template <typename In, int NT, int PPT>
 hipError_t launch_mode(const SamCemArgs& a, hipStream_t stream) {
   const bool s = a.sam_score != nullptr;
   const bool c = a.cem_score != nullptr;
+  if (s && c && NT > 16) {  // fused 32-target accumulators would not fit in registers
+    const hipError_t e = launch_one<In, NT, PPT, true, false>(a, stream);
+    return e != hipSuccess ? e : launch_one<In, NT, PPT, false, true>(a, stream);
+  }
   if (s && c) return launch_one<In, NT, PPT, true, true>(a, stream);
   if (s) return launch_one<In, NT, PPT, true, false>(a, stream);
   return launch_one<In, NT, PPT, false, true>(a, stream);
@@ -302,6 +366,40 @@ hipError_t launch_typed(const SamCemArgs& a, hipStream_t stream) {
 }
 
 }  // namespace
+
+template <int NT, int P, bool PERSIST>
+void launch_sam_batch_passes(const SamSceneDev* scenes, const int* block_scene, unsigned total_blocks, dim3 grid,
+                             hipStream_t stream) {
+  const dim3 block(256);
+  if constexpr (NT <= 16) {
+    hipLaunchKernelGGL((sam_cem_batch_kernel<float, NT, P, true, true, PERSIST>), grid, block, 0, stream, scenes,
+                       block_scene, total_blocks);
+  } else {
+    // 32 targets: SAM and CEM as two passes, the fused accumulators would
+    // not fit in registers
+    hipLaunchKernelGGL((sam_cem_batch_kernel<float, NT, P, true, false, PERSIST>), grid, block, 0, stream, scenes,
+                       block_scene, total_blocks);
+    hipLaunchKernelGGL((sam_cem_batch_kernel<float, NT, P, false, true, PERSIST>), grid, block, 0, stream, scenes,
+                       block_scene, total_blocks);
+  }
+}
+
+hipError_t launch_sam_cem_batch(int lib_stride, int ppt, const SamSceneDev* scenes, const int* block_scene,
+                                unsigned total_blocks, int workgroups, hipStream_t stream) {
+  if (total_blocks == 0) return hipSuccess;
+  const bool persist = workgroups > 0;
+  const dim3 grid(persist ? static_cast<unsigned>(workgroups) : total_blocks);
+#define HSI_B(NT, P)                                                                            \
+  if (lib_stride == NT && ppt == P) {                                                           \
+    if (persist) launch_sam_batch_passes<NT, P, true>(scenes, block_scene, total_blocks, grid, stream); \
+    else launch_sam_batch_passes<NT, P, false>(scenes, block_scene, total_blocks, grid, stream);        \
+    return hipGetLastError();                                                                   \
+  }
+  HSI_B(1, 4) HSI_B(2, 4) HSI_B(4, 4) HSI_B(8, 4) HSI_B(16, 2) HSI_B(32, 2)
+  HSI_B(1, 2) HSI_B(2, 2) HSI_B(4, 2) HSI_B(8, 2)
+#undef HSI_B
+  return hipErrorInvalidValue;
+}
 
 int sam_cem_lib_stride(int num_targets) {
   int nt = 1;

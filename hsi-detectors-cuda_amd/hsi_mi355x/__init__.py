@@ -102,6 +102,16 @@ class MlpTrainParams(C.Structure):
         super().__init__(learning_rate, beta1, beta2, epsilon, pos_weight)
 
 
+class Scene(C.Structure):
+    """hsi_scene: one member of a batch (device cube, host library, output maps)."""
+
+    _fields_ = [("d_cube", C.c_void_p), ("height", C.c_int), ("width", C.c_int), ("bands", C.c_int),
+                ("plane_stride", C.c_size_t), ("library", C.c_void_p), ("num_targets", C.c_int),
+                ("d_sam_score", C.c_void_p), ("d_sam_target", C.c_void_p),
+                ("d_cem_score", C.c_void_p), ("d_cem_target", C.c_void_p),
+                ("d_rx_score", C.c_void_p), ("d_ace_score", C.c_void_p), ("d_ace_target", C.c_void_p)]
+
+
 class SynthConfig(C.Structure):
     _fields_ = [("height", C.c_int32), ("width", C.c_int32), ("bands", C.c_int32),
                 ("num_targets", C.c_int32), ("num_endmembers", C.c_int32),
@@ -169,6 +179,17 @@ def lib():
                                         C.c_int, C.c_int, _vp, C.c_size_t, _vp]),
         "hsi_score_candidates": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                            C.c_size_t, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
+        "hsi_demosaic": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
+                                   C.c_size_t, _vp]),
+        "hsi_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+        "hsi_batch_destroy": (None, [_vp]),
+        "hsi_batch_last_error": (C.c_char_p, [_vp]),
+        "hsi_batch_set_scenes": (C.c_int, [_vp, C.POINTER(Scene), C.c_int]),
+        "hsi_batch_statistics": (C.c_int, [_vp, C.c_double, _vp]),
+        "hsi_batch_check": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+        "hsi_batch_score": (C.c_int, [_vp, _vp]),
+        "hsi_batch_autotune": (C.c_int, [_vp, _vp, C.c_char_p, C.c_size_t]),
+        "hsi_batch_context": (_vp, [_vp, C.c_int]),
         "hsi_synth_default": (None, [C.POINTER(SynthConfig)]),
         "hsi_synth_library": (C.c_int, [C.POINTER(SynthConfig), _vp]),
         "hsi_synth_anomaly": (C.c_int, [C.POINTER(SynthConfig), _vp]),
@@ -285,9 +306,9 @@ class Context:
         self.device = device
 
     def close(self):
-        if self.handle:
+        if self.handle and not getattr(self, "_borrowed", False):
             self._lib.hsi_ctx_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -415,6 +436,93 @@ class Context:
                                                    width, bands, cube.stride(0), _ptr(candidates),
                                                    _ptr(count), cap, _ptr(score), _ptr(target),
                                                    _ptr(aux), _stream(stream)))
+
+
+RAW_U8, RAW_U16, RAW_F32 = 0, 1, 2
+
+
+class Batch:
+    """hsi_batch: uneven scenes scored by one persistent launch per shape class."""
+
+    def __init__(self, device=0):
+        _torch()
+        self._lib = lib()
+        h = C.c_void_p()
+        st = self._lib.hsi_batch_create(device, C.byref(h))
+        if st != OK:
+            raise HsiError(st, f"cannot create a batch on device {device}")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.hsi_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != OK:
+            raise HsiError(st, self._lib.hsi_batch_last_error(self.handle).decode())
+
+    def set_scenes(self, scenes):
+        """scenes: list of dicts with cube (device (bands, stride) f32 tensor),
+        height, width, library (host [T, B] or None) and optional output
+        tensors sam_score, sam_target, cem_score, cem_target, rx, ace_score,
+        ace_target."""
+        arr = (Scene * len(scenes))()
+        self._keep = []
+        for i, d in enumerate(scenes):
+            cube = d["cube"]
+            libv = d.get("library")
+            if libv is not None:
+                libv = np.ascontiguousarray(libv, np.float32)
+                self._keep.append(libv)
+            arr[i] = Scene(cube.data_ptr(), d["height"], d["width"], cube.shape[0], cube.stride(0),
+                           None if libv is None else libv.ctypes.data, 0 if libv is None else libv.shape[0],
+                           *[None if d.get(k) is None else d[k].data_ptr() for k in
+                             ("sam_score", "sam_target", "cem_score", "cem_target", "rx", "ace_score",
+                              "ace_target")])
+        self._check(self._lib.hsi_batch_set_scenes(self.handle, arr, len(scenes)))
+
+    def statistics(self, loading=1e-6, stream=None):
+        self._check(self._lib.hsi_batch_statistics(self.handle, loading, _stream(stream)))
+
+    def check(self):
+        idx = C.c_int(-1)
+        self._check(self._lib.hsi_batch_check(self.handle, C.byref(idx)))
+
+    def score(self, stream=None):
+        self._check(self._lib.hsi_batch_score(self.handle, _stream(stream)))
+
+    def context(self, scene):
+        """The scene's context (owned by the batch) as a Context."""
+        h = self._lib.hsi_batch_context(self.handle, scene)
+        if not h:
+            raise IndexError(scene)
+        c = Context.__new__(Context)
+        c._lib = self._lib
+        c.handle = C.c_void_p(h)
+        c._borrowed = True
+        return c
+
+    def autotune(self, stream=None):
+        buf = C.create_string_buffer(8192)
+        self._check(self._lib.hsi_batch_autotune(self.handle, _stream(stream), buf, len(buf)))
+        return buf.value.decode()
+
+
+def demosaic(ctx, raw, raw_type, raw_height, raw_width, period, bands, cube, band_order=None, stream=None):
+    """Snapshot-mosaic frame (device tensor) -> BSQ cube tensor (bands, stride)."""
+    order = None
+    if band_order is not None:
+        order = np.ascontiguousarray(band_order, np.int32)
+    ctx._check(ctx._lib.hsi_demosaic(ctx.handle, _ptr(raw), raw_type, raw_height, raw_width, period, bands,
+                                     None if order is None else order.ctypes.data, _ptr(cube),
+                                     cube.stride(0), _stream(stream)))
 
 
 def mlp_param_count(bands):

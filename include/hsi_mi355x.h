@@ -296,6 +296,62 @@ hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float
                                 int capacity, float* d_score, int32_t* d_target,
                                 float* d_aux, hsi_stream stream);
 
+/* ---- batched scenes ----------------------------------------------------------
+ *
+ * No counterpart in the reference (its pipeline scores one shape at a time).
+ * Scenes of uneven size, band count (<= 256; RX/ACE <= 224) and library
+ * size (1-32 targets) are scored together: each scene keeps its own
+ * context (library, statistics), and SAM/CEM and RX/ACE each run as one
+ * persistent launch per kernel shape class over device tables of the
+ * scenes. hsi_batch_autotune times the candidate launch shapes (pixels per
+ * thread, grid size, tiles per work item) on the batch and keeps the
+ * fastest. Cubes are F32 with a hsi_plane_stride stride; SAM/CEM maps are
+ * 16-byte aligned. A NULL score map skips that detector for the scene; RX
+ * and ACE of a scene share one pass when both are requested. */
+typedef struct hsi_batch hsi_batch;
+
+typedef struct hsi_scene {
+  const float* d_cube;
+  int height, width, bands;
+  size_t plane_stride;
+  const float* library;  /* host [num_targets x bands], NULL if none */
+  int num_targets;
+  float* d_sam_score;
+  int32_t* d_sam_target;
+  float* d_cem_score;
+  int32_t* d_cem_target;
+  float* d_rx_score;
+  float* d_ace_score;
+  int32_t* d_ace_target;
+} hsi_scene;
+
+hsi_status hsi_batch_create(int device, hsi_batch** out);
+void hsi_batch_destroy(hsi_batch* batch);
+const char* hsi_batch_last_error(const hsi_batch* batch);
+
+/* Synchronous: install the scenes (libraries uploaded, device tables built).
+ * The cubes and maps named here must stay valid while the batch is used. */
+hsi_status hsi_batch_set_scenes(hsi_batch* batch, const hsi_scene* scenes, int count);
+
+/* Statistics of every scene that needs them (RX, ACE, CEM). Asynchronous. */
+hsi_status hsi_batch_statistics(hsi_batch* batch, double diagonal_loading, hsi_stream stream);
+
+/* Synchronous: HSI_OK, or the first scene whose factorisation failed. */
+hsi_status hsi_batch_check(hsi_batch* batch, int* failed_scene);
+
+/* All requested detectors of all scenes: one persistent launch per shape
+ * class. Asynchronous. */
+hsi_status hsi_batch_score(hsi_batch* batch, hsi_stream stream);
+
+/* Synchronous: time the candidate launch shapes of every class on the
+ * current scenes (statistics must be built) and keep the fastest; a text
+ * report of the choices is written to `report` (may be NULL). */
+hsi_status hsi_batch_autotune(hsi_batch* batch, hsi_stream stream, char* report, size_t report_len);
+
+/* The context of scene `scene` (for hsi_detect, hsi_stats_read, ...), or
+ * NULL. Owned by the batch. */
+hsi_ctx* hsi_batch_context(hsi_batch* batch, int scene);
+
 /* ---- ingest helpers ---------------------------------------------------------- */
 
 /* FP32 -> FP16 narrowing of `count` elements (round to nearest even).
@@ -308,6 +364,22 @@ hsi_status hsi_narrow_to_half(const float* d_src, void* d_dst, size_t count,
 hsi_status hsi_transpose_bip_to_bsq(const float* d_src, float* d_dst, int height,
                                     int width, int bands, size_t plane_stride,
                                     hsi_stream stream);
+
+/* Sample type of a raw snapshot-mosaic frame. */
+typedef enum hsi_raw_type { HSI_RAW_U8 = 0, HSI_RAW_U16 = 1, HSI_RAW_F32 = 2 } hsi_raw_type;
+
+/* Demosaic a snapshot-mosaic frame (raw_height x raw_width samples, a
+ * period x period filter tile) on the device into a BSQ cube of
+ * (raw_height / period) x (raw_width / period) pixels and `bands` planes:
+ * band b takes the samples at mosaic position band_order[b] (row
+ * position / period, column position % period of each tile), or position b
+ * when band_order is NULL; integer samples are scaled by 1/255 or 1/65535 in
+ * float. Replaces the host demosaic of the hot-folder loader
+ * (hot_loader.cpp:144-182). band_order is host memory, copied; synchronous
+ * with respect to that copy, then asynchronous. */
+hsi_status hsi_demosaic(hsi_ctx* ctx, const void* d_raw, hsi_raw_type raw_type, int raw_height,
+                        int raw_width, int period, int bands, const int32_t* band_order,
+                        float* d_cube, size_t plane_stride, hsi_stream stream);
 
 /* ---- host helpers ------------------------------------------------------------ */
 
