@@ -988,12 +988,62 @@ hipError_t upload_vec(T** dst, const std::vector<T>& v) {
   return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
+// wgs > 0: persistent grid; 0: one workgroup per block or item; -1: one
+// launch of the single-scene kernel per scene (the tuner keeps whichever
+// shape is fastest for the class).
+constexpr int kPerScene = -1;
+
 hipError_t run_sam_class(const hsi_batch::SamClass& c, int ppt, int wgs, hipStream_t s) {
+  if (wgs == kPerScene) {
+    for (const SamSceneDev& d : c.table) {
+      SamCemArgs a;
+      a.cube = d.cube;
+      a.dtype = HSI_F32;
+      a.pixels = d.pixels;
+      a.bands = d.bands;
+      a.stride = d.stride;
+      a.num_targets = d.num_targets;
+      a.lib_stride = c.lib_stride;
+      a.sam_lib = d.sam_lib;
+      a.sam_norms = d.sam_norms;
+      a.sam_rnorms = d.sam_rnorms;
+      a.sam_exact_all = d.exact_all;
+      a.cem_lib = d.cem_lib;
+      a.sam_score = d.sam_score;
+      a.sam_target = d.sam_target;
+      a.cem_score = d.cem_score;
+      a.cem_target = d.cem_target;
+      const hipError_t e = launch_sam_cem(a, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   const int k = ppt == 4 ? 0 : 1;
   return launch_sam_cem_batch(c.lib_stride, ppt, c.d_table[k], c.d_block_scene[k], c.blocks[k], wgs, s);
 }
 
 hipError_t run_rx_class(const hsi_batch::RxClass& c, int chunk, int wgs, hipStream_t s) {
+  if (wgs == kPerScene) {
+    for (const RxSceneDev& d : c.table) {
+      RxAceArgs a;
+      a.cube = d.cube;
+      a.dtype = HSI_F32;
+      a.pixels = d.pixels;
+      a.bands = d.bands;
+      a.stride = d.stride;
+      a.num_targets = d.num_targets;
+      a.mean = d.mean;
+      a.white_blocks = d.white_blocks;
+      a.ace_u = d.ace_u;
+      a.target_norm2 = d.target_norm2;
+      a.rx = d.rx;
+      a.ace_score = d.ace_score;
+      a.ace_target = d.ace_target;
+      const hipError_t e = launch_rxace_plain(a, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   return launch_rxace_batch(c.nb, c.tt, c.d_table, c.d_items[chunk], static_cast<int>(c.items[chunk].size()), wgs, s);
 }
 
@@ -1228,7 +1278,8 @@ hsi_status hsi_batch_autotune(hsi_batch* b, hsi_stream stream, char* report, siz
     float best = -1.f;
     for (int ppt : {4, 2}) {
       if (c.lib_stride > 8 && ppt == 4) continue;
-      for (int wgs : {0, 2048, 8192}) {
+      for (int wgs : {0, 2048, 8192, kPerScene}) {
+        if (wgs == kPerScene && ppt != (c.lib_stride <= 8 ? 4 : 2)) continue;  // the single kernel's own shape
         const float ms = time_ms([&] { return run_sam_class(c, ppt, wgs, s); });
         if (ms >= 0.f && (best < 0.f || ms < best)) {
           best = ms;
@@ -1239,13 +1290,15 @@ hsi_status hsi_batch_autotune(hsi_batch* b, hsi_stream stream, char* report, siz
     }
     text += "sam/cem targets<=" + std::to_string(c.lib_stride) + " scenes=" + std::to_string(c.table.size()) +
             ": ppt " + std::to_string(c.ppt) + ", workgroups " +
-            (c.workgroups > 0 ? std::to_string(c.workgroups) + " persistent" : std::string("one per block")) + ", " +
+            (c.workgroups > 0 ? std::to_string(c.workgroups) + " persistent"
+                              : std::string(c.workgroups == kPerScene ? "per-scene launches" : "one per block")) + ", " +
             std::to_string(best * 1e3f) + " us\n";
   }
   for (auto& c : b->rx) {
     float best = -1.f;
     for (int chunk = 0; chunk < 3; ++chunk)
-      for (int wgs : {0, 256, 512, 1024}) {
+      for (int wgs : {0, 256, 512, 1024, kPerScene}) {
+        if (wgs == kPerScene && chunk != 0) continue;
         const float ms = time_ms([&] { return run_rx_class(c, chunk, wgs, s); });
         if (ms >= 0.f && (best < 0.f || ms < best)) {
           best = ms;
@@ -1255,7 +1308,8 @@ hsi_status hsi_batch_autotune(hsi_batch* b, hsi_stream stream, char* report, siz
       }
     text += "rx/ace bands<=" + std::to_string(16 * c.nb) + " target tiles=" + std::to_string(c.tt) +
             " scenes=" + std::to_string(c.table.size()) + ": tiles/item " + std::to_string(kRxChunks[c.chunk]) +
-            ", workgroups " + (c.workgroups > 0 ? std::to_string(c.workgroups) + " persistent" : std::string("one per item")) +
+            ", workgroups " + (c.workgroups > 0 ? std::to_string(c.workgroups) + " persistent"
+                                               : std::string(c.workgroups == kPerScene ? "per-scene launches" : "one per item")) +
             ", " + std::to_string(best * 1e3f) + " us\n";
   }
   (void)hipEventDestroy(e0);
