@@ -8,6 +8,7 @@ namespace hsi_amd {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 
 // ---- SAM / CEM ---------------------------------------------------------------
 struct SamCemArgs {

@@ -40,44 +40,65 @@ __host__ __device__ constexpr int pair_row(int q) {
 }
 __host__ __device__ constexpr int pair_col(int q) { return q - pair_row(q) * (pair_row(q) + 1) / 2; }
 
-// One MFMA per owned block pair; the pair index is a template parameter so the
-// operand and accumulator indices are compile-time and stay in registers.
-// Pair q belongs to owner q % OWNERS (an owner is one wave of one of the
-// workgroups that share a pixel range).
-template <int Q, int NP, int NB, int OWNERS>
-__device__ __forceinline__ void mfma_pairs_f64(int owner, const double (&op)[NB],
-                                               d4 (&acc)[(NP + OWNERS - 1) / OWNERS]) {
-  if constexpr (Q < NP) {
-    if (owner == Q % OWNERS) {
-      acc[Q / OWNERS] = __builtin_amdgcn_mfma_f64_16x16x4f64(op[pair_row(Q)], op[pair_col(Q)],
-                                                            acc[Q / OWNERS], 0, 0, 0);
-    }
-    mfma_pairs_f64<Q + 1, NP, NB, OWNERS>(owner, op, acc);
+// Block pairs are dealt to owners (one wave of one of the workgroups that
+// share a pixel range) in contiguous runs of the row-major pair order, so an
+// owner's pairs touch few distinct 16-band blocks and it loads only those
+// operands. Owner position k holds pairs [first(k), first(k + 1)); the wave at
+// position k is chosen so that the two waves sharing a SIMD (w and w + 4) hold
+// adjacent runs, which balances the matrix-core work per SIMD.
+__host__ __device__ constexpr int run_first(int k, int np, int owners) { return (k * np + owners - 1) / owners; }
+__host__ __device__ constexpr int run_len(int np, int owners) {
+  int m = 0;
+  for (int k = 0; k < owners; ++k) {
+    const int c = run_first(k + 1, np, owners) - run_first(k, np, owners);
+    if (c > m) m = c;
+  }
+  return m;
+}
+__host__ __device__ constexpr int wave_position(int wave) { return 2 * (wave % 4) + wave / 4; }
+
+template <int Q, int NP, int NB, int OWNERS, int POS>
+__device__ __forceinline__ void mfma_run_f64(const double (&op)[NB], d4 (&acc)[run_len(NP, OWNERS)]) {
+  constexpr int F = run_first(POS, NP, OWNERS);
+  if constexpr (Q < run_first(POS + 1, NP, OWNERS)) {
+    constexpr int I = pair_row(Q), J = pair_col(Q);
+    acc[Q - F] = __builtin_amdgcn_mfma_f64_16x16x4f64(op[I], op[J], acc[Q - F], 0, 0, 0);
+    mfma_run_f64<Q + 1, NP, NB, OWNERS, POS>(op, acc);
   }
 }
 
-template <int Q, int NP, int NB, int OWNERS>
-__device__ __forceinline__ void mfma_pairs_f16(int owner, const h8 (&hi)[NB], const h8 (&lo)[NB],
-                                               f4 (&acc)[(NP + OWNERS - 1) / OWNERS]) {
-  if constexpr (Q < NP) {
-    if (owner == Q % OWNERS) {
-      constexpr int I = pair_row(Q), J = pair_col(Q), M = Q / OWNERS;
-      acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi[I], hi[J], acc[M], 0, 0, 0);
-      acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi[I], lo[J], acc[M], 0, 0, 0);
-      acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo[I], hi[J], acc[M], 0, 0, 0);
-    }
-    mfma_pairs_f16<Q + 1, NP, NB, OWNERS>(owner, hi, lo, acc);
+template <int Q, int NP, int NB, int OWNERS, int POS>
+__device__ __forceinline__ void mfma_run_f16(const h8 (&hi)[NB], const h8 (&lo)[NB],
+                                             f4 (&acc)[run_len(NP, OWNERS)]) {
+  constexpr int F = run_first(POS, NP, OWNERS);
+  if constexpr (Q < run_first(POS + 1, NP, OWNERS)) {
+    constexpr int I = pair_row(Q), J = pair_col(Q), M = Q - F;
+    acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi[I], hi[J], acc[M], 0, 0, 0);
+    acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi[I], lo[J], acc[M], 0, 0, 0);
+    acc[M] = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo[I], hi[J], acc[M], 0, 0, 0);
+    mfma_run_f16<Q + 1, NP, NB, OWNERS, POS>(hi, lo, acc);
   }
+}
+
+// Bit b set: block b is an operand of some pair in owner POS's run. Used
+// through a constexpr variable so it is folded at compile time.
+template <int NP, int OWNERS, int POS>
+__host__ __device__ constexpr unsigned long long run_blocks() {
+  unsigned long long m = 0;
+  for (int q = run_first(POS, NP, OWNERS); q < run_first(POS + 1, NP, OWNERS); ++q)
+    m |= (1ull << pair_row(q)) | (1ull << pair_col(q));
+  return m;
 }
 
 // float4 of band b at pixel px (px a multiple of 4) for the staging pass: the
 // cube values, the ones band, or zeros past the band count / pixel range.
 __device__ __forceinline__ float4 stage_load(const float* __restrict__ cube, std::size_t stride,
-                                             int bands, int b, std::size_t px, std::size_t p_end) {
+                                             int bands, int b, std::size_t px, std::size_t p_end,
+                                             bool ones_band = true) {
   float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
   if (px < p_end) {
     if (b < bands) v = *reinterpret_cast<const float4*>(cube + static_cast<std::size_t>(b) * stride + px);
-    else if (b == bands) v = make_float4(1.f, 1.f, 1.f, 1.f);
+    else if (ones_band && b == bands) v = make_float4(1.f, 1.f, 1.f, 1.f);
     if (px + 1 >= p_end) v.y = 0.f;
     if (px + 2 >= p_end) v.z = 0.f;
     if (px + 3 >= p_end) v.w = 0.f;
@@ -94,34 +115,58 @@ __device__ __forceinline__ float4 stage_load(const float* __restrict__ cube, std
 constexpr int kWavesPerWg = 8;
 constexpr int kXcds = 8;
 template <int NB> constexpr int f64_chunk() { return NB <= 8 ? 64 : 32; }
-template <int NB> constexpr int f16_chunk() { return 64; }
+template <int NB> constexpr int f16_chunk() { return NB <= 8 ? 128 : 64; }
 template <int NB> constexpr int f16_splits() { return NB > 12 ? 2 : 1; }
 
-template <int NB>
-__global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const float* __restrict__ cube,
-                                                                      std::size_t pixels, int bands,
-                                                                      std::size_t stride,
-                                                                      std::size_t px_per_wg,
-                                                                      double* __restrict__ partial) {
-  constexpr int CH = f64_chunk<NB>();
+// Plain-form split of the work. The matrix cores take the block pairs of the
+// first NBM 16-band blocks; a last block that would hold only one or two real
+// bands (113 = 7 x 16 + 1) is left to the vector ALUs instead of spending a
+// whole 16-row block pair per column block on it, and so are the sums: lane j
+// (j < bands) adds x_j and, per leftover band r, x_r * x_j over the
+// workgroup's pixels, reading the staged tile like the matrix-core operands.
+// The pixel count needs no accumulation.
+constexpr int kMaxVecRows = 2;
+
+struct F64Split {
+  int nbm;   // 16-band blocks on the matrix cores
+  int rows;  // leftover bands on the vector ALUs (0..kMaxVecRows)
+};
+__host__ __device__ inline F64Split f64_split(int bands) {
+  const int full = bands / 16, rem = bands % 16;
+  if (rem == 0) return {full, 0};
+  if (rem <= kMaxVecRows && full > 0) return {full, rem};
+  return {full + 1, 0};
+}
+// Per-workgroup partial: the block pairs, then the vector part ([1 + rows]
+// rows of vc = 16 * (NBM + 1) doubles: sums, then one row per leftover band).
+__host__ __device__ inline std::size_t f64_partial_doubles(int nbm) {
+  return static_cast<std::size_t>(nbm) * (nbm + 1) / 2 * 256 + (1 + kMaxVecRows) * 16 * (nbm + 1);
+}
+
+template <int NBM, int W, int POS>
+__device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube, std::size_t p_begin,
+                                                 std::size_t p_end, int bands, std::size_t stride,
+                                                 int vec_rows, double* tiles, double* __restrict__ out) {
+  constexpr int NBS = NBM + 1;                        // staged blocks (leftover bands in the last)
+  constexpr int CH = f64_chunk<NBS>();
   constexpr int THREADS = kWavesPerWg * 64;
-  constexpr int NP = NB * (NB + 1) / 2;
-  constexpr int PW = (NP + kWavesPerWg - 1) / kWavesPerWg;
-  constexpr int LDB = NB * 16 + 1;                    // odd pitch spreads LDS banks
-  constexpr int LOADS = NB * 16 * (CH / 4);           // float4 per chunk
+  constexpr int NP = NBM * (NBM + 1) / 2;
+  constexpr int PW = run_len(NP, kWavesPerWg);
+  constexpr int LDB = NBS * 16 + 1;                   // odd pitch spreads LDS banks
+  constexpr int LOADS = NBS * 16 * (CH / 4);          // float4 per chunk
   constexpr int PER = (LOADS + THREADS - 1) / THREADS;
-  __shared__ double tile[2][CH * LDB];
+  constexpr int VC = 16 * NBS;
+  constexpr int TILE = CH * LDB;
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
   d4 acc[PW];
 #pragma unroll
   for (int m = 0; m < PW; ++m) acc[m] = d4{0.0, 0.0, 0.0, 0.0};
-
-  const std::size_t p_begin = blockIdx.x * px_per_wg;
-  std::size_t p_end = p_begin + px_per_wg;
-  if (p_end > pixels) p_end = pixels;
-  if (p_begin >= p_end) return;
+  // Vector part: wave-uniform choice of the waves that own columns.
+  const bool vec_wave = W * 64 < bands;
+  const int vj = W * 64 + lane < bands ? W * 64 + lane : bands - 1;
+  double vsum = 0.0;
+  double vrow[kMaxVecRows] = {0.0, 0.0};
   const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
 
   float4 pre[PER];
@@ -130,7 +175,7 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const flo
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + THREADS * k;
       const int b = e / (CH / 4);
-      pre[k] = e < LOADS ? stage_load(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_end)
+      pre[k] = e < LOADS ? stage_load(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_end, false)
                          : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   };
@@ -150,32 +195,50 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const flo
   };
 
   load(p_begin);
-  store(tile[0]);
+  store(tiles);
   __syncthreads();
   for (int c = 0; c < chunks; ++c) {
     const bool more = c + 1 < chunks;
     if (more) load(p_begin + static_cast<std::size_t>(c + 1) * CH);
-    const double* t = tile[c & 1];
+    const double* t = tiles + (c & 1) * TILE;
 #pragma unroll 2
     for (int s = 0; s < CH / 4; ++s) {
       // v_mfma_f64_16x16x4 operands: lane l supplies row (l % 16) of the
       // 16-band block at pixel (l / 16) of this 4-pixel step, for A and B.
-      double op[NB];
+      double op[NBM];
       const double* row = t + (4 * s + (lane >> 4)) * LDB + (lane & 15);
+      constexpr unsigned long long used = run_blocks<NP, kWavesPerWg, POS>();
 #pragma unroll
-      for (int i = 0; i < NB; ++i) op[i] = row[16 * i];
-      mfma_pairs_f64<0, NP, NB, kWavesPerWg>(wave, op, acc);
+      for (int i = 0; i < NBM; ++i) op[i] = (used >> i) & 1 ? row[16 * i] : 0.0;
+      mfma_run_f64<run_first(POS, NP, kWavesPerWg), NP, NBM, kWavesPerWg, POS>(op, acc);
     }
-    if (more) store(tile[(c + 1) & 1]);
+    // The matrix cores drain this wave's last pairs (and serve the other
+    // wave of the SIMD) while the vector part runs.
+    if (vec_wave) {
+      if (vec_rows == 0) {
+#pragma unroll 8
+        for (int p = 0; p < CH; ++p) vsum += t[p * LDB + vj];
+      } else {
+#pragma unroll 4
+        for (int p = 0; p < CH; ++p) {
+          const double* px = t + p * LDB;
+          const double xj = px[vj];
+          vsum += xj;
+          vrow[0] = __builtin_fma(px[16 * NBM], xj, vrow[0]);
+          vrow[1] = __builtin_fma(px[16 * NBM + 1], xj, vrow[1]);   // zero row when vec_rows == 1
+        }
+      }
+    }
+    if (more) store(tiles + ((c + 1) & 1) * TILE);
     __syncthreads();
   }
 
-  // Partial layout: [wg][pair][lane][4], the accumulator registers verbatim.
-  double* out = partial + static_cast<std::size_t>(blockIdx.x) * NP * 256;
+  // Partial layout: [wg][pair][lane][4], the accumulator registers verbatim,
+  // then the vector part.
 #pragma unroll
   for (int m = 0; m < PW; ++m) {
-    const int q = wave + kWavesPerWg * m;
-    if (q < NP) {
+    const int q = run_first(POS, NP, kWavesPerWg) + m;
+    if (q < run_first(POS + 1, NP, kWavesPerWg)) {
       double* dst = out + (static_cast<std::size_t>(q) * 64 + lane) * 4;
       dst[0] = acc[m][0];
       dst[1] = acc[m][1];
@@ -183,41 +246,62 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const flo
       dst[3] = acc[m][3];
     }
   }
+  if (vec_wave && W * 64 + lane < bands) {
+    double* v = out + NP * 256;
+    v[vj] = vsum;
+#pragma unroll
+    for (int r = 0; r < kMaxVecRows; ++r)
+      if (r < vec_rows) v[(1 + r) * VC + vj] = vrow[r];
+  }
+}
+
+template <int NBM>
+__global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const float* __restrict__ cube,
+                                                                      std::size_t pixels, int bands,
+                                                                      std::size_t stride,
+                                                                      std::size_t px_per_wg, int vec_rows,
+                                                                      double* __restrict__ partial) {
+  constexpr int CH = f64_chunk<NBM + 1>();
+  constexpr int LDB = (NBM + 1) * 16 + 1;
+  __shared__ double tiles[2 * CH * LDB];
+  const std::size_t p_begin = blockIdx.x * px_per_wg;
+  std::size_t p_end = p_begin + px_per_wg;
+  if (p_end > pixels) p_end = pixels;
+  if (p_begin >= p_end) return;
+  double* out = partial + static_cast<std::size_t>(blockIdx.x) * f64_partial_doubles(NBM);
+  switch (threadIdx.x >> 6) {
+#define HSI_W(W)                                                                                   \
+  case W:                                                                                          \
+    moments_f64_wave<NBM, W, wave_position(W)>(cube, p_begin, p_end, bands, stride, vec_rows, tiles, \
+                                               out);                                             \
+    break;
+    HSI_W(0) HSI_W(1) HSI_W(2) HSI_W(3) HSI_W(4) HSI_W(5) HSI_W(6) HSI_W(7)
+#undef HSI_W
+  }
 }
 
 // Mixed form: d = x - shift as FP16 hi + lo in LDS, [band][pixel] with a
 // pitch of CH + 8 halves (16 lanes reading 16 bytes each at rows m = 0..15
-// land on 16 disjoint bank quads).
-template <int NB>
-__global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const float* __restrict__ cube,
-                                                                        std::size_t pixels, int bands,
-                                                                        std::size_t stride,
-                                                                        std::size_t px_per_wg,
-                                                                        const float* __restrict__ shift,
-                                                                        double* __restrict__ partial) {
+// land on 16 disjoint bank quads). Each owner position is its own code path,
+// holding only its run's accumulators and operand blocks.
+template <int NB, int POS>
+__device__ __forceinline__ void moments_mixed_owner(const float* __restrict__ cube, std::size_t p_begin,
+                                                    std::size_t p_end, int bands, std::size_t stride,
+                                                    const float* sh, _Float16* hi_t, _Float16* lo_t,
+                                                    double* __restrict__ out) {
   constexpr int CH = f16_chunk<NB>();
   constexpr int NP = NB * (NB + 1) / 2;
-  constexpr int SPLITS = f16_splits<NB>();
-  constexpr int OWNERS = kWavesPerWg * SPLITS;
+  constexpr int OWNERS = kWavesPerWg * f16_splits<NB>();
   constexpr int THREADS = kWavesPerWg * 64;
-  constexpr int PW = (NP + OWNERS - 1) / OWNERS;
+  constexpr int PW = run_len(NP, OWNERS);
   constexpr int PITCH = CH + 8;
   constexpr int ROWS = NB * 16;
   constexpr int LOADS = ROWS * (CH / 4);
   constexpr int PER = (LOADS + THREADS - 1) / THREADS;
   constexpr int KSTEPS = CH / 32;
-  __shared__ _Float16 hi_t[2][ROWS * PITCH];
-  __shared__ _Float16 lo_t[2][ROWS * PITCH];
-  __shared__ float sh[ROWS];
-
-  for (int b = threadIdx.x; b < ROWS; b += THREADS) sh[b] = b < bands ? shift[b] : 0.f;
-  __syncthreads();
+  constexpr int TILE = ROWS * PITCH;
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  // Workgroup j: pixel range (j / (8 * SPLITS)) * 8 + j % 8, pair half (j / 8) % SPLITS.
-  const unsigned range = (blockIdx.x / (kXcds * SPLITS)) * kXcds + blockIdx.x % kXcds;
-  const int owner = static_cast<int>((blockIdx.x / kXcds) % SPLITS) * kWavesPerWg + wave;
   f4 acc[PW];
   d4 tot[PW];
 #pragma unroll
@@ -225,11 +309,6 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const f
     acc[m] = f4{0.f, 0.f, 0.f, 0.f};
     tot[m] = d4{0.0, 0.0, 0.0, 0.0};
   }
-
-  const std::size_t p_begin = range * px_per_wg;
-  std::size_t p_end = p_begin + px_per_wg;
-  if (p_end > pixels) p_end = pixels;
-  if (p_begin >= p_end) return;
   const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
 
   float4 pre[PER];
@@ -253,13 +332,16 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const f
         // Shift real bands only, and only real pixels (padding stays zero).
         const float s = b < bands ? sh[b] : 0.f;
         float v[4] = {pre[k].x, pre[k].y, pre[k].z, pre[k].w};
+        h4 hv, lv;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const float d = (px + i < p_end) ? v[i] - s : 0.f;
           const _Float16 h = static_cast<_Float16>(d);
-          hi_t[buf][b * PITCH + k4 + i] = h;
-          lo_t[buf][b * PITCH + k4 + i] = static_cast<_Float16>(d - static_cast<float>(h));
+          hv[i] = h;
+          lv[i] = static_cast<_Float16>(d - static_cast<float>(h));
         }
+        *reinterpret_cast<h4*>(hi_t + buf * TILE + b * PITCH + k4) = hv;
+        *reinterpret_cast<h4*>(lo_t + buf * TILE + b * PITCH + k4) = lv;
       }
     }
   };
@@ -278,13 +360,19 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const f
       // row m of every 16-band block (A and B alike, so the pixel order
       // cancels out).
       h8 hi[NB], lo[NB];
+      constexpr unsigned long long used = run_blocks<NP, OWNERS, POS>();
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
-        const int off = (16 * i + (lane & 15)) * PITCH + 32 * ks + 8 * (lane >> 4);
-        hi[i] = *reinterpret_cast<const h8*>(&hi_t[buf][off]);
-        lo[i] = *reinterpret_cast<const h8*>(&lo_t[buf][off]);
+        const int off = buf * TILE + (16 * i + (lane & 15)) * PITCH + 32 * ks + 8 * (lane >> 4);
+        if ((used >> i) & 1) {
+          hi[i] = *reinterpret_cast<const h8*>(hi_t + off);
+          lo[i] = *reinterpret_cast<const h8*>(lo_t + off);
+        } else {
+          hi[i] = h8{};
+          lo[i] = h8{};
+        }
       }
-      mfma_pairs_f16<0, NP, NB, OWNERS>(owner, hi, lo, acc);
+      mfma_run_f16<run_first(POS, NP, OWNERS), NP, NB, OWNERS, POS>(hi, lo, acc);
     }
     // FP32 tiles of 128 pixels, then FP64.
     if ((c + 1) % (128 / CH) == 0 || !more) {
@@ -299,17 +387,57 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const f
     __syncthreads();
   }
 
-  double* out = partial + static_cast<std::size_t>(range) * NP * 256;
 #pragma unroll
   for (int m = 0; m < PW; ++m) {
-    const int q = owner + OWNERS * m;
-    if (q < NP) {
+    const int q = run_first(POS, NP, OWNERS) + m;
+    if (q < run_first(POS + 1, NP, OWNERS)) {
       double* dst = out + (static_cast<std::size_t>(q) * 64 + lane) * 4;
       dst[0] = tot[m][0];
       dst[1] = tot[m][1];
       dst[2] = tot[m][2];
       dst[3] = tot[m][3];
     }
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const float* __restrict__ cube,
+                                                                        std::size_t pixels, int bands,
+                                                                        std::size_t stride,
+                                                                        std::size_t px_per_wg,
+                                                                        const float* __restrict__ shift,
+                                                                        double* __restrict__ partial) {
+  constexpr int CH = f16_chunk<NB>();
+  constexpr int NP = NB * (NB + 1) / 2;
+  constexpr int SPLITS = f16_splits<NB>();
+  constexpr int THREADS = kWavesPerWg * 64;
+  constexpr int PITCH = CH + 8;
+  constexpr int ROWS = NB * 16;
+  __shared__ _Float16 hi_t[2 * ROWS * PITCH];
+  __shared__ _Float16 lo_t[2 * ROWS * PITCH];
+  __shared__ float sh[ROWS];
+
+  for (int b = threadIdx.x; b < ROWS; b += THREADS) sh[b] = b < bands ? shift[b] : 0.f;
+  __syncthreads();
+
+  // Workgroup j: pixel range (j / (8 * SPLITS)) * 8 + j % 8, pair half (j / 8) % SPLITS.
+  const unsigned range = (blockIdx.x / (kXcds * SPLITS)) * kXcds + blockIdx.x % kXcds;
+  const int split = static_cast<int>((blockIdx.x / kXcds) % SPLITS);
+  const std::size_t p_begin = range * px_per_wg;
+  std::size_t p_end = p_begin + px_per_wg;
+  if (p_end > pixels) p_end = pixels;
+  if (p_begin >= p_end) return;
+  double* out = partial + static_cast<std::size_t>(range) * NP * 256;
+  const int pos = split * kWavesPerWg + wave_position(threadIdx.x >> 6);
+  switch (pos) {
+#define HSI_POS(K)                                                                                 \
+  case K:                                                                                          \
+    if constexpr (K < kWavesPerWg * SPLITS)                                                        \
+      moments_mixed_owner<NB, K>(cube, p_begin, p_end, bands, stride, sh, hi_t, lo_t, out);        \
+    break;
+    HSI_POS(0) HSI_POS(1) HSI_POS(2) HSI_POS(3) HSI_POS(4) HSI_POS(5) HSI_POS(6) HSI_POS(7)
+    HSI_POS(8) HSI_POS(9) HSI_POS(10) HSI_POS(11) HSI_POS(12) HSI_POS(13) HSI_POS(14) HSI_POS(15)
+#undef HSI_POS
   }
 }
 
@@ -321,35 +449,68 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const f
 // workgroup count. v_mfma_f64_16x16x4 leaves (row l/16 + 4r, column l%16) of
 // a block in lane l, register r (rows interleaved); v_mfma_f32_16x16x32_f16
 // leaves row 4*(l/16) + r (blocked).
+//
+// Mixed form (ones band inside the blocks): row == bands carries the sums and
+// the count. Plain form: after the nbm*(nbm+1)/2 block pairs come the vector
+// part's rows (sums, then the leftover bands 16*nbm + r), and the count is
+// the pixel count.
 constexpr int kRedElems = 16;
 constexpr int kRedGroups = 16;
 
-__global__ __launch_bounds__(kRedElems * kRedGroups) void moments_reduce_kernel(
-    const double* __restrict__ partial, int wgs, int nb, int bands, int interleaved,
-    double* __restrict__ count, double* __restrict__ sum, double* __restrict__ sum_outer) {
+struct ReduceArgs {
+  const double* partial;
+  int wgs;
+  std::size_t per_wg;   // doubles per workgroup partial
+  int nb;               // blocks on the matrix cores
+  int bands;
+  int interleaved;      // 1: f64 matrix-core layout, 0: f16 layout
+  int ones_band;        // 1: mixed form (ones band in the blocks)
+  int vec_rows;         // plain form: leftover bands after the blocks
+  int vc;               // plain form: pitch of the vector part
+  double pixels;        // plain form: the count
+  double* count;
+  double* sum;
+  double* sum_outer;
+};
+
+__global__ __launch_bounds__(kRedElems * kRedGroups) void moments_reduce_kernel(ReduceArgs a) {
   __shared__ double part[kRedGroups][kRedElems];
-  const int np = nb * (nb + 1) / 2;
+  const std::size_t np = static_cast<std::size_t>(a.nb) * (a.nb + 1) / 2;
   const int local = threadIdx.x % kRedElems;
   const int group = threadIdx.x / kRedElems;
-  const int e = blockIdx.x * kRedElems + local;
+  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * kRedElems + local;
   double s = 0.0;
-  if (e < np * 256)
-    for (int w = group; w < wgs; w += kRedGroups) s += partial[static_cast<std::size_t>(w) * np * 256 + e];
+  if (e < a.per_wg)
+    for (int w = group; w < a.wgs; w += kRedGroups) s += a.partial[static_cast<std::size_t>(w) * a.per_wg + e];
   part[group][local] = s;
   __syncthreads();
-  if (group != 0 || e >= np * 256) return;
+  if (group != 0 || e >= a.per_wg) return;
   double total = 0.0;
   for (int g = 0; g < kRedGroups; ++g) total += part[g][local];
-  const int q = e / 256;
-  const int lane = (e % 256) / 4;
-  const int r = e % 4;
-  const int row = 16 * pair_row(q) + (interleaved ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
+  const int bands = a.bands;
+  if (!a.ones_band && e == 0) *a.count = a.pixels;
+  if (e >= np * 256) {                    // plain form, vector part
+    const int idx = static_cast<int>(e - np * 256);
+    const int which = idx / a.vc, col = idx % a.vc;
+    if (col >= bands || which > a.vec_rows) return;
+    if (which == 0) {
+      a.sum[col] = total;
+    } else {
+      const int row = 16 * a.nb + which - 1;
+      if (col <= row) a.sum_outer[static_cast<std::size_t>(row) * bands + col] = total;
+    }
+    return;
+  }
+  const int q = static_cast<int>(e / 256);
+  const int lane = static_cast<int>(e % 256) / 4;
+  const int r = static_cast<int>(e % 4);
+  const int row = 16 * pair_row(q) + (a.interleaved ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
   const int col = 16 * pair_col(q) + (lane & 15);
   if (row < bands) {
-    if (col <= row) sum_outer[static_cast<std::size_t>(row) * bands + col] = total;
-  } else if (row == bands) {
-    if (col < bands) sum[col] = total;
-    else if (col == bands) *count = total;
+    if (col <= row) a.sum_outer[static_cast<std::size_t>(row) * bands + col] = total;
+  } else if (a.ones_band && row == bands) {
+    if (col < bands) a.sum[col] = total;
+    else if (col == bands) *a.count = total;
   }
 }
 
@@ -419,9 +580,9 @@ std::size_t mom_wgs(std::size_t pixels) {
 }
 
 template <int NB>
-hipError_t launch_moments_nb(const float* cube, std::size_t pixels, int bands, std::size_t stride,
-                             hsi_precision precision, double* scratch, double* count, double* sum,
-                             double* sum_outer, hipStream_t stream) {
+hipError_t launch_mixed_nb(const float* cube, std::size_t pixels, int bands, std::size_t stride,
+                           double* scratch, double* count, double* sum, double* sum_outer,
+                           hipStream_t stream) {
   const std::size_t chunks = div_up(pixels, kChunk);
   const std::size_t wgs = mom_wgs(pixels);
   const std::size_t px_per_wg = div_up(chunks, wgs) * kChunk;
@@ -429,58 +590,90 @@ hipError_t launch_moments_nb(const float* cube, std::size_t pixels, int bands, s
   const int np = NB * (NB + 1) / 2;
   double* partial = scratch;
   hipError_t err;
-  if (precision == HSI_MIXED) {
-    // Shift vectors live after the partials.
-    double* shift_d = scratch + wgs * np * 256;
-    float* shift = reinterpret_cast<float*>(shift_d + kMaxBands);
-    const std::size_t run_stride = pixels > 64 * 64 ? div_up(pixels, 64) : 64;
-    hipLaunchKernelGGL(shift_kernel, dim3(bands), dim3(256), 0, stream, cube, pixels, stride,
-                       run_stride, shift, shift_d);
-    if ((err = hipGetLastError()) != hipSuccess) return err;
-    // Ranges rounded up to whole groups of 8 so the range/split decode is
-    // exact; ranges past the pixel count return at once and write nothing.
-    const std::size_t ranges = f16_splits<NB>() > 1 ? round_up(used, kXcds) : used;
-    hipLaunchKernelGGL(moments_mixed_kernel<NB>, dim3(static_cast<unsigned>(ranges * f16_splits<NB>())),
-                       dim3(kWavesPerWg * 64), 0,
-                       stream, cube, pixels, bands, stride, px_per_wg, shift, partial);
-    if ((err = hipGetLastError()) != hipSuccess) return err;
-    hipLaunchKernelGGL(moments_reduce_kernel, dim3(static_cast<unsigned>(np * 256 / kRedElems)),
-                       dim3(kRedElems * kRedGroups), 0, stream, partial, static_cast<int>(used), NB,
-                       bands, 0, count, sum, sum_outer);
-    if ((err = hipGetLastError()) != hipSuccess) return err;
-    const int tri = bands * (bands + 1) / 2;
-    hipLaunchKernelGGL(unshift_kernel, dim3(static_cast<unsigned>(div_up(tri, 256))), dim3(256), 0,
-                       stream, bands, shift_d, count, sum, sum_outer);
-    if ((err = hipGetLastError()) != hipSuccess) return err;
-    hipLaunchKernelGGL(unshift_sum_kernel, dim3(static_cast<unsigned>(div_up(bands, 256))), dim3(256),
-                       0, stream, bands, shift_d, count, sum);
-    return hipGetLastError();
-  }
-  hipLaunchKernelGGL(moments_f64_kernel<NB>, dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64), 0,
-                     stream,
-                     cube, pixels, bands, stride, px_per_wg, partial);
+  // Shift vectors live after the partials.
+  double* shift_d = scratch + wgs * np * 256;
+  float* shift = reinterpret_cast<float*>(shift_d + kMaxBands);
+  const std::size_t run_stride = pixels > 64 * 64 ? div_up(pixels, 64) : 64;
+  hipLaunchKernelGGL(shift_kernel, dim3(bands), dim3(256), 0, stream, cube, pixels, stride,
+                     run_stride, shift, shift_d);
   if ((err = hipGetLastError()) != hipSuccess) return err;
+  // Ranges rounded up to whole groups of 8 so the range/split decode is
+  // exact; ranges past the pixel count return at once and write nothing.
+  const std::size_t ranges = f16_splits<NB>() > 1 ? round_up(used, kXcds) : used;
+  hipLaunchKernelGGL(moments_mixed_kernel<NB>, dim3(static_cast<unsigned>(ranges * f16_splits<NB>())),
+                     dim3(kWavesPerWg * 64), 0,
+                     stream, cube, pixels, bands, stride, px_per_wg, shift, partial);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  ReduceArgs ra{partial, static_cast<int>(used), static_cast<std::size_t>(np) * 256, NB, bands, 0, 1,
+                0, 0, 0.0, count, sum, sum_outer};
   hipLaunchKernelGGL(moments_reduce_kernel, dim3(static_cast<unsigned>(np * 256 / kRedElems)),
-                     dim3(kRedElems * kRedGroups), 0, stream, partial, static_cast<int>(used), NB,
-                     bands, 1, count, sum, sum_outer);
+                     dim3(kRedElems * kRedGroups), 0, stream, ra);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  const int tri = bands * (bands + 1) / 2;
+  hipLaunchKernelGGL(unshift_kernel, dim3(static_cast<unsigned>(div_up(tri, 256))), dim3(256), 0,
+                     stream, bands, shift_d, count, sum, sum_outer);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  hipLaunchKernelGGL(unshift_sum_kernel, dim3(static_cast<unsigned>(div_up(bands, 256))), dim3(256),
+                     0, stream, bands, shift_d, count, sum);
   return hipGetLastError();
+}
+
+template <int NBM>
+hipError_t launch_f64_nbm(const float* cube, std::size_t pixels, int bands, std::size_t stride,
+                          int vec_rows, double* partial, double* count, double* sum, double* sum_outer,
+                          hipStream_t stream) {
+  const std::size_t chunks = div_up(pixels, kChunk);
+  const std::size_t wgs = mom_wgs(pixels);
+  const std::size_t px_per_wg = div_up(chunks, wgs) * kChunk;
+  const std::size_t used = div_up(pixels, px_per_wg);
+  hipLaunchKernelGGL(moments_f64_kernel<NBM>, dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64), 0,
+                     stream, cube, pixels, bands, stride, px_per_wg, vec_rows, partial);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  const std::size_t per_wg = f64_partial_doubles(NBM);
+  ReduceArgs ra{partial, static_cast<int>(used), per_wg, NBM, bands, 1, 0, vec_rows, 16 * (NBM + 1),
+                static_cast<double>(pixels), count, sum, sum_outer};
+  hipLaunchKernelGGL(moments_reduce_kernel, dim3(static_cast<unsigned>(div_up(per_wg, kRedElems))),
+                     dim3(kRedElems * kRedGroups), 0, stream, ra);
+  return hipGetLastError();
+}
+
+hipError_t launch_moments_f64(const float* cube, std::size_t pixels, int bands, std::size_t stride,
+                              double* scratch, double* count, double* sum, double* sum_outer,
+                              hipStream_t stream) {
+  const F64Split sp = f64_split(bands);
+  switch (sp.nbm) {
+#define HSI_NBM(N)                                                                                 \
+  case N:                                                                                          \
+    return launch_f64_nbm<N>(cube, pixels, bands, stride, sp.rows, scratch, count, sum, sum_outer, \
+                             stream);
+    HSI_NBM(1) HSI_NBM(2) HSI_NBM(3) HSI_NBM(4) HSI_NBM(5) HSI_NBM(6) HSI_NBM(7) HSI_NBM(8)
+    HSI_NBM(9) HSI_NBM(10) HSI_NBM(11) HSI_NBM(12) HSI_NBM(13) HSI_NBM(14) HSI_NBM(15) HSI_NBM(16)
+#undef HSI_NBM
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 std::size_t moments_scratch_doubles(std::size_t pixels, int bands) {
-  // partials, then a double and a float shift vector for the mixed form
-  return mom_wgs(pixels) * static_cast<std::size_t>(moment_pairs(bands)) * 256 + 2 * kMaxBands;
+  // mixed form: partials, then a double and a float shift vector; plain
+  // form: partials of block pairs and the vector part
+  const std::size_t mixed = mom_wgs(pixels) * static_cast<std::size_t>(moment_pairs(bands)) * 256 + 2 * kMaxBands;
+  const std::size_t plain = mom_wgs(pixels) * f64_partial_doubles(f64_split(bands).nbm);
+  return mixed > plain ? mixed : plain;
 }
 
 hipError_t launch_moments(const float* cube, std::size_t pixels, int bands, std::size_t stride,
                           hsi_precision precision, double* scratch, double* count, double* sum,
                           double* sum_outer, hipStream_t stream) {
+  if (precision == HSI_PLAIN)
+    return launch_moments_f64(cube, pixels, bands, stride, scratch, count, sum, sum_outer, stream);
+  if (precision != HSI_MIXED) return hipErrorInvalidValue;
   switch (moment_blocks(bands)) {
 #define HSI_NB(N)                                                                                  \
   case N:                                                                                          \
-    return launch_moments_nb<N>(cube, pixels, bands, stride, precision, scratch, count, sum,       \
-                                sum_outer, stream);
+    return launch_mixed_nb<N>(cube, pixels, bands, stride, scratch, count, sum, sum_outer, stream);
     HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8) HSI_NB(9)
     HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14) HSI_NB(15) HSI_NB(16) HSI_NB(17)
 #undef HSI_NB

@@ -360,6 +360,16 @@ class Context:
             "sum_outer")]))
         return out
 
+    def moments_read(self, bands):
+        """Raw FP64 moments of the last accumulation: count, sum[B] and
+        sum(x x^T) as [B][B] (lower triangle valid)."""
+        out = {"count": np.zeros(1, np.float64), "sum": np.zeros(bands, np.float64),
+               "sum_outer": np.zeros((bands, bands), np.float64)}
+        self._check(self._lib.hsi_stats_read(self.handle, None, None, None, None, None,
+                                             out["count"].ctypes.data, out["sum"].ctypes.data,
+                                             out["sum_outer"].ctypes.data))
+        return out
+
     def stats_upload(self, bands, mean, whitener, target_white=None, target_norm2=None,
                      cem_weight=None):
         def p(a):
