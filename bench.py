@@ -312,6 +312,7 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
     # timing on the last held-out frame (statistics are built for it)
     ms = {"plain_score": time_on_stream(torch, lambda: ps.plain(cube, h, w), repeats),
           "learned_score": time_on_stream(torch, lambda: ps.screen(cube, h, w), repeats),
+          "learned_score_two_pass": time_on_stream(torch, lambda: ps.screen_two_pass(cube, h, w), repeats),
           "stats_plain": time_on_stream(torch, lambda: (ctx.stats_accumulate(cube, h, w, b), ctx.stats_build(LOADING)), repeats),
           "stats_mixed": time_on_stream(torch, lambda: (ctx.stats_accumulate(cube, h, w, b, precision=H.MIXED), ctx.stats_build(LOADING)), repeats)}
     m = ps._buffers(h * w)

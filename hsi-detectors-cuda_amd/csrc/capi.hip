@@ -64,6 +64,13 @@ struct hsi_ctx {
   float* d_mlp_tail = nullptr;
   float* d_mlp_act = nullptr;
   std::size_t mlp_act_bytes = 0;
+  // Screening scratch: per-wave regions of records and spectra, counters.
+  void* d_screen_rec = nullptr;
+  std::size_t screen_rec_bytes = 0;
+  void* d_screen_spec = nullptr;
+  std::size_t screen_spec_bytes = 0;
+  void* d_screen_counts = nullptr;
+  std::size_t screen_counts_bytes = 0;
 
   // Demosaic band positions (uploaded when they change).
   int* d_positions = nullptr;
@@ -171,7 +178,7 @@ void hsi_ctx_destroy(hsi_ctx* ctx) {
                   ctx->d_ace_hl, ctx->d_col_scale, ctx->d_row_inv, ctx->d_target_inv,
                   ctx->d_mom_scratch, ctx->d_det_scratch, ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v,
                   ctx->d_mlp_grad, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, ctx->d_mlp_act,
-                  ctx->d_positions};
+                  ctx->d_positions, ctx->d_screen_rec, ctx->d_screen_spec, ctx->d_screen_counts};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete ctx;
@@ -867,10 +874,13 @@ hsi_status hsi_gather_pixels(hsi_ctx* ctx, const float* d_cube, int height, int 
   return HSI_OK;
 }
 
-hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_cube, int height, int width,
-                                int bands, size_t plane_stride, const hsi_detection* d_candidates,
-                                const uint32_t* d_count, int capacity, float* d_score, int32_t* d_target,
-                                float* d_aux, hsi_stream stream) {
+}  // extern "C"
+
+namespace {
+hsi_status score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_cube, int height, int width,
+                            int bands, size_t plane_stride, const hsi_detection* d_candidates,
+                            const uint32_t* d_count, int capacity, const float* d_spectra, size_t spectra_stride,
+                            float* d_score, int32_t* d_target, float* d_aux, hsi_stream stream) {
   if (!ctx) return HSI_ERR_INVALID_ARG;
   if (detector != HSI_DET_RX && detector != HSI_DET_ACE)
     return fail(ctx, HSI_ERR_INVALID_ARG, "candidate rescoring applies to RX and ACE");
@@ -910,8 +920,88 @@ hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float
     a.ace_score = d_score;
     a.ace_target = d_target;
   }
-  HSI_HIP(ctx, launch_rxace_candidates(a, d_candidates, d_count, static_cast<unsigned>(capacity), width, s),
+  HSI_HIP(ctx, launch_rxace_candidates(a, d_candidates, d_count, static_cast<unsigned>(capacity), width,
+                                       d_spectra, spectra_stride, s),
           "rescore candidates");
+  return HSI_OK;
+}
+}  // namespace
+
+extern "C" {
+
+hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_cube, int height, int width,
+                                int bands, size_t plane_stride, const hsi_detection* d_candidates,
+                                const uint32_t* d_count, int capacity, float* d_score, int32_t* d_target,
+                                float* d_aux, hsi_stream stream) {
+  return score_candidates(ctx, detector, d_cube, height, width, bands, plane_stride, d_candidates, d_count,
+                          capacity, nullptr, 0, d_score, d_target, d_aux, stream);
+}
+
+hsi_status hsi_score_candidate_spectra(hsi_ctx* ctx, hsi_detector detector, const float* d_cube, int height,
+                                       int width, int bands, size_t plane_stride,
+                                       const hsi_detection* d_candidates, const uint32_t* d_count, int capacity,
+                                       const float* d_spectra, size_t spectra_stride, float* d_score,
+                                       int32_t* d_target, float* d_aux, hsi_stream stream) {
+  if (!d_spectra || spectra_stride < static_cast<size_t>(bands < 0 ? 0 : bands))
+    return ctx ? fail(ctx, HSI_ERR_INVALID_ARG, "candidate spectra need a buffer with pitch >= bands")
+               : HSI_ERR_INVALID_ARG;
+  return score_candidates(ctx, detector, d_cube, height, width, bands, plane_stride, d_candidates, d_count,
+                          capacity, d_spectra, spectra_stride, d_score, d_target, d_aux, stream);
+}
+
+hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int width, int bands,
+                          size_t plane_stride, float cut, hsi_detection* d_candidates, uint32_t* d_count,
+                          int capacity, float* d_spectra, size_t spectra_stride, float* d_logit,
+                          hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, HSI_F32);
+  if (st != HSI_OK) return st;
+  if ((st = mlp_ready(ctx, bands)) != HSI_OK) return st;
+  if (!d_candidates || !d_count || capacity < 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "screening needs a candidate list, its count and a capacity >= 0");
+  const std::size_t pb = static_cast<std::size_t>((bands + 31) / 32 * 32);
+  if (d_spectra && (spectra_stride < pb || spectra_stride % 4 != 0))
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "candidate spectra need a pitch >= 32 * ceil(bands / 32), a multiple of 4");
+  if (ctx->stats_bands != bands)
+    return fail(ctx, HSI_ERR_NO_STATISTICS, "screening centres on the scene mean: build statistics first");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (ctx->mlp_dirty) {
+    HSI_HIP(ctx, launch_mlp_layouts(ctx->d_mlp_w, bands, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, s),
+            "pre-screen layouts");
+    ctx->mlp_dirty = false;
+  }
+  MlpInferArgs a;
+  a.cube = d_cube;
+  a.dtype = HSI_F32;
+  a.pixels = static_cast<std::size_t>(height) * width;
+  a.bands = bands;
+  a.stride = plane_stride;
+  a.center = ctx->d_mean;
+  a.a1 = ctx->d_mlp_a1;
+  a.a2 = ctx->d_mlp_a2;
+  a.tail = ctx->d_mlp_tail;
+  a.logit = d_logit;
+  a.cut = cut;
+  a.cand = d_candidates;
+  a.cand_count = d_count;
+  a.cand_cap = static_cast<unsigned>(capacity);
+  a.spectra = d_spectra;
+  a.spectra_stride = spectra_stride;
+  a.width = width;
+  const std::size_t cap = static_cast<std::size_t>(capacity);
+  HSI_HIP(ctx, grow(&ctx->d_screen_rec, &ctx->screen_rec_bytes, (cap ? cap : 1) * sizeof(hsi_detection)),
+          "screening scratch");
+  if (d_spectra)
+    HSI_HIP(ctx, grow(&ctx->d_screen_spec, &ctx->screen_spec_bytes, (cap ? cap : 1) * pb * sizeof(float)),
+            "screening scratch");
+  HSI_HIP(ctx, grow(&ctx->d_screen_counts, &ctx->screen_counts_bytes, 2 * kScreenMaxWaves * sizeof(unsigned)),
+          "screening scratch");
+  a.scratch_rec = static_cast<hsi_detection*>(ctx->d_screen_rec);
+  a.scratch_spectra = static_cast<float*>(ctx->d_screen_spec);
+  a.scratch_counts = static_cast<unsigned*>(ctx->d_screen_counts);
+  HSI_HIP(ctx, launch_mlp_infer(a, s), "pre-screen screening");
   return HSI_OK;
 }
 

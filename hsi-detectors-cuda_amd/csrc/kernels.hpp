@@ -128,9 +128,12 @@ struct RxAceArgs {
 hipError_t launch_rxace_plain(const RxAceArgs& args, hipStream_t stream);
 // The plain kernel over a candidate list (raster-order detections of a
 // pre-screen): scores only pixels (y * width + x) of the first
-// min(*count, capacity) records, written at their own positions.
+// min(*count, capacity) records, written at their own positions. With
+// spectra non-null, record i's bands are read from row i of spectra
+// (pitch spectra_stride >= bands) instead of the cube.
 hipError_t launch_rxace_candidates(const RxAceArgs& args, const hsi_detection* cand, const unsigned* count,
-                                   unsigned capacity, int width, hipStream_t stream);
+                                   unsigned capacity, int width, const float* spectra,
+                                   std::size_t spectra_stride, hipStream_t stream);
 hipError_t launch_fill_scores(float* score, int* target, float* aux, std::size_t pixels, hipStream_t stream);
 
 // Batched plain RX/ACE: scenes of one row-block count NB (= rx_blocks(bands))
@@ -195,8 +198,27 @@ struct MlpInferArgs {
   const void* a1 = nullptr;       // bf16 layer-1 fragments
   const void* a2 = nullptr;       // bf16 layer-2 fragments
   const float* tail = nullptr;    // b1 | b2 | w3 | b3
-  float* logit = nullptr;
+  float* logit = nullptr;         // nullable when screening
+  // Screening (cand non-null, F32 cube): pixels with logit >= cut are
+  // listed in cand[0 .. *cand_count) (x, y, 0, logit), in a fixed order
+  // (region of the wave that scanned them, then raster order); with spectra
+  // non-null each one's bands go to row `slot` of spectra (pixel-major,
+  // pitch spectra_stride >= 32 * ceil(bands / 32)). A count above cand_cap
+  // means the list overflowed. The scratch holds the per-wave regions:
+  // cand_cap records, cand_cap spectra rows of 32 * ceil(bands / 32) floats,
+  // 2 * kScreenMaxWaves counters.
+  float cut = 0.f;
+  hsi_detection* cand = nullptr;
+  unsigned* cand_count = nullptr;
+  unsigned cand_cap = 0;
+  float* spectra = nullptr;
+  std::size_t spectra_stride = 0;
+  int width = 0;
+  hsi_detection* scratch_rec = nullptr;
+  float* scratch_spectra = nullptr;
+  unsigned* scratch_counts = nullptr;
 };
+constexpr int kScreenMaxWaves = 4096;
 std::size_t mlp_param_count(int bands);
 std::size_t mlp_a1_halves(int bands);
 std::size_t mlp_a2_halves();

@@ -17,6 +17,8 @@
 // Adam kernel updates the masters. The loss is binary cross-entropy on the
 // logit with a weight on positives (recall matters more than precision for a
 // pre-screen).
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -102,11 +104,80 @@ __device__ __forceinline__ f4 load4(__amdgpu_buffer_rsrc_t r, std::uint32_t off,
 // weight fragment read from LDS serves four MFMAs and each load moves 1 KiB
 // (FP32) per wave. The next K block's loads are in flight while the matrix
 // cores work on this one.
-template <typename In, int NKB>
+// Screening outputs, per wave: wave w appends its candidates to its own
+// region (slots [w * rc, w * rc + rc)) of the record and spectra scratch and
+// leaves its count in counts[w] - no atomics, and the order is fixed (wave,
+// then tile, then pixel). A scan and a compaction pass make the list
+// contiguous afterwards.
+struct ScreenOut {
+  float cut;
+  hsi_detection* cand;       // region layout
+  float* spectra;            // region layout, pixel-major, pitch 32 * NKB
+  unsigned* counts;          // [waves]
+  unsigned rc;               // slots per wave
+  int width;
+};
+
+// Screening of one 64-pixel tile: lane q takes pixel q's logit (held by lane
+// q / 4, slot q % 4); flagged pixels get consecutive slots of the wave's
+// region, their records written by lane q and their spectra - still in the
+// lanes' registers (`keep`: band 32J + 8g + i of pixels 4n .. 4n + 3) - by
+// the four lanes of each pixel column.
+template <int NKB>
+__device__ __forceinline__ void screen_tile(std::size_t pixels, std::size_t tile, const f4& res,
+                                            const f4 (&keep)[NKB][8], const ScreenOut& so, std::size_t wave,
+                                            unsigned& local) {
+  constexpr int PB = 32 * NKB;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int n = lane & 15;
+  const int src = lane >> 2;
+  const float r0 = __shfl(res[0], src, 64), r1 = __shfl(res[1], src, 64);
+  const float r2 = __shfl(res[2], src, 64), r3 = __shfl(res[3], src, 64);
+  const int sub = lane & 3;
+  const float lg = sub == 0 ? r0 : (sub == 1 ? r1 : (sub == 2 ? r2 : r3));
+  const std::size_t px = tile * 64 + static_cast<std::size_t>(lane);
+  const bool flag = px < pixels && lg >= so.cut;
+  const unsigned long long m = __ballot(flag);
+  if (m == 0ull) return;
+  const std::size_t region = wave * so.rc;
+  if (flag) {
+    const unsigned s = local + static_cast<unsigned>(__popcll(m & ((1ull << lane) - 1ull)));
+    if (s < so.rc) {
+      const std::size_t w = static_cast<std::size_t>(so.width);
+      hsi_detection d;
+      d.x = static_cast<int>(px % w);
+      d.y = static_cast<int>(px / w);
+      d.target = 0;
+      d.score = lg;
+      so.cand[region + s] = d;
+    }
+  }
+  if (so.spectra) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int q = 4 * n + u;
+    if ((m >> q) & 1ull) {
+      const unsigned s = local + static_cast<unsigned>(__popcll(m & ((1ull << q) - 1ull)));
+      if (s < so.rc) {
+        float* dst = so.spectra + (region + s) * PB + 8 * g;
+#pragma unroll
+        for (int J = 0; J < NKB; ++J) {
+          *reinterpret_cast<f4*>(dst + 32 * J) = f4{keep[J][0][u], keep[J][1][u], keep[J][2][u], keep[J][3][u]};
+          *reinterpret_cast<f4*>(dst + 32 * J + 4) = f4{keep[J][4][u], keep[J][5][u], keep[J][6][u], keep[J][7][u]};
+        }
+      }
+    }
+  }
+  }
+  local += static_cast<unsigned>(__popcll(m));
+}
+
+template <typename In, int NKB, bool SCREEN>
 __global__ __launch_bounds__(kInferWaves * 64) void mlp_infer_kernel(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
     const float* __restrict__ center, const bf8* __restrict__ a1, const bf8* __restrict__ a2,
-    const float* __restrict__ tail, float* __restrict__ logit) {
+    const float* __restrict__ tail, float* __restrict__ logit, ScreenOut so) {
   __shared__ bf8 a1_lds[NKB * 4 * 64];
   __shared__ bf8 a2_lds[4 * 64];
   __shared__ float c_lds[NKB * 32];
@@ -132,40 +203,26 @@ __global__ __launch_bounds__(kInferWaves * 64) void mlp_infer_kernel(
     for (int i = 0; i < 8; ++i)
       xr[i] = load4(rsrc, base + static_cast<std::uint32_t>(static_cast<std::size_t>(i) * stride * sizeof(In)), In{});
   };
-  f4 cur[8], nxt[8];
-  if (wave_id < tiles) load_kblock(wave_id, 0, cur);
-
-  for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
-    // Re-read the weight fragments from LDS every tile rather than hoisting
-    // all of them into registers.
-    __asm__ volatile("" ::: "memory");
-    f4 z1[4][4];
+  // Layer 1 for one 32-band block: z1 += A1[J] x (x centred and rounded to
+  // bf16 here; padded bands read 0 and have weight 0).
+  auto layer1 = [&](int J, const f4 (&x)[8], f4 (&z1)[4][4]) {
+    bf8 xb[4];
 #pragma unroll
-    for (int sub = 0; sub < 4; ++sub)
+    for (int i = 0; i < 8; ++i) {
+      const float c = c_lds[32 * J + 8 * g + i];
 #pragma unroll
-      for (int I = 0; I < 4; ++I) z1[sub][I] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int J = 0; J < NKB; ++J) {
-      if (J + 1 < NKB) load_kblock(tile, J + 1, nxt);
-      else if (tile + waves < tiles) load_kblock(tile + waves, 0, nxt);
-      bf8 xb[4];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const float c = c_lds[32 * J + 8 * g + i];  // padded bands: value 0, weight 0
-#pragma unroll
-        for (int sub = 0; sub < 4; ++sub) xb[sub][i] = static_cast<__bf16>(cur[i][sub] - c);
-      }
-#pragma unroll
-      for (int I = 0; I < 4; ++I) {
-        const bf8 a = a1_lds[(J * 4 + I) * 64 + lane];
-#pragma unroll
-        for (int sub = 0; sub < 4; ++sub)
-          z1[sub][I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[sub], z1[sub][I], 0, 0, 0);
-      }
-#pragma unroll
-      for (int i = 0; i < 8; ++i) cur[i] = nxt[i];
+      for (int sub = 0; sub < 4; ++sub) xb[sub][i] = static_cast<__bf16>(x[i][sub] - c);
     }
-
+#pragma unroll
+    for (int I = 0; I < 4; ++I) {
+      const bf8 a = a1_lds[(J * 4 + I) * 64 + lane];
+#pragma unroll
+      for (int sub = 0; sub < 4; ++sub)
+        z1[sub][I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[sub], z1[sub][I], 0, 0, 0);
+    }
+  };
+  // Layers 2 and 3: lane (g, n) gets the logits of pixels 4n .. 4n + 3.
+  auto layers23 = [&](const f4 (&z1)[4][4]) {
     f4 res = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int sub = 0; sub < 4; ++sub) {
@@ -197,9 +254,11 @@ __global__ __launch_bounds__(kInferWaves * 64) void mlp_infer_kernel(
       out += __shfl_xor(out, 32, 64);
       res[sub] = out + t_lds[kH1 + 2 * kH2];
     }
-    // lane (0, n) holds the logits of pixels 4n .. 4n + 3
+    return res;
+  };
+  auto store_logits = [&](std::size_t tile, const f4& res) {
     const std::size_t p = tile * 64 + 4 * static_cast<std::size_t>(n);
-    if (g == 0) {
+    if (logit && g == 0) {
       if (p + 4 <= pixels) {
         *reinterpret_cast<f4*>(logit + p) = res;
       } else {
@@ -208,6 +267,140 @@ __global__ __launch_bounds__(kInferWaves * 64) void mlp_infer_kernel(
           if (p + sub < pixels) logit[p + sub] = res[sub];
       }
     }
+  };
+  unsigned local = 0;
+
+  if constexpr (SCREEN) {
+    // Screening keeps each tile's raw inputs (`keep`, all of its bands) until
+    // its logits are known. Loads run two 32-band blocks ahead: blocks 2.. of
+    // a tile are issued into `keep` as the tile starts, and blocks 0 and 1 of
+    // the next tile into `nx` once block 1 is scored.
+    constexpr int AHEAD = NKB < 2 ? NKB : 2;
+    f4 keep[NKB][8], nx[AHEAD][8];
+    if (wave_id < tiles) {
+#pragma unroll
+      for (int J = 0; J < AHEAD; ++J) load_kblock(wave_id, J, nx[J]);
+    }
+    for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
+      __asm__ volatile("" ::: "memory");
+#pragma unroll
+      for (int J = 0; J < AHEAD; ++J)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) keep[J][i] = nx[J][i];
+#pragma unroll
+      for (int J = AHEAD; J < NKB; ++J) load_kblock(tile, J, keep[J]);
+      f4 z1[4][4];
+#pragma unroll
+      for (int sub = 0; sub < 4; ++sub)
+#pragma unroll
+        for (int I = 0; I < 4; ++I) z1[sub][I] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int J = 0; J < NKB; ++J) {
+        layer1(J, keep[J], z1);
+        if (J == AHEAD - 1 && tile + waves < tiles) {
+#pragma unroll
+          for (int K = 0; K < AHEAD; ++K) load_kblock(tile + waves, K, nx[K]);
+        }
+      }
+      const f4 res = layers23(z1);
+      store_logits(tile, res);
+      screen_tile<NKB>(pixels, tile, res, keep, so, wave_id, local);
+    }
+  } else {
+    f4 cur[8], nxt[8];
+    if (wave_id < tiles) load_kblock(wave_id, 0, cur);
+    for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
+      // Re-read the weight fragments from LDS every tile rather than hoisting
+      // all of them into registers.
+      __asm__ volatile("" ::: "memory");
+      f4 z1[4][4];
+#pragma unroll
+      for (int sub = 0; sub < 4; ++sub)
+#pragma unroll
+        for (int I = 0; I < 4; ++I) z1[sub][I] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int J = 0; J < NKB; ++J) {
+        if (J + 1 < NKB) load_kblock(tile, J + 1, nxt);
+        else if (tile + waves < tiles) load_kblock(tile + waves, 0, nxt);
+        layer1(J, cur, z1);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cur[i] = nxt[i];
+      }
+      store_logits(tile, layers23(z1));
+    }
+  }
+  if constexpr (SCREEN) {
+    if ((threadIdx.x & 63) == 0) so.counts[wave_id] = local;
+  }
+}
+
+// Exclusive scan of the per-wave counts (one workgroup), the total, and an
+// overflow mark: a wave whose candidates did not fit its region lost some, so
+// the count is raised past the capacity and the rescoring scores the frame.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void screen_scan_kernel(const unsigned* __restrict__ counts, int waves,
+                                                                   unsigned rc, unsigned cap,
+                                                                   unsigned* __restrict__ prefix,
+                                                                   unsigned* __restrict__ total_out) {
+  __shared__ unsigned part[kScanThreads];
+  __shared__ unsigned over;
+  const int per = (waves + kScanThreads - 1) / kScanThreads;
+  const int b = threadIdx.x * per;
+  unsigned s = 0;
+  bool lost = false;
+  for (int k = 0; k < per; ++k)
+    if (b + k < waves) {
+      const unsigned c = counts[b + k];
+      lost |= c > rc;
+      s += c < rc ? c : rc;
+    }
+  if (threadIdx.x == 0) over = 0u;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (lost) atomicOr(&over, 1u);
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const unsigned v = threadIdx.x >= static_cast<unsigned>(off) ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned run = part[threadIdx.x] - s;   // exclusive
+  for (int k = 0; k < per; ++k)
+    if (b + k < waves) {
+      prefix[b + k] = run;
+      const unsigned c = counts[b + k];
+      run += c < rc ? c : rc;
+    }
+  if (threadIdx.x == kScanThreads - 1) {
+    const unsigned total = part[kScanThreads - 1];
+    *total_out = (over != 0u && total <= cap) ? cap + 1u : total;
+  }
+}
+
+// Region layout -> contiguous list: one wave per region copies its records
+// and spectra rows (pitch pb in, out_pitch out) to their final slots.
+__global__ __launch_bounds__(256) void screen_compact_kernel(const unsigned* __restrict__ counts,
+                                                             const unsigned* __restrict__ prefix, int waves,
+                                                             unsigned rc, unsigned cap, int pb,
+                                                             const hsi_detection* __restrict__ rec_in,
+                                                             const float* __restrict__ spec_in,
+                                                             hsi_detection* __restrict__ rec_out,
+                                                             float* __restrict__ spec_out, std::size_t out_pitch) {
+  const int w = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+  if (w >= waves) return;
+  const int lane = threadIdx.x & 63;
+  const unsigned c = min(counts[w], rc);
+  const unsigned base = prefix[w];
+  const std::size_t region = static_cast<std::size_t>(w) * rc;
+  for (unsigned j = lane; j < c; j += 64)
+    if (base + j < cap) rec_out[base + j] = rec_in[region + j];
+  if (!spec_out) return;
+  const int q = pb / 4;   // float4 per row
+  for (unsigned e = lane; e < c * static_cast<unsigned>(q); e += 64) {
+    const unsigned j = e / q, t = e % q;
+    if (base + j < cap)
+      *reinterpret_cast<f4*>(spec_out + (base + j) * out_pitch + 4 * t) =
+          *reinterpret_cast<const f4*>(spec_in + (region + j) * pb + 4 * t);
   }
 }
 
@@ -215,24 +408,66 @@ template <typename In>
 hipError_t launch_infer_typed(const MlpInferArgs& a, hipStream_t stream) {
   if (!fits_buffer(a.bands, a.stride, sizeof(In))) return hipErrorInvalidValue;
   const std::size_t tiles = div_up(a.pixels, 64);
-  std::size_t wgs = div_up(tiles, kInferWaves);
-  if (wgs > 2048) wgs = 2048;
-  const dim3 grid(static_cast<unsigned>(wgs)), block(kInferWaves * 64);
   const In* cube = static_cast<const In*>(a.cube);
   const bf8* a1 = reinterpret_cast<const bf8*>(a.a1);
   const bf8* a2 = reinterpret_cast<const bf8*>(a.a2);
-  switch (div_up(a.bands, 32)) {
+  const int nkb = div_up(a.bands, 32);
+  if (!a.cand) {
+    if (!a.logit) return hipErrorInvalidValue;
+    std::size_t wgs = div_up(tiles, kInferWaves);
+    if (wgs > 2048) wgs = 2048;
+    const ScreenOut none{};
+    switch (nkb) {
 #define HSI_K(N)                                                                                   \
   case N:                                                                                          \
-    hipLaunchKernelGGL((mlp_infer_kernel<In, N>), grid, block, 0, stream, cube, a.pixels, a.bands, \
-                       a.stride, a.center, a1, a2, a.tail, a.logit);                              \
+    hipLaunchKernelGGL((mlp_infer_kernel<In, N, false>), dim3(static_cast<unsigned>(wgs)),         \
+                       dim3(kInferWaves * 64), 0, stream, cube, a.pixels, a.bands, a.stride,        \
+                       a.center, a1, a2, a.tail, a.logit, none);                                    \
     break;
-    HSI_K(1) HSI_K(2) HSI_K(3) HSI_K(4) HSI_K(5) HSI_K(6) HSI_K(7) HSI_K(8)
+      HSI_K(1) HSI_K(2) HSI_K(3) HSI_K(4) HSI_K(5) HSI_K(6) HSI_K(7) HSI_K(8)
 #undef HSI_K
-    default:
-      return hipErrorInvalidValue;
+      default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
   }
-  return hipGetLastError();
+  // Screening: at least four tiles per wave, at most 1024 workgroups.
+  if constexpr (!std::is_same_v<In, float>) {
+    return hipErrorInvalidValue;
+  } else {
+    const int pb = 32 * nkb;
+    if (!a.cand_count || a.width <= 0 || !a.scratch_rec || !a.scratch_counts ||
+        (a.spectra && (a.spectra_stride < static_cast<std::size_t>(pb) || !a.scratch_spectra)))
+      return hipErrorInvalidValue;
+    std::size_t wgs = div_up(tiles, static_cast<std::size_t>(4 * kInferWaves));
+    if (wgs > kScreenMaxWaves / kInferWaves) wgs = kScreenMaxWaves / kInferWaves;
+    if (wgs == 0) wgs = 1;
+    const int waves = static_cast<int>(wgs) * kInferWaves;
+    const unsigned rc = a.cand_cap / static_cast<unsigned>(waves);
+    ScreenOut so{a.cut, a.scratch_rec, a.spectra ? a.scratch_spectra : nullptr, a.scratch_counts, rc, a.width};
+    switch (nkb) {
+#define HSI_K(N)                                                                                   \
+  case N:                                                                                          \
+    hipLaunchKernelGGL((mlp_infer_kernel<float, N, true>), dim3(static_cast<unsigned>(wgs)),       \
+                       dim3(kInferWaves * 64), 0, stream, cube, a.pixels, a.bands, a.stride,        \
+                       a.center, a1, a2, a.tail, a.logit, so);                                      \
+    break;
+      HSI_K(1) HSI_K(2) HSI_K(3) HSI_K(4) HSI_K(5) HSI_K(6) HSI_K(7) HSI_K(8)
+#undef HSI_K
+      default:
+        return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    unsigned* prefix = a.scratch_counts + kScreenMaxWaves;
+    hipLaunchKernelGGL(screen_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, a.scratch_counts, waves, rc,
+                       a.cand_cap, prefix, a.cand_count);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(screen_compact_kernel, dim3(static_cast<unsigned>(div_up(waves, 4))), dim3(256), 0, stream,
+                       a.scratch_counts, prefix, waves, rc, a.cand_cap, pb, a.scratch_rec, a.scratch_spectra,
+                       a.cand, a.spectra, a.spectra_stride);
+    return hipGetLastError();
+  }
 }
 
 // ---- training ----------------------------------------------------------------

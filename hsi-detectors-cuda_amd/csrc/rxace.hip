@@ -17,6 +17,8 @@
 // reference narrows them to float: detector.cpp:83-84) and are widened per
 // MFMA; each wave keeps its pixel tile's centred spectrum in registers in the
 // MFMA B-operand layout and walks tiles persistently.
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -42,25 +44,35 @@ __device__ __forceinline__ void ace_merge(double& s, int& t, double s2, int t2) 
 // GATHER: score only the pixels named by a candidate list (detection records
 // in raster order, pixel y * width + x), *cand_count of them (the whole frame
 // if that exceeds cand_cap); each 16-record slice of the list is one tile.
+// Candidate list of the GATHER form (see rxace_tiles).
+struct GatherDev {
+  const hsi_detection* cand = nullptr;
+  const unsigned* count = nullptr;
+  unsigned cap = 0;
+  int width = 0;
+  const float* spectra = nullptr;     // nullable: candidate spectra, row i = record i
+  std::size_t spectra_stride = 0;
+};
+
 template <typename In, int NB, int TT, bool GATHER>
 __device__ __forceinline__ void rxace_tiles(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride, int num_targets,
     const float* __restrict__ target_norm2, float* __restrict__ rx_out, float* __restrict__ ace_score,
     int* __restrict__ ace_target, const float* w_lds, const float* u_lds,
-    const float* mean_lds, std::size_t first, std::size_t step, std::size_t last,
-    const hsi_detection* __restrict__ cand, const unsigned* __restrict__ cand_count, unsigned cand_cap,
-    int width) {
+    const float* mean_lds, std::size_t first, std::size_t step, std::size_t last, const GatherDev gd) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;   // lane group: K index of the A/B operands
   const int n = lane & 15;   // pixel column within the tile
   std::size_t items = pixels;
   bool gather = GATHER;
+  bool from_spectra = false;
   if constexpr (GATHER) {
     // A list that overflowed its capacity means the screen flagged more than
     // it can hold: fall back to scoring the whole frame.
-    const unsigned c = *cand_count;
-    if (c > cand_cap) gather = false;
+    const unsigned c = *gd.count;
+    if (c > gd.cap) gather = false;
     else items = c;
+    from_spectra = gather && gd.spectra != nullptr;
   }
   std::size_t tiles = div_up(items, 16);
   if (last < tiles) tiles = last;
@@ -71,11 +83,12 @@ __device__ __forceinline__ void rxace_tiles(
     if constexpr (GATHER) {
       if (!gather) return i;
       if (!ok) return 0;
-      const hsi_detection d = cand[i];
+      const hsi_detection d = gd.cand[i];
       // a record outside the frame names no pixel
-      const std::size_t rows = pixels / static_cast<std::size_t>(width);
-      ok = d.x >= 0 && d.x < width && d.y >= 0 && static_cast<std::size_t>(d.y) < rows;
-      return ok ? static_cast<std::size_t>(d.y) * static_cast<std::size_t>(width) + static_cast<std::size_t>(d.x) : 0;
+      const std::size_t rows = pixels / static_cast<std::size_t>(gd.width);
+      ok = d.x >= 0 && d.x < gd.width && d.y >= 0 && static_cast<std::size_t>(d.y) < rows;
+      return ok ? static_cast<std::size_t>(d.y) * static_cast<std::size_t>(gd.width) + static_cast<std::size_t>(d.x)
+                : 0;
     } else {
       return i;
     }
@@ -97,15 +110,26 @@ __device__ __forceinline__ void rxace_tiles(
   // Raw inputs of a 16-pixel tile, all issued at once: band 16J + 4kk + g of
   // pixel tile*16 + n (the B-operand layout). The next tile's loads are in
   // flight while the matrix cores work on this one.
+  // With candidate spectra, element i of the list reads row i of spectra
+  // (its bands contiguous) instead of its pixel in the cube.
   auto load_tile = [&](std::size_t t, float (&xr)[NB][4]) {
     bool pix_ok;
     const std::size_t pp = pixel_of(t, pix_ok);
+    const In* src = cube;
+    std::size_t src_stride = stride, sp = pp;
+    if constexpr (GATHER && std::is_same_v<In, float>) {
+      if (from_spectra) {
+        src = gd.spectra;
+        src_stride = 1;
+        sp = (t * 16 + n) * gd.spectra_stride;
+      }
+    }
 #pragma unroll
     for (int J = 0; J < NB; ++J)
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         const int b = 16 * J + 4 * kk + g;
-        xr[J][kk] = (pix_ok && b < bands) ? static_cast<float>(cube[static_cast<std::size_t>(b) * stride + pp]) : 0.f;
+        xr[J][kk] = (pix_ok && b < bands) ? static_cast<float>(src[static_cast<std::size_t>(b) * src_stride + sp]) : 0.f;
       }
   };
   float cur[NB][4], nxt[NB][4];
@@ -224,8 +248,7 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
     int num_targets, const float* __restrict__ mean, const float* __restrict__ white_blocks,
     const float* __restrict__ ace_u, const float* __restrict__ target_norm2,
     float* __restrict__ rx_out, float* __restrict__ ace_score, int* __restrict__ ace_target,
-    const hsi_detection* __restrict__ cand, const unsigned* __restrict__ cand_count, unsigned cand_cap,
-    int width) {
+    const GatherDev gd) {
   __shared__ float w_lds[RxLds<NB, TT>::WFLOATS];
   __shared__ float u_lds[RxLds<NB, TT>::UFLOATS];
   __shared__ float mean_lds[NB * 16];
@@ -235,7 +258,7 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
   rxace_tiles<In, NB, TT, GATHER>(cube, pixels, bands, stride, num_targets, target_norm2, rx_out, ace_score,
                                   ace_target, w_lds, u_lds, mean_lds, wave_id,
                                   static_cast<std::size_t>(gridDim.x) * kRxWaves, ~static_cast<std::size_t>(0),
-                                  cand, cand_count, cand_cap, width);
+                                  gd);
 }
 
 // Batched scenes of one shape class in one launch over work items (a scene
@@ -267,7 +290,7 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_batch_kernel(const RxScen
       rxace_tiles<float, NB, TT, false>(static_cast<const float*>(sc.cube), sc.pixels, sc.bands, sc.stride,
                                         sc.num_targets, sc.target_norm2, sc.rx, sc.ace_score, sc.ace_target, w_lds,
                                         u_lds, mean_lds, w.tile_begin + (threadIdx.x >> 6), kRxWaves, w.tile_end,
-                                        nullptr, nullptr, 0, 0);
+                                        GatherDev{});
     }
   } else {
     if (static_cast<int>(blockIdx.x) >= num_items) return;
@@ -278,30 +301,27 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_batch_kernel(const RxScen
     rxace_tiles<float, NB, TT, false>(static_cast<const float*>(sc.cube), sc.pixels, sc.bands, sc.stride,
                                       sc.num_targets, sc.target_norm2, sc.rx, sc.ace_score, sc.ace_target, w_lds,
                                       u_lds, mean_lds, w.tile_begin + (threadIdx.x >> 6), kRxWaves, w.tile_end,
-                                      nullptr, nullptr, 0, 0);
+                                      GatherDev{});
   }
 }
 
-struct Gather {
-  const hsi_detection* cand = nullptr;
-  const unsigned* count = nullptr;
-  unsigned capacity = 0;
-  int width = 0;
-};
+using Gather = GatherDev;
 
 template <typename In, int NB, int TT, bool GATHER>
 hipError_t launch_plain_t(const RxAceArgs& a, const Gather& gth, hipStream_t stream) {
   // A candidate list's length is known only on the device: size the grid for
   // its capacity and let the kernel stop at the count.
-  const std::size_t tiles = div_up(GATHER ? static_cast<std::size_t>(gth.capacity) : a.pixels, 16);
+  const std::size_t tiles = div_up(GATHER ? static_cast<std::size_t>(gth.cap) : a.pixels, 16);
   std::size_t wgs = div_up(tiles, kRxWaves);
-  if (wgs > 1024) wgs = 1024;
+  // A candidate list is usually far shorter than its capacity: one
+  // workgroup per CU keeps the whitener staging per tile scored low.
+  const std::size_t max_wgs = GATHER ? 256 : 1024;
+  if (wgs > max_wgs) wgs = max_wgs;
   if (wgs == 0) wgs = 1;
   hipLaunchKernelGGL((rxace_plain_kernel<In, NB, TT, GATHER>), dim3(static_cast<unsigned>(wgs)),
                      dim3(kRxWaves * 64), 0, stream, static_cast<const In*>(a.cube), a.pixels,
                      a.bands, a.stride, a.num_targets, a.mean, a.white_blocks, a.ace_u,
-                     a.target_norm2, a.rx, a.ace_score, a.ace_target, gth.cand, gth.count, gth.capacity,
-                     gth.width);
+                     a.target_norm2, a.rx, a.ace_score, a.ace_target, gth);
   return hipGetLastError();
 }
 
@@ -418,13 +438,17 @@ hipError_t launch_rxace_plain(const RxAceArgs& a, hipStream_t stream) {
 }
 
 hipError_t launch_rxace_candidates(const RxAceArgs& a, const hsi_detection* cand, const unsigned* count,
-                                   unsigned capacity, int width, hipStream_t stream) {
+                                   unsigned capacity, int width, const float* spectra,
+                                   std::size_t spectra_stride, hipStream_t stream) {
   if (a.dtype != HSI_F32) return hipErrorInvalidValue;  // candidates are rescored from the FP32 cube
+  if (spectra && spectra_stride < static_cast<std::size_t>(a.bands)) return hipErrorInvalidValue;
   Gather gth;
   gth.cand = cand;
   gth.count = count;
-  gth.capacity = capacity;
+  gth.cap = capacity;
   gth.width = width;
+  gth.spectra = spectra;
+  gth.spectra_stride = spectra_stride;
   return launch_plain_typed<float, true>(a, gth, stream);
 }
 

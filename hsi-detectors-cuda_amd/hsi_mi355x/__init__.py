@@ -179,6 +179,11 @@ def lib():
                                         C.c_int, C.c_int, _vp, C.c_size_t, _vp]),
         "hsi_score_candidates": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                            C.c_size_t, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
+        "hsi_mlp_screen": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_float,
+                                     _vp, _vp, C.c_int, _vp, C.c_size_t, _vp, _vp]),
+        "hsi_score_candidate_spectra": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+                                                  C.c_size_t, _vp, _vp, C.c_int, _vp, C.c_size_t,
+                                                  _vp, _vp, _vp, _vp]),
         "hsi_demosaic": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
                                    C.c_size_t, _vp]),
         "hsi_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
@@ -446,6 +451,25 @@ class Context:
                                                    width, bands, cube.stride(0), _ptr(candidates),
                                                    _ptr(count), cap, _ptr(score), _ptr(target),
                                                    _ptr(aux), _stream(stream)))
+
+
+    def mlp_screen(self, cube, height, width, bands, cut, candidates, count, spectra=None,
+                   logit=None, stream=None):
+        """One pass: candidates at the cut (fixed order, not raster) and with
+        `spectra` ([capacity, >= 32 * ceil(bands / 32)] float32) their bands."""
+        cap = candidates.shape[0]
+        self._check(self._lib.hsi_mlp_screen(self.handle, _ptr(cube), height, width, bands,
+                                             cube.stride(0), float(cut), _ptr(candidates), _ptr(count),
+                                             cap, _ptr(spectra), spectra.stride(0) if spectra is not None else 0,
+                                             _ptr(logit), _stream(stream)))
+
+    def score_candidate_spectra(self, detector, cube, height, width, bands, candidates, count, spectra,
+                                score, target=None, aux=None, stream=None):
+        cap = candidates.shape[0]
+        self._check(self._lib.hsi_score_candidate_spectra(
+            self.handle, int(detector), _ptr(cube), height, width, bands, cube.stride(0),
+            _ptr(candidates), _ptr(count), cap, _ptr(spectra), spectra.stride(0), _ptr(score),
+            _ptr(target), _ptr(aux), _stream(stream)))
 
 
 RAW_U8, RAW_U16, RAW_F32 = 0, 1, 2

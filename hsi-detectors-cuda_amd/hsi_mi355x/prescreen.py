@@ -2,11 +2,13 @@
 
 A small network (B -> 64 -> 32 -> 1) is trained on the GPU by the library's
 own forward/backward/Adam kernels to predict which pixels the plain RX/ACE
-detectors will put over threshold. At run time its logits over a frame are
-compacted at a conservative cut (hsi_detect on the logit map, no NMS), only
-those candidates are rescored exactly (hsi_score_candidates: the plain
-kernel, so their scores are bit-identical to the full-frame path), and the
-usual threshold + NMS runs on the result. The plain full-frame detectors are
+detectors will put over threshold. At run time one pass over the frame
+(hsi_mlp_screen) computes the logits, lists the pixels at or above a
+conservative cut and copies their spectra to a compact buffer; only those
+candidates are rescored exactly
+(hsi_score_candidate_spectra: the plain kernel on the copied spectra, so
+their scores are bit-identical to the full-frame path), and the usual
+threshold + NMS runs on the result. The plain full-frame detectors are
 the judge (training labels, recall) and the fallback (a candidate list that
 overflows its capacity makes the device score the whole frame).
 
@@ -28,7 +30,7 @@ from . import Context, DetectionParams, Detector, MlpTrainParams, rx_threshold
 
 class Prescreen:
     def __init__(self, ctx: Context, bands: int, rx_false_alarm_rate=1e-6, ace_threshold=0.5,
-                 nms_radius=2, seed=1, capacity=1 << 18, label_slack=0.7):
+                 nms_radius=2, seed=1, capacity=1 << 19, label_slack=0.7):
         import torch
         self.torch = torch
         self.ctx = ctx
@@ -58,6 +60,7 @@ class Prescreen:
             self._maps = {"pixels": pixels, "ace": t.empty(pixels, **f32), "rx": t.empty(pixels, **f32),
                           "target": t.empty(pixels, **i32), "logit": t.empty(pixels, **f32),
                           "cand": t.empty((self.capacity, 4), **i32), "count": t.empty(1, **i32),
+                          "spectra": t.empty((self.capacity, (self.bands + 31) // 32 * 32), **f32),
                           "dets": t.empty((1 << 16, 4), **i32), "ndet": t.empty(1, **i32)}
         return self._maps
 
@@ -145,8 +148,19 @@ class Prescreen:
 
     # -- run time --------------------------------------------------------------------
     def screen(self, cube, h, w):
-        """Learned path (statistics must be built): logits, candidates at the
-        cut, exact rescoring of the candidates; returns the score buffers."""
+        """Learned path (statistics must be built): one screening pass
+        (candidates at the cut and their spectra), exact rescoring of the
+        candidates; returns the score buffers."""
+        m = self._buffers(h * w)
+        self.ctx.mlp_screen(cube, h, w, self.bands, self.cut, m["cand"], m["count"], m["spectra"])
+        self.ctx.score_candidate_spectra(Detector.ACE, cube, h, w, self.bands, m["cand"], m["count"],
+                                         m["spectra"], m["ace"], m["target"], m["rx"])
+        return m
+
+    def screen_two_pass(self, cube, h, w):
+        """The same result in separate steps: the logit map, hsi_detect at the
+        cut (raster order), rescoring that gathers the candidates from the
+        cube."""
         m = self._buffers(h * w)
         self.ctx.mlp_infer(cube, h, w, self.bands, m["logit"], center=True)
         self.ctx.detect(m["logit"], None, h, w, DetectionParams(self.cut, 0, self.capacity, True),

@@ -288,13 +288,42 @@ hsi_status hsi_gather_pixels(hsi_ctx* ctx, const float* d_cube, int height, int 
  * first *d_count records of d_candidates (x, y as written by hsi_detect over
  * a logit map). Every other pixel of the score map gets 0 (ACE: target -1,
  * aux 0). If *d_count exceeds `capacity` the list overflowed and the whole
- * frame is scored instead (decided on the device, no host round trip). Candidates' scores equal hsi_score's for the same
- * pixels; records outside the frame are skipped. F32 cube. Asynchronous. */
+ * frame is scored instead (decided on the device, no host round trip).
+ * Candidates' scores equal hsi_score's for the same pixels; records outside
+ * the frame are skipped. F32 cube. Asynchronous. */
 hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_cube,
                                 int height, int width, int bands, size_t plane_stride,
                                 const hsi_detection* d_candidates, const uint32_t* d_count,
                                 int capacity, float* d_score, int32_t* d_target,
                                 float* d_aux, hsi_stream stream);
+
+/* Screening in one pass over the cube (F32, centred on the statistics'
+ * mean, which must be built for this band count): every pixel whose logit
+ * is >= cut is listed in d_candidates[0 .. *d_count) as {x, y, 0, logit}, in
+ * a fixed order that is not raster order, and with d_spectra non-null its
+ * bands are copied to row i of d_spectra ([capacity][spectra_stride],
+ * spectra_stride >= 32 * ceil(bands / 32), a multiple of 4, 16-byte
+ * aligned rows). The pass splits the capacity
+ * evenly over its waves; if any wave finds more candidates than its share,
+ * or the total exceeds `capacity`, *d_count is set above `capacity` (the
+ * list overflowed; hsi_score_candidate_spectra then scores the whole frame).
+ * d_logit (nullable) receives the full logit map. The library keeps scratch
+ * of the list's size. Asynchronous. */
+hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int width, int bands,
+                          size_t plane_stride, float cut, hsi_detection* d_candidates,
+                          uint32_t* d_count, int capacity, float* d_spectra,
+                          size_t spectra_stride, float* d_logit, hsi_stream stream);
+
+/* hsi_score_candidates over the rows that hsi_mlp_screen copied: record i
+ * is scored from row i of d_spectra (pitch spectra_stride >= bands) and
+ * written at its (x, y). The whole-frame fallback on overflow reads d_cube.
+ * Same results as hsi_score_candidates. Asynchronous. */
+hsi_status hsi_score_candidate_spectra(hsi_ctx* ctx, hsi_detector detector, const float* d_cube,
+                                       int height, int width, int bands, size_t plane_stride,
+                                       const hsi_detection* d_candidates, const uint32_t* d_count,
+                                       int capacity, const float* d_spectra, size_t spectra_stride,
+                                       float* d_score, int32_t* d_target, float* d_aux,
+                                       hsi_stream stream);
 
 /* ---- batched scenes ----------------------------------------------------------
  *

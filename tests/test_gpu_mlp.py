@@ -186,3 +186,80 @@ def test_candidate_rescoring_equals_full_frame(H, det):
     torch.cuda.synchronize()
     assert np.array_equal(s.cpu().numpy().view(np.uint32), want.view(np.uint32))
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["scene_32x40x113_t4", "scene_13x37x113_t8"])
+def test_one_pass_screen_matches_logits_and_full_frame(H, name):
+    """hsi_mlp_screen's candidates are exactly the pixels whose logit (as
+    hsi_mlp_infer computes it) reaches the cut, with those logits and their
+    cube spectra, listed the same way on a second run; rescoring from the
+    copied spectra equals the full-frame plain ACE bit for bit; an
+    overflowing list falls back to the whole frame."""
+    import torch
+    from conftest import golden
+    g = golden(name)
+    b, h, w = g["cube"].shape
+    p = h * w
+    cube = H.to_device_cube(g["cube"], h, w)
+    ctx = H.Context(0)
+    ctx.set_library(g["library"])
+    ctx.stats_accumulate(cube, h, w, b)
+    ctx.stats_build(1e-6)
+    ctx.mlp_init(b, seed=21)
+    f32 = dict(dtype=torch.float32, device="cuda")
+    i32 = dict(dtype=torch.int32, device="cuda")
+    logit = torch.empty(p, **f32)
+    ctx.mlp_infer(cube, h, w, b, logit, center=True)
+    torch.cuda.synchronize()
+    lg = logit.cpu().numpy()
+    cut = float(np.quantile(lg, 0.8))
+    want = np.flatnonzero(lg >= cut)
+    cap = p + 3
+    cand = torch.full((cap, 4), -9, **i32)
+    count = torch.full((1,), 12345, **i32)
+    spectra = torch.full((cap, (b + 31) // 32 * 32), float("nan"), **f32)
+    logit2 = torch.empty(p, **f32)
+    print(f"screen {name}: cut {cut:.4f}, {want.size} of {p} expected", flush=True)
+    ctx.mlp_screen(cube, h, w, b, cut, cand, count, spectra, logit2)
+    torch.cuda.synchronize()
+    n = int(count.item())
+    assert n == want.size
+    assert torch.equal(logit, logit2)
+    rec = cand[:n].cpu().numpy()
+    got_px = rec[:, 1].astype(np.int64) * w + rec[:, 0]
+    assert np.array_equal(np.sort(got_px), want)
+    assert np.all(rec[:, 2] == 0)
+    assert np.array_equal(rec[:, 3].view(np.float32), lg[got_px])
+    host = g["cube"].reshape(b, p)
+    again = torch.full((cap, 4), -9, **i32)
+    ctx.mlp_screen(cube, h, w, b, cut, again, count, spectra)
+    torch.cuda.synchronize()
+    assert torch.equal(again[:n], cand[:n])
+    assert np.array_equal(spectra[:n, :b].cpu().numpy(), host[:, got_px].T)
+
+    full_s, full_a = torch.empty(p, **f32), torch.empty(p, **f32)
+    full_t = torch.empty(p, **i32)
+    ctx.score(H.Detector.ACE, cube, h, w, b, full_s, full_t, full_a)
+    s = torch.full((p,), float("nan"), **f32)
+    a = torch.full((p,), float("nan"), **f32)
+    t = torch.full((p,), -7, **i32)
+    ctx.score_candidate_spectra(H.Detector.ACE, cube, h, w, b, cand, count, spectra, s, t, a)
+    torch.cuda.synchronize()
+    mask = np.zeros(p, bool)
+    mask[want] = True
+    for got, ref, fill in ((s, full_s, 0.0), (a, full_a, 0.0), (t, full_t, -1)):
+        gv, rv = got.cpu().numpy(), ref.cpu().numpy()
+        assert np.array_equal(gv[mask].view(np.uint32), rv[mask].view(np.uint32))
+        assert np.all(gv[~mask] == fill)
+
+    # capacity below the count: the list overflows, the rescoring covers the frame
+    small = want.size // 2
+    cand_s = torch.full((small, 4), -9, **i32)
+    spec_s = torch.empty((small, (b + 31) // 32 * 32), **f32)
+    ctx.mlp_screen(cube, h, w, b, cut, cand_s, count, spec_s)
+    ctx.score_candidate_spectra(H.Detector.ACE, cube, h, w, b, cand_s, count, spec_s, s, t, a)
+    torch.cuda.synchronize()
+    assert int(count.item()) > small
+    assert np.array_equal(s.cpu().numpy().view(np.uint32), full_s.cpu().numpy().view(np.uint32))
+    assert np.array_equal(t.cpu().numpy(), full_t.cpu().numpy())
+    ctx.close()
