@@ -64,6 +64,9 @@ struct hsi_ctx {
   float* d_mlp_tail = nullptr;
   float* d_mlp_act = nullptr;
   std::size_t mlp_act_bytes = 0;
+  // Evaluation scratch.
+  void* d_eval = nullptr;
+  std::size_t eval_bytes = 0;
   // Screening scratch: per-wave regions of records and spectra, counters.
   void* d_screen_rec = nullptr;
   std::size_t screen_rec_bytes = 0;
@@ -178,7 +181,8 @@ void hsi_ctx_destroy(hsi_ctx* ctx) {
                   ctx->d_ace_hl, ctx->d_col_scale, ctx->d_row_inv, ctx->d_target_inv,
                   ctx->d_mom_scratch, ctx->d_det_scratch, ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v,
                   ctx->d_mlp_grad, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, ctx->d_mlp_act,
-                  ctx->d_positions, ctx->d_screen_rec, ctx->d_screen_spec, ctx->d_screen_counts};
+                  ctx->d_positions, ctx->d_screen_rec, ctx->d_screen_spec, ctx->d_screen_counts,
+                  ctx->d_eval};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete ctx;
@@ -642,6 +646,35 @@ hsi_status hsi_transpose_bip_to_bsq(const float* d_src, float* d_dst, int height
   if (launch_transpose_bip_to_bsq(d_src, d_dst, height, width, bands, plane_stride,
                                   static_cast<hipStream_t>(stream)) != hipSuccess)
     return HSI_ERR_DEVICE;
+  return HSI_OK;
+}
+
+hsi_status hsi_evaluate(hsi_ctx* ctx, const float* d_score, const uint8_t* d_labels, size_t n, int target_class,
+                        int max_label, int higher_is_better, float threshold, hsi_eval_result* result,
+                        double* class_stats, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (!d_score || !d_labels || !result || n == 0 || n > 0x7fffffffu)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "evaluate: need a score map, labels, a result and 1..2^31-1 pixels");
+  if (max_label < 0 || max_label >= kEvalMaxClasses || target_class < 0 || target_class > max_label)
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "evaluate: need 0 <= target_class <= max_label < " + std::to_string(kEvalMaxClasses));
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, grow(&ctx->d_eval, &ctx->eval_bytes, eval_scratch_bytes(n)), "evaluation scratch");
+  double out[8 + 3 * kEvalMaxClasses];
+  HSI_HIP(ctx, launch_evaluate(d_score, d_labels, n, target_class, max_label, higher_is_better != 0, threshold,
+                               ctx->d_eval, ctx->eval_bytes, out, static_cast<hipStream_t>(stream)),
+          "evaluate");
+  result->positives = static_cast<uint64_t>(out[0]);
+  result->negatives = static_cast<uint64_t>(out[1]);
+  result->tp = static_cast<uint64_t>(out[2]);
+  result->fp = static_cast<uint64_t>(out[3]);
+  result->fn = static_cast<uint64_t>(out[4]);
+  result->tn = static_cast<uint64_t>(out[5]);
+  const double pn = out[0] * out[1];
+  result->auc = pn > 0.0 ? out[6] / (2.0 * pn) : std::numeric_limits<double>::quiet_NaN();
+  if (class_stats)
+    for (int c = 0; c <= max_label; ++c)
+      for (int k = 0; k < 3; ++k) class_stats[3 * c + k] = out[8 + 3 * c + k];
   return HSI_OK;
 }
 

@@ -136,6 +136,16 @@ hipError_t launch_rxace_candidates(const RxAceArgs& args, const hsi_detection* c
                                    std::size_t spectra_stride, hipStream_t stream);
 hipError_t launch_fill_scores(float* score, int* target, float* aux, std::size_t pixels, hipStream_t stream);
 
+// ---- offline evaluation ------------------------------------------------------------
+constexpr int kEvalMaxClasses = 32;
+std::size_t eval_scratch_bytes(std::size_t n);
+// Synchronous on `stream`. out_host: [0] P, [1] N, [2..5] tp fp fn tn,
+// [6] AUC numerator (2 * below + tied, summed over positives), class c at
+// 8 + 3c: count, mean, min.
+hipError_t launch_evaluate(const float* score, const std::uint8_t* label, std::size_t n, int target, int max_label,
+                           int higher, float threshold, void* scratch, std::size_t scratch_bytes, double* out_host,
+                           hipStream_t stream);
+
 // Batched plain RX/ACE: scenes of one row-block count NB (= rx_blocks(bands))
 // and target-tile count TT (0: RX only, 1: <= 16 targets, 2: <= 32), FP32
 // cubes, statistics layouts already derived.

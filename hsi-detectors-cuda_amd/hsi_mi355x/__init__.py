@@ -102,6 +102,12 @@ class MlpTrainParams(C.Structure):
         super().__init__(learning_rate, beta1, beta2, epsilon, pos_weight)
 
 
+class EvalResult(C.Structure):
+    """hsi_eval_result."""
+    _fields_ = [("auc", C.c_double), ("positives", C.c_uint64), ("negatives", C.c_uint64),
+                ("tp", C.c_uint64), ("fp", C.c_uint64), ("fn", C.c_uint64), ("tn", C.c_uint64)]
+
+
 class Scene(C.Structure):
     """hsi_scene: one member of a batch (device cube, host library, output maps)."""
 
@@ -184,6 +190,8 @@ def lib():
         "hsi_score_candidate_spectra": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                                   C.c_size_t, _vp, _vp, C.c_int, _vp, C.c_size_t,
                                                   _vp, _vp, _vp, _vp]),
+        "hsi_evaluate": (C.c_int, [_vp, _vp, _vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_float,
+                                   C.POINTER(EvalResult), _vp, _vp]),
         "hsi_demosaic": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
                                    C.c_size_t, _vp]),
         "hsi_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
@@ -470,6 +478,25 @@ class Context:
             self.handle, int(detector), _ptr(cube), height, width, bands, cube.stride(0),
             _ptr(candidates), _ptr(count), cap, _ptr(spectra), spectra.stride(0), _ptr(score),
             _ptr(target), _ptr(aux), _stream(stream)))
+
+
+    def evaluate(self, score, labels, target_class=1, max_label=8, higher_is_better=False,
+                 threshold=0.10, stream=None):
+        """hsi_evaluate on device tensors (float32 scores, uint8 labels):
+        AUC, positives/negatives, tp/fp/fn/tn and the per-class table
+        {label: (count, mean, min)} for labels present."""
+        n = score.numel()
+        if labels.numel() != n:
+            raise ValueError("score map and labels differ in size")
+        res = EvalResult()
+        stats = np.zeros(3 * (max_label + 1), np.float64)
+        self._check(self._lib.hsi_evaluate(self.handle, _ptr(score), _ptr(labels), n, target_class,
+                                           max_label, int(bool(higher_is_better)), float(threshold),
+                                           C.byref(res), stats.ctypes.data, _stream(stream)))
+        out = {k: getattr(res, k) for k, _ in EvalResult._fields_}
+        out["classes"] = {c: (int(stats[3 * c]), float(stats[3 * c + 1]), float(stats[3 * c + 2]))
+                          for c in range(max_label + 1) if stats[3 * c] > 0}
+        return out
 
 
 RAW_U8, RAW_U16, RAW_F32 = 0, 1, 2

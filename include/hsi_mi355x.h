@@ -325,6 +325,35 @@ hsi_status hsi_score_candidate_spectra(hsi_ctx* ctx, hsi_detector detector, cons
                                        float* d_score, int32_t* d_target, float* d_aux,
                                        hsi_stream stream);
 
+/* ---- offline evaluation ------------------------------------------------------
+ *
+ * Replaces scripts/eval_detection.py (roc_auc :40-66, operating point and
+ * class table :100-140) for a device score map and a per-pixel class mask
+ * (uint8). Pixels with label <= max_label are annotated; label ==
+ * target_class is positive, every other annotated label negative.
+ *   auc: Mann-Whitney AUC with ties counted half (the rank sum with
+ *        tie-averaged ranks), exact; the score is ranked as is when
+ *        higher_is_better, negated otherwise (SAM angles); NaN when one side
+ *        is empty. NaN scores rank above every number.
+ *   tp, fp, fn, tn: predicted = score >= threshold (higher_is_better) or
+ *        score <= threshold, against the positive label.
+ * class_stats (nullable, host, 3 * (max_label + 1) doubles): per label its
+ * pixel count, mean score and minimum score (NaN scores skipped in the
+ * minimum). Synchronous; the library keeps scratch of the map's size. */
+typedef struct hsi_eval_result {
+  double auc;
+  uint64_t positives;
+  uint64_t negatives;
+  uint64_t tp;
+  uint64_t fp;
+  uint64_t fn;
+  uint64_t tn;
+} hsi_eval_result;
+
+hsi_status hsi_evaluate(hsi_ctx* ctx, const float* d_score, const uint8_t* d_labels, size_t n,
+                        int target_class, int max_label, int higher_is_better, float threshold,
+                        hsi_eval_result* result, double* class_stats, hsi_stream stream);
+
 /* ---- batched scenes ----------------------------------------------------------
  *
  * No counterpart in the reference (its pipeline scores one shape at a time).
