@@ -218,6 +218,11 @@ def precision_form(H, torch, repeats):
         ctx.stats_build(LOADING)
         ctx.score(H.Detector.ACE, c16, h, w, b, ace_s, ace_t, rx_s, precision=H.MIXED)
 
+    def mixed32():
+        ctx.stats_accumulate(c32, h, w, b, precision=H.MIXED)
+        ctx.stats_build(LOADING)
+        ctx.score(H.Detector.ACE, c32, h, w, b, ace_s, ace_t, rx_s, precision=H.MIXED)
+
     def score_only(prec, cube):
         return lambda: ctx.score(H.Detector.ACE, cube, h, w, b, ace_s, ace_t, rx_s, precision=prec)
 
@@ -247,6 +252,9 @@ def precision_form(H, torch, repeats):
     pa, pt, pr, prx, pace = outputs(plain)
     ms["mixed_score"] = time_on_stream(torch, score_only(H.MIXED, c16), repeats)
     ma, mt, mr, mrx, mace = outputs(mixed)
+    ms["mixed32_path"] = time_on_stream(torch, mixed32, repeats)
+    ms["mixed32_score"] = time_on_stream(torch, score_only(H.MIXED, c32), repeats)
+    qa, qt, qr, qrx, qace = outputs(mixed32)
     log("  precision form: " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in ms.items()))
     recall = {
         "rx": {"plain_detections": len(prx), "mixed_detections": len(mrx),
@@ -255,6 +263,7 @@ def precision_form(H, torch, repeats):
                 "recall": round(len(pace & mace) / max(len(pace), 1), 6)},
     }
     rel_rx = np.abs(mr.astype(np.float64) - pr) / np.maximum(np.abs(pr.astype(np.float64)), 1.0)
+    rel_rx32 = np.abs(qr.astype(np.float64) - pr) / np.maximum(np.abs(pr.astype(np.float64)), 1.0)
     ctx.close()
     pix = p / 1e6
     return {
@@ -270,7 +279,17 @@ def precision_form(H, torch, repeats):
         "recall_vs_plain": recall,
         "error_vs_plain_fp64": {"rx_max_rel": float(f"{rel_rx.max():.3g}"),
                                 "ace_max_abs": float(f"{np.abs(ma.astype(np.float64) - pa).max():.3g}"),
-                                "ace_target_agree": round(float((mt == pt).mean()), 6)},
+                                "ace_target_agree": round(float((mt == pt).mean()), 6),
+                                "note": "mostly the FP16 storage of the cube itself"},
+        "mixed_fp32_cube": {
+            "path_us": round(ms["mixed32_path"] * 1e3, 1), "score_us": round(ms["mixed32_score"] * 1e3, 1),
+            "speedup_path": round(ms["plain_path"] / ms["mixed32_path"], 3),
+            "speedup_score": round(ms["plain_score"] / ms["mixed32_score"], 3),
+            "recall_vs_plain": {"rx": round(len(prx & qrx) / max(len(prx), 1), 6),
+                                "ace": round(len(pace & qace) / max(len(pace), 1), 6)},
+            "error_vs_plain_fp64": {"rx_max_rel": float(f"{rel_rx32.max():.3g}"),
+                                    "ace_max_abs": float(f"{np.abs(qa.astype(np.float64) - pa).max():.3g}")},
+            "note": "same arithmetic on the FP32 cube (hi/lo split of the pixels too): no input quantisation"},
     }
 
 

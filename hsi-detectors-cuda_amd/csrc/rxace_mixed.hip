@@ -36,13 +36,13 @@ __host__ __device__ constexpr int block_prefix(int I) {
   return (I % 2 == 0) ? (I / 2) * (I / 2 + 1) : (I / 2 + 1) * (I / 2 + 1);
 }
 
-// Launch shape by row-block count: the whitener's LDS copy grows as NB^2 and
-// above 10 blocks only one workgroup fits a CU, so those workgroups carry 16
-// waves (4 per SIMD) and issue MFMAs for 2 row blocks at a time to stay
-// within 128 registers; up to 8 blocks a wave keeps a whole tile of loads in
-// flight (4-wave workgroups, several per CU); in between, 8-wave workgroups
-// stream one K block ahead.
-__host__ __device__ constexpr int mixed_waves(int NB) { return NB > 10 ? 16 : (NB > 8 ? 8 : 4); }
+// Launch shape by row-block count: up to 8 blocks a wave keeps a whole tile
+// of loads in flight (4-wave workgroups, several per CU); above that,
+// 8-wave workgroups stream one K block ahead. Above 10 blocks the whitener's
+// LDS copy leaves room for one workgroup per CU and MFMAs are issued for 2
+// row blocks at a time; 8 waves (256 registers each) rather than 16 (128
+// each) keeps the 224-band kernels free of spills (2.2x faster).
+__host__ __device__ constexpr int mixed_waves(int NB) { return NB > 8 ? 8 : 4; }
 __host__ __device__ constexpr int mixed_group(int NB) { return NB > 10 ? 2 : 4; }
 
 __device__ __forceinline__ float to_f(float v) { return v; }
