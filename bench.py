@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--frames", type=int, default=8, help="distinct resident frames cycled")
+    ap.add_argument("--in-flight", type=int, default=3,
+                    help="frames in flight (slots of two streams and two contexts each); 3 is the reference Pipeline default")
     ap.add_argument("--repeats", type=int, default=20, help="per-kernel timing repeats")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=1024, help="rows of a frame in the CPU sample")
@@ -496,18 +498,43 @@ def main():
     def build():
         ctx.stats_build(LOADING)
 
+    # Frames in flight, as the reference's Pipeline keeps several (its
+    # --streams): slot k owns a context and a stream for the statistics + CEM
+    # chain, another pair for SAM, and its own score and detection buffers.
+    # Frames go to slots round robin; a slot's streams keep its frames in
+    # order, and different slots' frames overlap (one slot's single-CU build
+    # runs beside another's moments). With one slot, SAM of a frame starts
+    # only after the previous frame finished, so frames do not overlap at all.
+    inflight = max(1, args.in_flight)
+    slots = []
+    for k in range(inflight):
+        if k == 0:
+            c, cs, a, b_ = ctx, ctx_sam, main_stream, side
+            bufs = (sam_s, sam_t, cem_s, cem_t, sam_d, cem_d, sam_n, cem_n)
+        else:
+            c, cs = H.Context(0), H.Context(0)
+            for x in (c, cs):
+                x.set_library(library)
+                x.reserve(pixels, B_)
+            a, b_ = torch.cuda.Stream(), torch.cuda.Stream()
+            bufs = (torch.empty(pixels, **f32), torch.empty(pixels, **i32), torch.empty(pixels, **f32),
+                    torch.empty(pixels, **i32), torch.empty((cap, 4), **i32), torch.empty((cap, 4), **i32),
+                    torch.empty(1, **i32), torch.empty(1, **i32))
+        slots.append((c, cs, a, b_, bufs))
+
     def step(i):
         cube = frames[i % len(frames)]
-        # stream B may start only once stream A's previous frame is done with
-        # the shared score buffers of the previous step
-        side.wait_stream(main_stream)
-        ctx_sam.score(H.Detector.SAM, cube, H_, W_, B_, sam_s, sam_t, stream=side)
-        ctx_sam.detect(sam_s, sam_t, H_, W_, p_sam, sam_d, sam_n, stream=side)
-        ctx.stats_accumulate(cube, H_, W_, B_)
-        ctx.stats_build(LOADING)
-        ctx.score(H.Detector.CEM, cube, H_, W_, B_, cem_s, cem_t)
-        ctx.detect(cem_s, cem_t, H_, W_, p_cem, cem_d, cem_n)
-        main_stream.wait_stream(side)
+        c, cs, a, b_, (ss, st, cms, cmt, sd, cd, sn, cn) = slots[i % inflight]
+        if inflight == 1:
+            b_.wait_stream(a)
+        cs.score(H.Detector.SAM, cube, H_, W_, B_, ss, st, stream=b_)
+        cs.detect(ss, st, H_, W_, p_sam, sd, sn, stream=b_)
+        c.stats_accumulate(cube, H_, W_, B_, stream=a)
+        c.stats_build(LOADING, stream=a)
+        c.score(H.Detector.CEM, cube, H_, W_, B_, cms, cmt, stream=a)
+        c.detect(cms, cmt, H_, W_, p_cem, cd, cn, stream=a)
+        if inflight == 1:
+            a.wait_stream(b_)
 
     step(0)
     ctx.stats_check()
@@ -522,7 +549,8 @@ def main():
         step(i)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ctx.stats_check()
+    for c, *_ in slots:
+        c.stats_check()
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = pixels * args.steps / (t1 - t0) / 1e6
     log(f"step {ms_step:.3f} ms, {value:.1f} Mpix/s")
@@ -621,7 +649,7 @@ def main():
         "data": "synthetic (seeded generator, frames resident in HBM)",
         "config": {"workload": "plain FP32 video frame: moments+build+CEM+detect || SAM+detect",
                    "height": H_, "width": W_, "bands": B_, "targets": T_,
-                   "resident_frames": args.frames, "loading": LOADING},
+                   "resident_frames": args.frames, "frames_in_flight": max(1, args.in_flight), "loading": LOADING},
         "roofline": roof,
         "cpu_baseline": cpu,
         "detectors": detectors,
