@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--in-flight", type=int, default=3,
                     help="frames in flight (slots of two streams and two contexts each); 3 is the reference Pipeline default")
     ap.add_argument("--repeats", type=int, default=20, help="per-kernel timing repeats")
+    ap.add_argument("--separate-sam", action="store_true",
+                    help="score SAM on its own stream (before the statistics) instead of fused with CEM")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=1024, help="rows of a frame in the CPU sample")
     ap.add_argument("--no-batch", action="store_true",
@@ -525,15 +527,21 @@ def main():
     def step(i):
         cube = frames[i % len(frames)]
         c, cs, a, b_, (ss, st, cms, cmt, sd, cd, sn, cn) = slots[i % inflight]
-        if inflight == 1:
-            b_.wait_stream(a)
-        cs.score(H.Detector.SAM, cube, H_, W_, B_, ss, st, stream=b_)
-        cs.detect(ss, st, H_, W_, p_sam, sd, sn, stream=b_)
+        if args.separate_sam:
+            if inflight == 1:
+                b_.wait_stream(a)
+            cs.score(H.Detector.SAM, cube, H_, W_, B_, ss, st, stream=b_)
+            cs.detect(ss, st, H_, W_, p_sam, sd, sn, stream=b_)
         c.stats_accumulate(cube, H_, W_, B_, stream=a)
         c.stats_build(LOADING, stream=a)
-        c.score(H.Detector.CEM, cube, H_, W_, B_, cms, cmt, stream=a)
+        if args.separate_sam:
+            c.score(H.Detector.CEM, cube, H_, W_, B_, cms, cmt, stream=a)
+        else:
+            # SAM and CEM from one read of the cube once the statistics exist
+            c.score_sam_cem(cube, H_, W_, B_, ss, st, cms, cmt, stream=a)
+            c.detect(ss, st, H_, W_, p_sam, sd, sn, stream=a)
         c.detect(cms, cmt, H_, W_, p_cem, cd, cn, stream=a)
-        if inflight == 1:
+        if args.separate_sam and inflight == 1:
             a.wait_stream(b_)
 
     step(0)
@@ -647,7 +655,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 in, f64 accumulate",
         "data": "synthetic (seeded generator, frames resident in HBM)",
-        "config": {"workload": "plain FP32 video frame: moments+build+CEM+detect || SAM+detect",
+        "config": {"workload": "plain FP32 video frame: moments+build+fused SAM/CEM+detect x2, frames in flight" if not args.separate_sam else "plain FP32 video frame: moments+build+CEM+detect || SAM+detect, frames in flight",
                    "height": H_, "width": W_, "bands": B_, "targets": T_,
                    "resident_frames": args.frames, "frames_in_flight": max(1, args.in_flight), "loading": LOADING},
         "roofline": roof,

@@ -31,7 +31,7 @@ namespace hsi_amd {
 namespace {
 
 constexpr int kChunk = 64;       // pixel granularity of the workgroup ranges
-constexpr int kMaxMomWgs = 256;
+constexpr int kMaxMomWgs = 512;  // most workgroups of any launch (sizes the scratch)
 
 __host__ __device__ constexpr int pair_row(int q) {
   int i = 0;
@@ -107,14 +107,18 @@ __device__ __forceinline__ float4 stage_load(const float* __restrict__ cube, std
 }
 
 // Launch shape of each form, per block count NB. Eight waves per workgroup
-// (two per SIMD) let one wave's LDS traffic overlap another's MFMAs; chunks
-// of 64 pixels where LDS allows keep ~a chunk of loads in flight per CU.
+// let one wave's LDS traffic overlap another's MFMAs. The plain form stages
+// 32-pixel chunks, so up to 9 staged blocks two workgroups share a CU (four
+// waves per SIMD; 468 -> 381 us at 113 bands against one workgroup with
+// 64-pixel chunks). The mixed form stages 128-pixel chunks up to 8 blocks.
 // Beyond 12 blocks the mixed form splits a pixel range's block pairs over two
 // workgroups (eight ids apart, so the dispatcher puts both on one XCD and the
 // second read of each chunk hits that XCD's L2) to fit its accumulators.
 constexpr int kWavesPerWg = 8;
 constexpr int kXcds = 8;
-template <int NB> constexpr int f64_chunk() { return NB <= 8 ? 64 : 32; }
+template <int NB> constexpr int f64_chunk() { return 32; }
+template <int NBS> constexpr int f64_max_wgs() { return NBS <= 9 ? 512 : 256; }
+constexpr int kMixedMaxWgs = 256;
 template <int NB> constexpr int f16_chunk() { return NB <= 8 ? 128 : 64; }
 template <int NB> constexpr int f16_splits() { return NB > 12 ? 2 : 1; }
 
@@ -574,9 +578,9 @@ __global__ void unshift_sum_kernel(int bands, const double* __restrict__ r,
   if (i < bands) sum[i] = sum[i] + *count * r[i];
 }
 
-std::size_t mom_wgs(std::size_t pixels) {
+std::size_t mom_wgs(std::size_t pixels, int max_wgs = kMixedMaxWgs) {
   const std::size_t chunks = div_up(pixels, kChunk);
-  return chunks < kMaxMomWgs ? chunks : kMaxMomWgs;
+  return chunks < static_cast<std::size_t>(max_wgs) ? chunks : static_cast<std::size_t>(max_wgs);
 }
 
 template <int NB>
@@ -623,7 +627,7 @@ hipError_t launch_f64_nbm(const float* cube, std::size_t pixels, int bands, std:
                           int vec_rows, double* partial, double* count, double* sum, double* sum_outer,
                           hipStream_t stream) {
   const std::size_t chunks = div_up(pixels, kChunk);
-  const std::size_t wgs = mom_wgs(pixels);
+  const std::size_t wgs = mom_wgs(pixels, f64_max_wgs<NBM + 1>());
   const std::size_t px_per_wg = div_up(chunks, wgs) * kChunk;
   const std::size_t used = div_up(pixels, px_per_wg);
   hipLaunchKernelGGL(moments_f64_kernel<NBM>, dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64), 0,
@@ -660,7 +664,7 @@ std::size_t moments_scratch_doubles(std::size_t pixels, int bands) {
   // mixed form: partials, then a double and a float shift vector; plain
   // form: partials of block pairs and the vector part
   const std::size_t mixed = mom_wgs(pixels) * static_cast<std::size_t>(moment_pairs(bands)) * 256 + 2 * kMaxBands;
-  const std::size_t plain = mom_wgs(pixels) * f64_partial_doubles(f64_split(bands).nbm);
+  const std::size_t plain = mom_wgs(pixels, kMaxMomWgs) * f64_partial_doubles(f64_split(bands).nbm);
   return mixed > plain ? mixed : plain;
 }
 
