@@ -166,11 +166,18 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
   d4 acc[PW];
 #pragma unroll
   for (int m = 0; m < PW; ++m) acc[m] = d4{0.0, 0.0, 0.0, 0.0};
-  // Vector part: wave-uniform choice of the waves that own columns.
-  const bool vec_wave = W * 64 < bands;
-  const int vj = W * 64 + lane < bands ? W * 64 + lane : bands - 1;
-  double vsum = 0.0;
-  double vrow[kMaxVecRows] = {0.0, 0.0};
+  // Vector part, spread over all eight waves so none holds the barrier
+  // back: lane (g, c) of wave W takes columns 16W + c + 128k (k < KC) and
+  // pixels g, g + 4, ... of each chunk; the four pixel phases are summed by
+  // shuffles at the end.
+  constexpr int KC = (16 * NBS + 127) / 128;        // column rounds (bands + ones <= 16 NBS)
+  const int vg = lane >> 4;
+  double vsum[KC], vrow[KC][kMaxVecRows];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    vsum[k] = 0.0;
+    vrow[k][0] = vrow[k][1] = 0.0;
+  }
   const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
 
   float4 pre[PER];
@@ -217,19 +224,20 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
       mfma_run_f64<run_first(POS, NP, kWavesPerWg), NP, NBM, kWavesPerWg, POS>(op, acc);
     }
     // The matrix cores drain this wave's last pairs (and serve the other
-    // wave of the SIMD) while the vector part runs.
-    if (vec_wave) {
-      if (vec_rows == 0) {
-#pragma unroll 8
-        for (int p = 0; p < CH; ++p) vsum += t[p * LDB + vj];
-      } else {
-#pragma unroll 4
-        for (int p = 0; p < CH; ++p) {
-          const double* px = t + p * LDB;
-          const double xj = px[vj];
-          vsum += xj;
-          vrow[0] = __builtin_fma(px[16 * NBM], xj, vrow[0]);
-          vrow[1] = __builtin_fma(px[16 * NBM + 1], xj, vrow[1]);   // zero row when vec_rows == 1
+    // waves of the SIMD) while the vector part runs. Columns past the staged
+    // rows read row 0 (their sums are never written).
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int col = 16 * W + (lane & 15) + 128 * k;
+      const int cj = col < 16 * NBS ? col : 0;
+#pragma unroll
+      for (int p = vg; p < CH; p += 4) {
+        const double* px = t + p * LDB;
+        const double xj = px[cj];
+        vsum[k] += xj;
+        if (vec_rows > 0) {
+          vrow[k][0] = __builtin_fma(px[16 * NBM], xj, vrow[k][0]);
+          vrow[k][1] = __builtin_fma(px[16 * NBM + 1], xj, vrow[k][1]);   // zero row when vec_rows == 1
         }
       }
     }
@@ -250,12 +258,22 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
       dst[3] = acc[m][3];
     }
   }
-  if (vec_wave && W * 64 + lane < bands) {
-    double* v = out + NP * 256;
-    v[vj] = vsum;
+  double* v = out + NP * 256;
 #pragma unroll
-    for (int r = 0; r < kMaxVecRows; ++r)
-      if (r < vec_rows) v[(1 + r) * VC + vj] = vrow[r];
+  for (int k = 0; k < KC; ++k) {
+    double a = vsum[k], r0 = vrow[k][0], r1 = vrow[k][1];
+    a += __shfl_xor(a, 16, 64);
+    a += __shfl_xor(a, 32, 64);
+    r0 += __shfl_xor(r0, 16, 64);
+    r0 += __shfl_xor(r0, 32, 64);
+    r1 += __shfl_xor(r1, 16, 64);
+    r1 += __shfl_xor(r1, 32, 64);
+    const int col = 16 * W + (lane & 15) + 128 * k;
+    if (vg == 0 && col < bands) {
+      v[col] = a;
+      if (vec_rows > 0) v[VC + col] = r0;
+      if (vec_rows > 1) v[2 * VC + col] = r1;
+    }
   }
 }
 
