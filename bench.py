@@ -341,6 +341,10 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
     m = ps._buffers(h * w)
     logit = m["logit"]
     ms["infer"] = time_on_stream(torch, lambda: ctx.mlp_infer(cube, h, w, b, logit, center=True), repeats)
+    # SAM and CEM alone on the same 2048x2048x113 frame (8 targets)
+    s_map, t_map = m["ace"], m["target"]
+    ms["sam"] = time_on_stream(torch, lambda: ctx.score(H.Detector.SAM, cube, h, w, b, s_map, t_map), repeats)
+    ms["cem"] = time_on_stream(torch, lambda: ctx.score(H.Detector.CEM, cube, h, w, b, s_map, t_map), repeats)
     ctx.close()
     log("  learned: " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in ms.items()))
     rec = {d: min(e[d]["recall"] for e in evals) for d in ("rx", "ace")}
@@ -361,6 +365,10 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
         "speedup_path_with_statistics": round((ms["stats_plain"] + ms["plain_score"]) /
                                               (ms["stats_mixed"] + ms["learned_score"]), 3),
         "infer_hbm_gb_s": round(b * 4 * pix / (ms["infer"] / 1e3) / 1e9, 1),
+        "sam_cem_2048": {k: {"us": round(ms[k] * 1e3, 1),
+                             "hbm_gb_s": round((b * 4 + 8) * pix / (ms[k] / 1e3) / 1e9, 1),
+                             "hbm_frac": round((b * 4 + 8) * pix / (ms[k] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+                         for k in ("sam", "cem")},
     }
 
 

@@ -27,11 +27,11 @@ template <typename In, int PPT>
 struct VecOf;
 template <>
 struct VecOf<float, 4> {
-  using type = float4;
+  typedef float type __attribute__((ext_vector_type(4)));
 };
 template <>
 struct VecOf<float, 2> {
-  using type = float2;
+  typedef float type __attribute__((ext_vector_type(2)));
 };
 template <>
 struct VecOf<_Float16, 4> {
@@ -136,16 +136,18 @@ __device__ __forceinline__ void sam_cem_group(
   }
 
   const In* base = cube + p0;
-  // Band planes in flight per lane: 8 for CEM and the fused form (CEM alone
-  // 88 -> 80 us at 1024x1024x113, fused 132 -> 125 us); SAM alone, with its
-  // heavier epilogue and more registers, stays faster at 4.
-  constexpr int U = (DO_SAM && !DO_CEM) ? 4 : 8;
+  // Band planes in flight per lane: 16 for SAM alone (two pixels per lane),
+  // 8 for CEM and the fused form (CEM alone 88 -> 80 us at 1024x1024x113,
+  // fused 132 -> 125 us).
+  constexpr int U = (DO_SAM && !DO_CEM) ? 16 : 8;
   int b = 0;
   for (; b + U <= bands; b += U) {
     V v[U];
+    // Non-temporal: the cube is read once, so it need not displace other
+    // lines in L2 and the last-level cache.
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      v[u] = *reinterpret_cast<const V*>(base + static_cast<std::size_t>(b + u) * stride);
+      v[u] = __builtin_nontemporal_load(reinterpret_cast<const V*>(base + static_cast<std::size_t>(b + u) * stride));
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       double x[PPT];
@@ -298,10 +300,13 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
     const double* __restrict__ srnorm, int exact_all,
     const double* __restrict__ clib, int num_targets, float* __restrict__ s_score,
     int* __restrict__ s_target, float* __restrict__ c_score, int* __restrict__ c_target) {
-  const std::size_t p0 = (blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x) * PPT;
-  if (p0 >= pixels) return;
-  sam_cem_group<In, NT, PPT, DO_SAM, DO_CEM>(cube, pixels, bands, stride, slib, snorm, srnorm, exact_all, clib,
-                                            num_targets, s_score, s_target, c_score, c_target, p0);
+  // Grid-stride over pixel groups: a grid of at most four workgroups per CU
+  // keeps every CU streaming to the end of the frame.
+  const std::size_t step = static_cast<std::size_t>(gridDim.x) * blockDim.x * PPT;
+  for (std::size_t p0 = (blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x) * PPT; p0 < pixels;
+       p0 += step)
+    sam_cem_group<In, NT, PPT, DO_SAM, DO_CEM>(cube, pixels, bands, stride, slib, snorm, srnorm, exact_all, clib,
+                                              num_targets, s_score, s_target, c_score, c_target, p0);
 }
 
 // Batched scenes of one target-count class (library stride NT) in one
@@ -330,11 +335,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
   }
 }
 
+// Four workgroups per CU on 256 CUs: measured on a 2048x2048x113 frame, a
+// grid of this size streams at 5.6 TB/s where one workgroup per pixel group
+// block (4096) reaches 5.1.
+constexpr std::size_t kSamMaxWgs = 1024;
+
 template <typename In, int NT, int PPT, bool S, bool Cm>
 hipError_t launch_one(const SamCemArgs& a, hipStream_t stream) {
   const std::size_t groups = div_up(a.pixels, PPT);
-  const unsigned blocks = static_cast<unsigned>(div_up(groups, 256));
-  hipLaunchKernelGGL((sam_cem_kernel<In, NT, PPT, S, Cm>), dim3(blocks), dim3(256), 0, stream,
+  std::size_t blocks = div_up(groups, 256);
+  if (blocks > kSamMaxWgs) blocks = kSamMaxWgs;
+  hipLaunchKernelGGL((sam_cem_kernel<In, NT, PPT, S, Cm>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream,
                      static_cast<const In*>(a.cube), a.pixels, a.bands, a.stride, a.sam_lib,
                      a.sam_norms, a.sam_rnorms, a.sam_exact_all, a.cem_lib, a.num_targets,
                      a.sam_score, a.sam_target,
@@ -351,7 +362,10 @@ hipError_t launch_mode(const SamCemArgs& a, hipStream_t stream) {
     return e != hipSuccess ? e : launch_one<In, NT, PPT, false, true>(a, stream);
   }
   if (s && c) return launch_one<In, NT, PPT, true, true>(a, stream);
-  if (s) return launch_one<In, NT, PPT, true, false>(a, stream);
+  // SAM alone: two pixels per lane and sixteen planes in flight (its
+  // epilogue keeps a wave off the memory system for a while, so each wave
+  // needs more loads outstanding; 2048x2048x113: 423 -> 367 us)
+  if (s) return launch_one<In, NT, 2, true, false>(a, stream);
   return launch_one<In, NT, PPT, false, true>(a, stream);
 }
 
