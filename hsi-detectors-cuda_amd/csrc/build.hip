@@ -92,6 +92,16 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Lane `src`'s value (src uniform across the wave) through v_readlane into
+// scalar registers: a few cycles, against a round trip through LDS for a
+// shuffle, on the serial pivot chain below.
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u & 0xffffffffu), src));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), src));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // Factor the 16x16 diagonal tile and invert it, in one wave. Lane i < 16
 // holds row i; returns false if a pivot fails. Writes L_KK into `lt` and
 // L_KK^-1 into `xt` (both swizzled, upper triangle zero).
@@ -105,7 +115,7 @@ __device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_do
   const double tol = 4.0 * 256 * 2.220446049250313e-16;
 #pragma unroll
   for (int j = 0; j < kTile; ++j) {
-    const double pivot = __shfl(row[j], j, kWave);
+    const double pivot = lane_value(row[j], j);
     const double d0 = diag0[j];
     if (!(pivot > 0.0) || !(pivot > tol * d0)) ok = false;
     const double d = ok ? sqrt(pivot) : 1.0;
@@ -115,7 +125,7 @@ __device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_do
     const double lij = row[j];
 #pragma unroll
     for (int k = j + 1; k < kTile; ++k) {
-      const double lkj = __shfl(lij, k, kWave);
+      const double lkj = lane_value(lij, k);
       if (i >= k) row[k] -= lij * lkj;
     }
   }
@@ -141,7 +151,7 @@ __device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_do
     const double lij = lt[sw(i, j)];
 #pragma unroll
     for (int c = 0; c <= j; ++c) {
-      const double xjc = __shfl(x[c], j, kWave);
+      const double xjc = lane_value(x[c], j);
       if (i > j) x[c] -= lij * xjc;
     }
   }
