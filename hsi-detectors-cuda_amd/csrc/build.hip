@@ -102,12 +102,26 @@ __device__ __forceinline__ double lane_value(double v, int src) {
   return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
-// Factor the 16x16 diagonal tile and invert it, in one wave. Lane i < 16
-// holds row i; returns false if a pivot fails. Writes L_KK into `lt` and
-// L_KK^-1 into `xt` (both swizzled, upper triangle zero).
+// 1/sqrt(p) to within an ulp or so: the hardware estimate and one Newton
+// step (its relative error squares, from ~1e-15 to rounding level).
+__device__ __forceinline__ double rsqrt_refined(double p) {
+  const double r = __builtin_amdgcn_rsq(p);
+  return r * __builtin_fma(-0.5 * p * r, r, 1.5);
+}
+
+// Factor the 16x16 diagonal tile and invert it, in one wave; both halves are
+// serial chains, so each step is kept short.
+//   Cholesky: lane i < 16 holds row i; pivot j's value and L[k][j] reach the
+//   other lanes through v_readlane; sqrt and the division by the pivot come
+//   from one refined reciprocal square root (d = p / sqrt(p) = p * r).
+//   Inverse: lane c < 16 computes column c of L^-1 by forward substitution,
+//   reading L's rows as LDS broadcasts, so no lane waits on another.
+// Returns false if a pivot fails. Writes L_KK into `lt` and L_KK^-1 into
+// `xt` (both swizzled, upper triangle zero).
 template <typename P>
 __device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_double* diag0, int lane) {
   double row[kTile];
+  double rinv[kTile];
   const int i = lane & 15;
 #pragma unroll
   for (int c = 0; c < kTile; ++c) row[c] = (lane < 16 && c <= i) ? lt[sw(i, c)] : 0.0;
@@ -118,10 +132,10 @@ __device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_do
     const double pivot = lane_value(row[j], j);
     const double d0 = diag0[j];
     if (!(pivot > 0.0) || !(pivot > tol * d0)) ok = false;
-    const double d = ok ? sqrt(pivot) : 1.0;
-    const double rd = 1.0 / d;
-    if (i == j) row[j] = d;
-    if (i > j) row[j] = row[j] * rd;
+    const double r = ok ? rsqrt_refined(pivot) : 1.0;
+    rinv[j] = r;
+    if (i == j) row[j] = ok ? pivot * r : 1.0;
+    if (i > j) row[j] = row[j] * r;
     const double lij = row[j];
 #pragma unroll
     for (int k = j + 1; k < kTile; ++k) {
@@ -134,48 +148,55 @@ __device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_do
     for (int c = 0; c < kTile; ++c) lt[sw(i, c)] = c <= i ? row[c] : 0.0;
   }
   wave_sync();
-  // X = L^-1, right-looking over rows: row j final, then rows below subtract.
-  // L is re-read from the tile so only one 16-element row is live at a time.
+  // Column c of X = L^-1: X[c][c] = 1/L[c][c]; X[r][c] = -(sum_{k<r} L[r][k] X[k][c]) / L[r][r].
+  const int c = lane & 15;
   double x[kTile];
 #pragma unroll
-  for (int c = 0; c < kTile; ++c) x[c] = 0.0;
+  for (int r = 0; r < kTile; ++r) {
+    double acc = 0.0;
 #pragma unroll
-  for (int j = 0; j < kTile; ++j) {
-    // One reciprocal per pivot; row j of X is then scaled by it (a full
-    // IEEE division per element would cost sixteen per step).
-    const double rjj = 1.0 / lt[sw(j, j)];
-    if (i == j) {
-#pragma unroll
-      for (int c = 0; c < kTile; ++c) x[c] = (c == j) ? rjj : x[c] * rjj;
-    }
-    const double lij = lt[sw(i, j)];
-#pragma unroll
-    for (int c = 0; c <= j; ++c) {
-      const double xjc = lane_value(x[c], j);
-      if (i > j) x[c] -= lij * xjc;
-    }
+    for (int k = 0; k < r; ++k) acc = __builtin_fma(static_cast<double>(lt[sw(r, k)]), x[k], acc);
+    x[r] = r < c ? 0.0 : (r == c ? rinv[r] : -acc * rinv[r]);
   }
   if (lane < 16) {
 #pragma unroll
-    for (int c = 0; c < kTile; ++c) xt[sw(i, c)] = c <= i ? x[c] : 0.0;
+    for (int r = 0; r < kTile; ++r) xt[sw(r, c)] = x[r];
   }
   return ok;
 }
 
 // Blocked Cholesky of A (nt x nt tiles, lower) in place; the inverses of the
 // diagonal tiles go to X's diagonal. Returns false uniformly on failure.
+// Lookahead: in panel K's trailing update wave 0 updates the next diagonal
+// tile first and factors it at once, while the other waves finish the rest
+// of the trailing update, so the serial diagonal factor of K + 1 is off the
+// critical path.
 template <typename P>
 __device__ __attribute__((noinline)) bool block_cholesky(Mat<P> A, Mat<P> X, int nt, const lds_double* diag0,
                                                          __attribute__((address_space(3))) int* flag) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  auto factor = [&](int K) {
+    const bool ok = diag_factor(A.tile(K, K), X.tile(K, K), diag0 + K * kTile, lane);
+    if (lane == 0 && !ok) *flag = 1;
+  };
+  auto update = [&](int K, int q) {
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= q) ++r;
+    const int I = K + 1 + r;
+    const int J = K + 1 + (q - r * (r + 1) / 2);
+    P* aij = A.tile(I, J);
+    const P* lik = A.tile(I, K);
+    const P* ljk = A.tile(J, K);
+    d4 c = load_c(aij, lane);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) c = mfma(-op(lik, lane, s), op(ljk, lane, s), c);
+    store_c(aij, lane, c);
+  };
+  if (wave == 0) factor(0);
+  __syncthreads();
+  if (*flag) return false;
   for (int K = 0; K < nt; ++K) {
-    if (wave == 0) {
-      const bool ok = diag_factor(A.tile(K, K), X.tile(K, K), diag0 + K * kTile, lane);
-      if (lane == 0 && !ok) *flag = 1;
-    }
-    __syncthreads();
-    if (*flag) return false;
     // Panel: L_IK = A_IK * Linv_KK^T.
     const P* xkk = X.tile(K, K);
     for (int I = K + 1 + wave; I < nt; I += kWaves) {
@@ -187,23 +208,20 @@ __device__ __attribute__((noinline)) bool block_cholesky(Mat<P> A, Mat<P> X, int
       store_c(aik, lane, c);
     }
     __syncthreads();
-    // Trailing: A_IJ -= L_IK L_JK^T for K < J <= I.
+    if (K + 1 == nt) break;
+    // Trailing: A_IJ -= L_IK L_JK^T for K < J <= I. Tile q = 0 is the next
+    // diagonal tile (K + 1, K + 1).
     const int rest = nt - K - 1;
     const int count = rest * (rest + 1) / 2;
-    for (int q = wave; q < count; q += kWaves) {
-      int r = 0;
-      while ((r + 1) * (r + 2) / 2 <= q) ++r;
-      const int I = K + 1 + r;
-      const int J = K + 1 + (q - r * (r + 1) / 2);
-      P* aij = A.tile(I, J);
-      const P* lik = A.tile(I, K);
-      const P* ljk = A.tile(J, K);
-      d4 c = load_c(aij, lane);
-#pragma unroll
-      for (int s = 0; s < 4; ++s) c = mfma(-op(lik, lane, s), op(ljk, lane, s), c);
-      store_c(aij, lane, c);
+    if (wave == 0) {
+      update(K, 0);
+      wave_sync();
+      factor(K + 1);
+    } else {
+      for (int q = wave; q < count; q += kWaves - 1) update(K, q);
     }
     __syncthreads();
+    if (*flag) return false;
   }
   return true;
 }
