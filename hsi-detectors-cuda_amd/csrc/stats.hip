@@ -502,8 +502,19 @@ __global__ __launch_bounds__(kRedElems * kRedGroups) void moments_reduce_kernel(
   const int group = threadIdx.x / kRedElems;
   const std::size_t e = static_cast<std::size_t>(blockIdx.x) * kRedElems + local;
   double s = 0.0;
-  if (e < a.per_wg)
-    for (int w = group; w < a.wgs; w += kRedGroups) s += a.partial[static_cast<std::size_t>(w) * a.per_wg + e];
+  if (e < a.per_wg) {
+    // Eight loads in flight per round, summed in the same workgroup order.
+    for (int w0 = group; w0 < a.wgs; w0 += 8 * kRedGroups) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int w = w0 + u * kRedGroups;
+        v[u] = w < a.wgs ? a.partial[static_cast<std::size_t>(w) * a.per_wg + e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+  }
   part[group][local] = s;
   __syncthreads();
   if (group != 0 || e >= a.per_wg) return;
