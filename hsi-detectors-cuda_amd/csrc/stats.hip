@@ -90,19 +90,31 @@ __host__ __device__ constexpr unsigned long long run_blocks() {
   return m;
 }
 
-// float4 of band b at pixel px (px a multiple of 4) for the staging pass: the
-// cube values, the ones band, or zeros past the band count / pixel range.
-__device__ __forceinline__ float4 stage_load(const float* __restrict__ cube, std::size_t stride,
-                                             int bands, int b, std::size_t px, std::size_t p_end,
-                                             bool ones_band = true) {
-  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (px < p_end) {
-    if (b < bands) v = *reinterpret_cast<const float4*>(cube + static_cast<std::size_t>(b) * stride + px);
-    else if (ones_band && b == bands) v = make_float4(1.f, 1.f, 1.f, 1.f);
-    if (px + 1 >= p_end) v.y = 0.f;
-    if (px + 2 >= p_end) v.z = 0.f;
-    if (px + 3 >= p_end) v.w = 0.f;
+// Staging loads, in two steps. stage_raw issues the float4 load of band b at
+// pixel px (a multiple of 4) from an address clamped into the cube - band
+// below `bands`, pixel inside [p_begin, p_end) - with no branch around it, so
+// every load of a chunk is in flight at once and the wait for them comes
+// where the values are first used. (A load inside a bounds branch, with its
+// value edited in the same branch, makes the compiler wait for each load
+// before issuing the next.) stage_fix then substitutes what lies outside:
+// the ones band, or zeros past the band count and the pixel range.
+__device__ __forceinline__ float4 stage_raw(const float* __restrict__ cube, std::size_t stride, int bands,
+                                            int b, std::size_t px, std::size_t p_begin, std::size_t p_end) {
+  const int bc = b < bands ? b : bands - 1;
+  const std::size_t pc = px < p_end ? px : p_begin;
+  return *reinterpret_cast<const float4*>(cube + static_cast<std::size_t>(bc) * stride + pc);
+}
+
+__device__ __forceinline__ float4 stage_fix(float4 v, int bands, int b, std::size_t px, std::size_t p_end,
+                                            bool ones_band) {
+  if (b >= bands) {
+    const float o = ones_band && b == bands ? 1.f : 0.f;
+    v = make_float4(o, o, o, o);
   }
+  if (px >= p_end) v.x = 0.f;
+  if (px + 1 >= p_end) v.y = 0.f;
+  if (px + 2 >= p_end) v.z = 0.f;
+  if (px + 3 >= p_end) v.w = 0.f;
   return v;
 }
 
@@ -186,31 +198,32 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + THREADS * k;
       const int b = e / (CH / 4);
-      pre[k] = e < LOADS ? stage_load(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_end, false)
-                         : make_float4(0.f, 0.f, 0.f, 0.f);
+      pre[k] = stage_raw(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_begin, p_end);
     }
   };
-  auto store = [&](double* t) {
+  auto store = [&](double* t, std::size_t p) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + THREADS * k;
       if (e < LOADS) {
         const int b = e / (CH / 4);
         const int k4 = (e % (CH / 4)) * 4;
-        t[(k4 + 0) * LDB + b] = pre[k].x;
-        t[(k4 + 1) * LDB + b] = pre[k].y;
-        t[(k4 + 2) * LDB + b] = pre[k].z;
-        t[(k4 + 3) * LDB + b] = pre[k].w;
+        const float4 v = stage_fix(pre[k], bands, b, p + k4, p_end, false);
+        t[(k4 + 0) * LDB + b] = v.x;
+        t[(k4 + 1) * LDB + b] = v.y;
+        t[(k4 + 2) * LDB + b] = v.z;
+        t[(k4 + 3) * LDB + b] = v.w;
       }
     }
   };
 
   load(p_begin);
-  store(tiles);
+  store(tiles, p_begin);
   __syncthreads();
   for (int c = 0; c < chunks; ++c) {
     const bool more = c + 1 < chunks;
-    if (more) load(p_begin + static_cast<std::size_t>(c + 1) * CH);
+    const std::size_t pn = p_begin + static_cast<std::size_t>(c + 1) * CH;
+    if (more) load(pn);
     const double* t = tiles + (c & 1) * TILE;
 #pragma unroll 2
     for (int s = 0; s < CH / 4; ++s) {
@@ -241,7 +254,7 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
         }
       }
     }
-    if (more) store(tiles + ((c + 1) & 1) * TILE);
+    if (more) store(tiles + ((c + 1) & 1) * TILE, pn);
     __syncthreads();
   }
 
@@ -339,8 +352,7 @@ __device__ __forceinline__ void moments_mixed_owner(const float* __restrict__ cu
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + THREADS * k;
       const int b = e / (CH / 4);
-      pre[k] = e < LOADS ? stage_load(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_end)
-                         : make_float4(0.f, 0.f, 0.f, 0.f);
+      pre[k] = stage_raw(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_begin, p_end);
     }
   };
   auto store = [&](int buf, std::size_t p) {
@@ -353,7 +365,8 @@ __device__ __forceinline__ void moments_mixed_owner(const float* __restrict__ cu
         const std::size_t px = p + k4;
         // Shift real bands only, and only real pixels (padding stays zero).
         const float s = b < bands ? sh[b] : 0.f;
-        float v[4] = {pre[k].x, pre[k].y, pre[k].z, pre[k].w};
+        const float4 f = stage_fix(pre[k], bands, b, px, p_end, true);
+        float v[4] = {f.x, f.y, f.z, f.w};
         h4 hv, lv;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {

@@ -102,9 +102,14 @@ hsi_status hsi_ctx_reserve(hsi_ctx* ctx, size_t max_pixels, int max_bands);
 
 /* ---- geometry ----------------------------------------------------------- */
 
-/* Plane stride in elements of a device BSQ cube: pixels rounded up to a
- * multiple of 8, so every band plane starts 16-byte aligned for FP32 and
- * FP16 alike. Replaces bsq_plane_stride (sam_cuda.hpp:41-52, sam.cu:338-343). */
+/* Plane stride in elements of a device BSQ cube. Below 65,536 pixels: the
+ * pixel count rounded up to a multiple of 8, so every band plane starts
+ * 16-byte aligned for FP32 and FP16 alike. From 65,536 pixels on: rounded up
+ * to an odd multiple of 1024 elements, so consecutive planes start on
+ * different HBM channels (a power-of-two plane size would put every plane on
+ * the same one). Any stride >= height*width that is a multiple of 8 is
+ * accepted by the scoring entry points; this one is the fastest to read.
+ * Replaces bsq_plane_stride (sam_cuda.hpp:41-52, sam.cu:338-343). */
 size_t hsi_plane_stride(int height, int width);
 
 /* ---- target library ----------------------------------------------------- */

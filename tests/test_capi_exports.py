@@ -44,10 +44,23 @@ def test_library_exports_every_declared_symbol():
 
 def test_plane_stride_pads_to_eight():
     import hsi_mi355x
-    assert hsi_mi355x.plane_stride(512, 512) == 512 * 512
     assert hsi_mi355x.plane_stride(13, 37) == 488
     assert hsi_mi355x.plane_stride(1, 1) == 8
+    assert hsi_mi355x.plane_stride(128, 256) == 128 * 256
     assert hsi_mi355x.plane_stride(0, 5) == 0
+
+
+def test_plane_stride_spreads_large_planes_over_channels():
+    import hsi_mi355x
+    # power-of-two planes get one extra 4 KiB page
+    assert hsi_mi355x.plane_stride(512, 512) == 512 * 512 + 1024
+    assert hsi_mi355x.plane_stride(2048, 2048) == 2048 * 2048 + 1024
+    # already an odd number of pages after rounding up
+    assert hsi_mi355x.plane_stride(1000, 1000) == 977 * 1024
+    for h, w in [(256, 256), (1024, 1024), (2048, 1088), (1000, 1000), (333, 777), (4096, 1024)]:
+        s = hsi_mi355x.plane_stride(h, w)
+        assert s >= h * w and s % 1024 == 0 and (s // 1024) % 2 == 1
+        assert s - h * w < 2048
 
 
 def test_rx_threshold_matches_oracle_and_closed_form(oracle_lib):
