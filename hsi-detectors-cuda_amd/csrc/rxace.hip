@@ -112,7 +112,10 @@ __device__ __forceinline__ void rxace_tiles(
   // flight while the matrix cores work on this one.
   // With candidate spectra, element i of the list reads row i of spectra
   // (its bands contiguous) instead of its pixel in the cube.
-  auto load_tile = [&](std::size_t t, float (&xr)[NB][4]) {
+  // The loads are unconditional (an out-of-range element reads element 0 and
+  // is zeroed at use), so the whole tile is in flight at once and the raw
+  // values are converted only where they are used.
+  auto load_tile = [&](std::size_t t, In (&xr)[NB][4], bool& ok) {
     bool pix_ok;
     const std::size_t pp = pixel_of(t, pix_ok);
     const In* src = cube;
@@ -124,16 +127,19 @@ __device__ __forceinline__ void rxace_tiles(
         sp = (t * 16 + n) * gd.spectra_stride;
       }
     }
+    ok = pix_ok;
 #pragma unroll
     for (int J = 0; J < NB; ++J)
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         const int b = 16 * J + 4 * kk + g;
-        xr[J][kk] = (pix_ok && b < bands) ? static_cast<float>(src[static_cast<std::size_t>(b) * src_stride + sp]) : 0.f;
+        const std::size_t off = (pix_ok && b < bands) ? static_cast<std::size_t>(b) * src_stride + sp : 0;
+        xr[J][kk] = src[off];
       }
   };
-  float cur[NB][4], nxt[NB][4];
-  if (wave_id < tiles) load_tile(wave_id, cur);
+  In cur[NB][4], nxt[NB][4];
+  bool cur_ok = false, nxt_ok = false;
+  if (wave_id < tiles) load_tile(wave_id, cur, cur_ok);
 
   for (std::size_t tile = wave_id; tile < tiles; tile += waves) {
     // Read the whitener blocks from LDS at each use rather than hoisting them
@@ -141,7 +147,7 @@ __device__ __forceinline__ void rxace_tiles(
     __asm__ volatile("" ::: "memory");
     bool live;
     const std::size_t p = pixel_of(tile, live);
-    if (tile + waves < tiles) load_tile(tile + waves, nxt);
+    if (tile + waves < tiles) load_tile(tile + waves, nxt, nxt_ok);
     // Z = W Y accumulated one input band block J at a time.
     d4 z[NB];
 #pragma unroll
@@ -153,7 +159,8 @@ __device__ __forceinline__ void rxace_tiles(
       for (int kk = 0; kk < 4; ++kk) {
         const int b = 16 * J + 4 * kk + g;
         // padded bands and pixels read 0; their whitener columns are 0 too
-        y[kk] = (b < bands) ? static_cast<double>(cur[J][kk]) - static_cast<double>(mean_lds[b]) : 0.0;
+        const double x = cur_ok ? static_cast<double>(static_cast<float>(cur[J][kk])) : 0.0;
+        y[kk] = (b < bands) ? x - static_cast<double>(mean_lds[b]) : 0.0;
       }
 #pragma unroll
       for (int I = J; I < NB; ++I) {
@@ -170,6 +177,7 @@ __device__ __forceinline__ void rxace_tiles(
     for (int J = 0; J < NB; ++J)
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) cur[J][kk] = nxt[J][kk];
+    cur_ok = nxt_ok;
 
     double zz = 0.0;
     d4 zu[TT > 0 ? TT : 1];

@@ -116,6 +116,10 @@ __global__ __launch_bounds__(mixed_waves(NB) * 64) void rxace_mixed_kernel(
   __shared__ float mean_lds[NKB * 32];
   __shared__ float scale_lds[NKB * 32];
   __shared__ float rinv_lds[NB * 16];
+  // Per-target constants of the ACE epilogue, read per lane there: from LDS
+  // they cost no global load per tile.
+  __shared__ float tn2_lds[(TT > 0 ? TT : 1) * 16];
+  __shared__ float tinv_lds[(TT > 0 ? TT : 1) * 16];
 
   {
     const h8* src = reinterpret_cast<const h8*>(white_hl);
@@ -129,6 +133,11 @@ __global__ __launch_bounds__(mixed_waves(NB) * 64) void rxace_mixed_kernel(
       scale_lds[e] = e < bands ? col_scale[e] : 0.f;
     }
     for (int e = threadIdx.x; e < NB * 16; e += blockDim.x) rinv_lds[e] = row_inv[e];
+    if constexpr (TT > 0)
+      for (int e = threadIdx.x; e < TT * 16; e += blockDim.x) {
+        tn2_lds[e] = e < num_targets ? target_norm2[e] : 0.f;
+        tinv_lds[e] = e < num_targets ? target_inv[e] : 0.f;
+      }
   }
   __syncthreads();
 
@@ -139,9 +148,6 @@ __global__ __launch_bounds__(mixed_waves(NB) * 64) void rxace_mixed_kernel(
   const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(WAVES) + (threadIdx.x >> 6);
   const std::size_t waves = static_cast<std::size_t>(gridDim.x) * WAVES;
 
-  // Raw inputs of one K block of a 16-pixel tile: band 32J + 8g + i of pixel
-  // tile*16 + n. The next K block (of this tile, or block 0 of the next
-  // tile) is in flight while the matrix cores work on the current one.
   // Raw inputs of one K block of a 16-pixel tile: band 32J + 8g + i of pixel
   // tile*16 + n, through a buffer view of the cube. Bands past the last read
   // 0 (beyond the view); pixels past the last read padding or the next
@@ -254,9 +260,9 @@ __global__ __launch_bounds__(mixed_waves(NB) * 64) void rxace_mixed_kernel(
         for (int r = 0; r < 4; ++r) {
           const int t = 16 * tt + 4 * g + r;
           if (t < num_targets) {
-            const double u2 = static_cast<double>(target_norm2[t]);
+            const double u2 = static_cast<double>(tn2_lds[t]);
             if (u2 > 0.0 && zz > 0.0) {
-              const double zu_t = static_cast<double>(zu[tt][r]) * target_inv[t];
+              const double zu_t = static_cast<double>(zu[tt][r]) * tinv_lds[t];
               const double s = (zu_t * zu_t) / (u2 * zz);
               if (s == s) ace_merge(best, best_t, s, t);
             }
