@@ -159,6 +159,46 @@ __host__ __device__ inline std::size_t f64_partial_doubles(int nbm) {
   return static_cast<std::size_t>(nbm) * (nbm + 1) / 2 * 256 + (1 + kMaxVecRows) * 16 * (nbm + 1);
 }
 
+// Column blocks of the vector part, one per (round k, wave w) slot
+// k * kWavesPerWg + w (-1: none). A wave preferably takes a block that its
+// own matrix-core run already loads, so the value is in a register; the
+// blocks left over (always the last, which holds the leftover bands) are
+// read from LDS.
+constexpr unsigned long long run_blocks_of(int np, int owners, int pos) {
+  unsigned long long m = 0;
+  for (int q = run_first(pos, np, owners); q < run_first(pos + 1, np, owners); ++q)
+    m |= (1ull << pair_row(q)) | (1ull << pair_col(q));
+  return m;
+}
+template <int KC>
+struct VecBlocks {
+  int blk[kWavesPerWg * KC];
+};
+template <int NBM, int KC>
+__host__ __device__ constexpr VecBlocks<KC> vec_blocks() {
+  constexpr int NBS = NBM + 1, NP = NBM * (NBM + 1) / 2;
+  VecBlocks<KC> r{};
+  bool taken[NBS] = {};
+  for (int i = 0; i < kWavesPerWg * KC; ++i) r.blk[i] = -1;
+  for (int k = 0; k < KC; ++k)
+    for (int w = 0; w < kWavesPerWg; ++w) {
+      const unsigned long long used = run_blocks_of(NP, kWavesPerWg, wave_position(w));
+      for (int b = 0; b < NBM; ++b)
+        if (((used >> b) & 1) && !taken[b]) {
+          r.blk[k * kWavesPerWg + w] = b;
+          taken[b] = true;
+          break;
+        }
+    }
+  for (int b = 0; b < NBS; ++b)
+    for (int i = 0; i < kWavesPerWg * KC && !taken[b]; ++i)
+      if (r.blk[i] < 0) {
+        r.blk[i] = b;
+        taken[b] = true;
+      }
+  return r;
+}
+
 template <int NBM, int W, int POS>
 __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube, std::size_t p_begin,
                                                  std::size_t p_end, int bands, std::size_t stride,
@@ -179,10 +219,12 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
 #pragma unroll
   for (int m = 0; m < PW; ++m) acc[m] = d4{0.0, 0.0, 0.0, 0.0};
   // Vector part, spread over all eight waves so none holds the barrier
-  // back: lane (g, c) of wave W takes columns 16W + c + 128k (k < KC) and
-  // pixels g, g + 4, ... of each chunk; the four pixel phases are summed by
-  // shuffles at the end.
+  // back: lane (g, c) of wave W takes column c of block vbs.blk[8k + W]
+  // (k < KC) and, at each 4-pixel step, the pixel whose matrix-core operands
+  // it has just read (4s + g), so the column value is mostly already in a
+  // register; the four pixel phases are summed by shuffles at the end.
   constexpr int KC = (16 * NBS + 127) / 128;        // column rounds (bands + ones <= 16 NBS)
+  constexpr VecBlocks<KC> vbs = vec_blocks<NBM, KC>();
   const int vg = lane >> 4;
   double vsum[KC], vrow[KC][kMaxVecRows];
 #pragma unroll
@@ -225,7 +267,10 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
     const std::size_t pn = p_begin + static_cast<std::size_t>(c + 1) * CH;
     if (more) load(pn);
     const double* t = tiles + (c & 1) * TILE;
-#pragma unroll 2
+    // Fully unrolled up to 7 blocks; beyond that two steps at a time keep the
+    // registers within the occupancy the launch needs.
+    constexpr int kStepUnroll = NBM <= 7 ? CH / 4 : 2;
+#pragma unroll kStepUnroll
     for (int s = 0; s < CH / 4; ++s) {
       // v_mfma_f64_16x16x4 operands: lane l supplies row (l % 16) of the
       // 16-band block at pixel (l / 16) of this 4-pixel step, for A and B.
@@ -235,22 +280,18 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
 #pragma unroll
       for (int i = 0; i < NBM; ++i) op[i] = (used >> i) & 1 ? row[16 * i] : 0.0;
       mfma_run_f64<run_first(POS, NP, kWavesPerWg), NP, NBM, kWavesPerWg, POS>(op, acc);
-    }
-    // The matrix cores drain this wave's last pairs (and serve the other
-    // waves of the SIMD) while the vector part runs. Columns past the staged
-    // rows read row 0 (their sums are never written).
+      // Vector part of this pixel while the matrix cores work.
 #pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      const int col = 16 * W + (lane & 15) + 128 * k;
-      const int cj = col < 16 * NBS ? col : 0;
-#pragma unroll
-      for (int p = vg; p < CH; p += 4) {
-        const double* px = t + p * LDB;
-        const double xj = px[cj];
-        vsum[k] += xj;
-        if (vec_rows > 0) {
-          vrow[k][0] = __builtin_fma(px[16 * NBM], xj, vrow[k][0]);
-          vrow[k][1] = __builtin_fma(px[16 * NBM + 1], xj, vrow[k][1]);   // zero row when vec_rows == 1
+      for (int k = 0; k < KC; ++k) {
+        const int vb = vbs.blk[k * kWavesPerWg + W];
+        if (vb >= 0) {
+          const double xj = (vb < NBM && ((used >> vb) & 1)) ? op[vb < NBM ? vb : 0] : row[16 * vb];
+          vsum[k] += xj;
+          if (vec_rows > 0) {
+            const double* lr = row - (lane & 15) + 16 * NBM;  // leftover bands of this pixel
+            vrow[k][0] = __builtin_fma(lr[0], xj, vrow[k][0]);
+            if (vec_rows > 1) vrow[k][1] = __builtin_fma(lr[1], xj, vrow[k][1]);
+          }
         }
       }
     }
@@ -281,8 +322,9 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
     r0 += __shfl_xor(r0, 32, 64);
     r1 += __shfl_xor(r1, 16, 64);
     r1 += __shfl_xor(r1, 32, 64);
-    const int col = 16 * W + (lane & 15) + 128 * k;
-    if (vg == 0 && col < bands) {
+    const int vb = vbs.blk[k * kWavesPerWg + W];
+    const int col = 16 * vb + (lane & 15);
+    if (vb >= 0 && vg == 0 && col < bands) {
       v[col] = a;
       if (vec_rows > 0) v[VC + col] = r0;
       if (vec_rows > 1) v[2 * VC + col] = r1;
