@@ -315,6 +315,24 @@ def test_detect_edge_maps(H, torch, ctx, case):
     assert np.array_equal(np.stack([dets["x"], dets["y"], dets["target"]], 1).reshape(-1, 3), want)
 
 
+@pytest.mark.parametrize("nms,higher", [(1, True), (2, False), (3, True), (5, False)])
+def test_detect_large_ragged_map(H, torch, ctx, oracle_lib, nms, higher):
+    """A 1000 x 1037 map: 1013 workgroup spans of 1024 pixels, the last one
+    ragged. Scores are quantised to 1/64 so plateaus of equal neighbours
+    exercise the raster-order tie rule; about 5% of pixels pass."""
+    rng = np.random.default_rng(11 + nms)
+    h, w = 1000, 1037
+    score = (np.round(rng.random((h, w)) * 64) / 64).astype(np.float32)
+    target = rng.integers(0, 8, size=h * w).astype(np.int32)
+    thr = 0.97 if higher else 0.03
+    want, found_cpu = oracle_lib.detect(score, target, h, w, thr, nms, higher)
+    dets, found = run_detect(H, torch, ctx, score, target, h, w, thr, nms, higher)
+    assert found == found_cpu > 1000
+    for k in ("x", "y", "target"):
+        assert np.array_equal(dets[k], want[k])
+    assert same_bits(dets["score"], want["score"])
+
+
 def test_detect_saturates_without_overrun(H, torch, ctx, oracle_lib):
     rng = np.random.default_rng(3)
     h, w = 300, 500
