@@ -136,10 +136,10 @@ __device__ __forceinline__ void sam_cem_group(
   }
 
   const In* base = cube + p0;
-  // Band planes in flight per lane: 16 for SAM alone (two pixels per lane),
-  // 8 for CEM and the fused form (CEM alone 88 -> 80 us at 1024x1024x113,
-  // fused 132 -> 125 us).
-  constexpr int U = (DO_SAM && !DO_CEM) ? 16 : 8;
+  // Band planes in flight per lane: for SAM alone 12 at four pixels per lane
+  // and 16 at two, 8 for CEM and the fused form (CEM alone 88 -> 80 us at
+  // 1024x1024x113, fused 132 -> 125 us).
+  constexpr int U = (DO_SAM && !DO_CEM) ? (PPT == 4 ? 12 : 16) : 8;
   int b = 0;
   for (; b + U <= bands; b += U) {
     V v[U];
@@ -337,14 +337,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
 
 // Four workgroups per CU on 256 CUs: measured on a 2048x2048x113 frame, a
 // grid of this size streams at 5.6 TB/s where one workgroup per pixel group
-// block (4096) reaches 5.1.
+// block (4096) reaches 5.1. SAM alone at four pixels per lane holds three
+// waves per SIMD at 8 targets (155 VGPRs), so its grid there is three
+// workgroups per CU.
 constexpr std::size_t kSamMaxWgs = 1024;
+constexpr std::size_t kSamAloneWgs = 768;
 
 template <typename In, int NT, int PPT, bool S, bool Cm>
-hipError_t launch_one(const SamCemArgs& a, hipStream_t stream) {
+hipError_t launch_one(const SamCemArgs& a, hipStream_t stream, std::size_t max_wgs = kSamMaxWgs) {
   const std::size_t groups = div_up(a.pixels, PPT);
   std::size_t blocks = div_up(groups, 256);
-  if (blocks > kSamMaxWgs) blocks = kSamMaxWgs;
+  if (blocks > max_wgs) blocks = max_wgs;
   hipLaunchKernelGGL((sam_cem_kernel<In, NT, PPT, S, Cm>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream,
                      static_cast<const In*>(a.cube), a.pixels, a.bands, a.stride, a.sam_lib,
                      a.sam_norms, a.sam_rnorms, a.sam_exact_all, a.cem_lib, a.num_targets,
@@ -362,10 +365,16 @@ hipError_t launch_mode(const SamCemArgs& a, hipStream_t stream) {
     return e != hipSuccess ? e : launch_one<In, NT, PPT, false, true>(a, stream);
   }
   if (s && c) return launch_one<In, NT, PPT, true, true>(a, stream);
-  // SAM alone: two pixels per lane and sixteen planes in flight (its
-  // epilogue keeps a wave off the memory system for a while, so each wave
-  // needs more loads outstanding; 2048x2048x113: 423 -> 367 us)
-  if (s) return launch_one<In, NT, 2, true, false>(a, stream);
+  // SAM alone keeps more planes in flight per wave than CEM (its epilogue
+  // keeps a wave off the memory system for a while): four pixels per lane
+  // and twelve planes up to 8 targets (2048x2048x113 frame: 345 -> 327 us
+  // against two pixels and sixteen planes), two pixels and sixteen planes
+  // for 16 and 32 targets, whose accumulators need the registers.
+  if (s) {
+    if constexpr (NT <= 8)
+      return launch_one<In, NT, 4, true, false>(a, stream, NT == 8 ? kSamAloneWgs : kSamMaxWgs);
+    return launch_one<In, NT, 2, true, false>(a, stream);
+  }
   return launch_one<In, NT, PPT, false, true>(a, stream);
 }
 
