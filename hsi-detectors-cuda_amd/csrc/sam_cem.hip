@@ -309,6 +309,22 @@ __global__ __launch_bounds__(256) void sam_cem_kernel(
                                               num_targets, s_score, s_target, c_score, c_target, p0);
 }
 
+// The fused form of up to 8 targets at two pixels per lane, held to 128
+// VGPRs so its waves fit beside another frame's moments waves on a CU.
+template <typename In, int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void sam_cem_fused_kernel(
+    const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
+    const double* __restrict__ slib, const float* __restrict__ snorm,
+    const double* __restrict__ srnorm, int exact_all,
+    const double* __restrict__ clib, int num_targets, float* __restrict__ s_score,
+    int* __restrict__ s_target, float* __restrict__ c_score, int* __restrict__ c_target) {
+  const std::size_t step = static_cast<std::size_t>(gridDim.x) * blockDim.x * 2;
+  for (std::size_t p0 = (blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x) * 2; p0 < pixels;
+       p0 += step)
+    sam_cem_group<In, NT, 2, true, true>(cube, pixels, bands, stride, slib, snorm, srnorm, exact_all, clib,
+                                        num_targets, s_score, s_target, c_score, c_target, p0);
+}
+
 // Batched scenes of one target-count class (library stride NT) in one
 // persistent launch: the scenes' 256*PPT-pixel blocks are numbered in scene
 // order (block_begin prefix), and each workgroup takes every gridDim.x-th
@@ -364,7 +380,18 @@ hipError_t launch_mode(const SamCemArgs& a, hipStream_t stream) {
     const hipError_t e = launch_one<In, NT, PPT, true, false>(a, stream);
     return e != hipSuccess ? e : launch_one<In, NT, PPT, false, true>(a, stream);
   }
-  if (s && c) return launch_one<In, NT, PPT, true, true>(a, stream);
+  if (s && c) {
+    if constexpr (NT <= 8) {
+      std::size_t blocks = div_up(div_up(a.pixels, 2), 256);
+      if (blocks > kSamMaxWgs) blocks = kSamMaxWgs;
+      hipLaunchKernelGGL((sam_cem_fused_kernel<In, NT>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream,
+                         static_cast<const In*>(a.cube), a.pixels, a.bands, a.stride, a.sam_lib, a.sam_norms,
+                         a.sam_rnorms, a.sam_exact_all, a.cem_lib, a.num_targets, a.sam_score, a.sam_target,
+                         a.cem_score, a.cem_target);
+      return hipGetLastError();
+    }
+    return launch_one<In, NT, PPT, true, true>(a, stream);
+  }
   // SAM alone keeps more planes in flight per wave than CEM (its epilogue
   // keeps a wave off the memory system for a while): four pixels per lane
   // and twelve planes up to 8 targets (2048x2048x113 frame: 345 -> 327 us
