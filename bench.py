@@ -5,13 +5,15 @@ Workload (the plain-FP32 video configuration): frames of 1024x1024 pixels x
 113 bands, BSQ fp32, drawn by the seeded generator (Dirichlet mix of 8
 endmembers, per-frame drift of the mix, 8-target library planted at 0.5%
 each, gain U(0.3,1.2), noise N(0,0.005)). One step = one frame through the
-whole per-frame path, with the frame already resident in HBM, on two streams:
-    stream A: moments (FP64 matrix cores) -> statistics build (FP64)
-              -> CEM -> threshold+NMS+compact
-    stream B: SAM -> threshold+NMS+compact
-SAM needs no scene statistics, so it streams the cube through HBM while the
-moments keep the matrix cores busy. `value` is frames x pixels / wall time of
-the timed steps (Mpix/s).
+whole per-frame path, with the frame already resident in HBM:
+    moments (FP64 matrix cores) -> statistics build (FP64)
+    -> fused SAM+CEM (one read of the cube) -> two threshold+NMS+compact
+Three frames are in flight (a context and a stream per slot, as the
+reference Pipeline's default of three), so one frame's HBM-bound SAM+CEM and
+single-CU build overlap the next frame's moments; --in-flight 1 gives the
+serial form and --separate-sam scores SAM on a second stream before the
+statistics. `value` is frames x pixels / wall time of the timed steps
+(Mpix/s).
 
 Per-kernel rates come from HIP events on the launch stream, averaged over
 repeats, and the roofline entry is for the kernel that takes the largest
@@ -46,8 +48,8 @@ H_, W_, B_, T_ = 1024, 1024, 113, 8
 # KiB x 2.0, WRITE KiB x 1.0); the runs and the table are recorded in
 # profiles/hbm_traffic_1024x1024x113.md.
 MEASURED_TRAFFIC_BYTES = {
-    "moments_f64_kernel": 231595.6 * 2.0 * 1024 + 29600.0 * 1024,
-    "sam_cem_kernel": 232005.3 * 2.0 * 1024 + 16384.0 * 1024,
+    "moments_f64_kernel": 231619.3 * 2.0 * 1024 + 29600.0 * 1024,
+    "sam_cem_fused_kernel": 232821.4 * 2.0 * 1024 + 18432.0 * 1024,
 }
 LOADING = 1e-6
 SAM_THRESHOLD = 0.10
@@ -620,7 +622,7 @@ def main():
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "per_unit": f"{flops_px['moments']} FP64 flop/pixel (B(B+1)), {pixels} pixels/launch"}
     elif dom == "sam_cem_fused":
-        roof = {"kernel": "sam_cem_kernel", "bound": "hbm",
+        roof = {"kernel": "sam_cem_fused_kernel", "bound": "hbm",
                 "achieved": round(bytes_px["sam_cem_fused"] * pixels / (ms["sam_cem_fused"] / 1e3) / 1e9, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "per_unit": f"{bytes_px['sam_cem_fused']} B/pixel, {pixels} pixels/launch"}
