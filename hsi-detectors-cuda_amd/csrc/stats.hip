@@ -199,6 +199,20 @@ __host__ __device__ constexpr VecBlocks<KC> vec_blocks() {
   return r;
 }
 
+// Every staged block 0 .. NBM is the vector part's block of exactly one slot.
+template <int NBM, int KC>
+__host__ __device__ constexpr bool vec_blocks_complete() {
+  constexpr VecBlocks<KC> r = vec_blocks<NBM, KC>();
+  for (int b = 0; b <= NBM; ++b) {
+    int n = 0;
+    for (int i = 0; i < kWavesPerWg * KC; ++i) n += r.blk[i] == b ? 1 : 0;
+    if (n != 1) return false;
+  }
+  for (int i = 0; i < kWavesPerWg * KC; ++i)
+    if (r.blk[i] < -1 || r.blk[i] > NBM) return false;
+  return true;
+}
+
 template <int NBM, int W, int POS>
 __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube, std::size_t p_begin,
                                                  std::size_t p_end, int bands, std::size_t stride,
@@ -225,6 +239,7 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
   // register; the four pixel phases are summed by shuffles at the end.
   constexpr int KC = (16 * NBS + 127) / 128;        // column rounds (bands + ones <= 16 NBS)
   constexpr VecBlocks<KC> vbs = vec_blocks<NBM, KC>();
+  static_assert(vec_blocks_complete<NBM, KC>(), "vector-part column blocks must cover every staged block once");
   const int vg = lane >> 4;
   double vsum[KC], vrow[KC][kMaxVecRows];
 #pragma unroll
