@@ -85,6 +85,31 @@ __device__ __forceinline__ bool passes(const float* __restrict__ score, std::siz
   return local_best<R>(score, p, width, height, radius, self, higher);
 }
 
+// Suppression test against a workgroup's neighbourhood staged in LDS: row
+// dy + R of `nb` holds the linear run p0 + dy * width - R .. p0 + dy *
+// width + kDetBlock - 1 + R, so pixel p0 + t finds neighbour (dy, dx) at
+// nb[dy + R][t + R + dx]. A run may wrap across image rows; the coordinate
+// test rejects every neighbour that is not really inside the image.
+template <int R>
+__device__ __forceinline__ bool local_best_lds(const float (*nb)[kDetBlock + 2 * R], int t, std::size_t p, int width,
+                                               int height, float self, bool higher) {
+  const int x = static_cast<int>(p % static_cast<std::size_t>(width));
+  const int y = static_cast<int>(p / static_cast<std::size_t>(width));
+  bool ok = true;
+#pragma unroll
+  for (int dy = -R; dy <= R; ++dy)
+#pragma unroll
+    for (int dx = -R; dx <= R; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      const int ny = y + dy, nx = x + dx;
+      const bool inside = ny >= 0 && ny < height && nx >= 0 && nx < width;
+      const float other = nb[dy + R][t + R + dx];
+      const bool before = dy < 0 || (dy == 0 && dx < 0);
+      if (inside && (better(other, self, higher) || (other == self && before))) ok = false;
+    }
+  return ok;
+}
+
 template <int R>
 __global__ __launch_bounds__(kDetBlock) void detect_flag_kernel(
     const float* __restrict__ score, int width, int height, float threshold, int radius,
@@ -92,7 +117,26 @@ __global__ __launch_bounds__(kDetBlock) void detect_flag_kernel(
   __shared__ unsigned wave_counts[kDetBlock / kWave];
   const std::size_t pixels = static_cast<std::size_t>(width) * height;
   const std::size_t p = blockIdx.x * static_cast<std::size_t>(kDetBlock) + threadIdx.x;
-  const bool pass = p < pixels && passes<R>(score, p, width, height, threshold, radius, higher != 0);
+  bool pass;
+  if constexpr (R > 0) {
+    // The (2R + 1) runs of the workgroup's neighbourhood, loaded coalesced
+    // (about 2R + 1 loads per thread instead of (2R + 1)^2 per candidate).
+    constexpr int RUN = kDetBlock + 2 * R;
+    __shared__ float nb[2 * R + 1][RUN];
+    const long long p0 = static_cast<long long>(blockIdx.x) * kDetBlock;
+    for (int e = threadIdx.x; e < (2 * R + 1) * RUN; e += kDetBlock) {
+      const int row = e / RUN, col = e % RUN;
+      const long long q = p0 + static_cast<long long>(row - R) * width + (col - R);
+      nb[row][col] = (q >= 0 && q < static_cast<long long>(pixels)) ? score[q] : 0.f;
+    }
+    __syncthreads();
+    const float self = nb[R][threadIdx.x + R];
+    const bool hi = higher != 0;
+    pass = p < pixels && (hi ? self >= threshold : self <= threshold) &&
+           local_best_lds<R>(nb, threadIdx.x, p, width, height, self, hi);
+  } else {
+    pass = p < pixels && passes<R>(score, p, width, height, threshold, radius, higher != 0);
+  }
   const unsigned long long ballot = __ballot(pass);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
