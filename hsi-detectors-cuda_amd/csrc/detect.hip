@@ -164,16 +164,29 @@ __global__ __launch_bounds__(kScanThreads) void detect_scan_kernel(
     const unsigned b = begin + i;
     if (b < blocks) local += block_counts[b];
   }
-  partial[threadIdx.x] = local;
-  __syncthreads();
-  // Hillis-Steele inclusive scan over the thread sums.
-  for (unsigned off = 1; off < kScanThreads; off <<= 1) {
-    const unsigned v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0u;
-    __syncthreads();
-    partial[threadIdx.x] += v;
-    __syncthreads();
+  // Inclusive scan of the thread sums: within each wave by shuffles, then
+  // over the wave totals by the first wave (two barriers in all).
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned v = local;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const unsigned u = __shfl_up(v, off, kWave);
+    if (lane >= off) v += u;
   }
-  unsigned run = partial[threadIdx.x] - local;
+  if (lane == kWave - 1) partial[wave] = v;
+  __syncthreads();
+  if (wave == 0) {
+    unsigned w = lane < kScanThreads / kWave ? partial[lane] : 0u;
+#pragma unroll
+    for (int off = 1; off < kScanThreads / kWave; off <<= 1) {
+      const unsigned u = __shfl_up(w, off, kWave);
+      if (lane >= off) w += u;
+    }
+    if (lane < kScanThreads / kWave) partial[lane] = w;
+  }
+  __syncthreads();
+  const unsigned inclusive = v + (wave > 0 ? partial[wave - 1] : 0u);
+  unsigned run = inclusive - local;
   for (unsigned i = 0; i < per; ++i) {
     const unsigned b = begin + i;
     if (b < blocks) {
@@ -181,7 +194,7 @@ __global__ __launch_bounds__(kScanThreads) void detect_scan_kernel(
       run += block_counts[b];
     }
   }
-  if (threadIdx.x == kScanThreads - 1) *total_out = partial[threadIdx.x];
+  if (threadIdx.x == kScanThreads - 1) *total_out = inclusive;
 }
 
 __global__ __launch_bounds__(kDetBlock) void detect_write_kernel(
