@@ -122,7 +122,10 @@ __device__ __forceinline__ float4 stage_fix(float4 v, int bands, int b, std::siz
 // let one wave's LDS traffic overlap another's MFMAs. The plain form stages
 // 32-pixel chunks, so up to 9 staged blocks two workgroups share a CU (four
 // waves per SIMD; 468 -> 381 us at 113 bands against one workgroup with
-// 64-pixel chunks). The mixed form stages 128-pixel chunks up to 8 blocks.
+// 64-pixel chunks). The mixed form stages 64-pixel chunks; up to 8 blocks
+// its registers are held to 128 (amdgpu_waves_per_eu(4)), so two workgroups
+// share a CU (1024x1024x113: 143 -> 136 us against one workgroup with
+// 128-pixel chunks and 197 VGPRs).
 // Beyond 12 blocks the mixed form splits a pixel range's block pairs over two
 // workgroups (eight ids apart, so the dispatcher puts both on one XCD and the
 // second read of each chunk hits that XCD's L2) to fit its accumulators.
@@ -130,8 +133,9 @@ constexpr int kWavesPerWg = 8;
 constexpr int kXcds = 8;
 template <int NB> constexpr int f64_chunk() { return 32; }
 template <int NBS> constexpr int f64_max_wgs() { return NBS <= 9 ? 512 : 256; }
-constexpr int kMixedMaxWgs = 256;
-template <int NB> constexpr int f16_chunk() { return NB <= 8 ? 128 : 64; }
+constexpr int kMixedMaxWgs = 512;  // most mixed-form workgroups (sizes the scratch)
+template <int NB> constexpr int f16_max_wgs() { return NB <= 8 ? 512 : 256; }
+template <int NB> constexpr int f16_chunk() { return 64; }
 template <int NB> constexpr int f16_splits() { return NB > 12 ? 2 : 1; }
 
 // Plain-form split of the work. The matrix cores take the block pairs of the
@@ -493,7 +497,7 @@ __device__ __forceinline__ void moments_mixed_owner(const float* __restrict__ cu
 }
 
 template <int NB>
-__global__ __launch_bounds__(kWavesPerWg * 64) void moments_mixed_kernel(const float* __restrict__ cube,
+__global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_eu(4))) void moments_mixed_kernel(const float* __restrict__ cube,
                                                                         std::size_t pixels, int bands,
                                                                         std::size_t stride,
                                                                         std::size_t px_per_wg,
@@ -687,7 +691,7 @@ hipError_t launch_mixed_nb(const float* cube, std::size_t pixels, int bands, std
                            double* scratch, double* count, double* sum, double* sum_outer,
                            hipStream_t stream) {
   const std::size_t chunks = div_up(pixels, kChunk);
-  const std::size_t wgs = mom_wgs(pixels);
+  const std::size_t wgs = mom_wgs(pixels, f16_max_wgs<NB>());
   const std::size_t px_per_wg = div_up(chunks, wgs) * kChunk;
   const std::size_t used = div_up(pixels, px_per_wg);
   const int np = NB * (NB + 1) / 2;
