@@ -91,6 +91,48 @@ def test_sam_fp16_cube_matches_oracle_on_the_same_halves(H, torch, ctx, oracle_l
     assert (buf["target"].cpu().numpy() == want_t).mean() > 0.999
 
 
+def test_sam_full_frame_against_oracle(H, torch, ctx, oracle_lib):
+    """A 1021 x 1031 x 113 frame of the seeded generator with 8 targets
+    (1,052,651 pixels: more than one sweep of the SAM grid, and a ragged last
+    four-pixel group) against the oracle's sam_best; the fused SAM+CEM launch
+    equals the separate ones bit for bit at this size."""
+    h, w, b, t = 1021, 1031, 113, 8
+    cfg = H.synth_config(h, w, b, t, seed=20261203 + 77)
+    host = H.synth_frame(cfg, 0, threads=8)
+    lib = np.asarray(H.synth_library(cfg), np.float32)
+    p = h * w
+    cube = torch.from_numpy(host).cuda()
+    ctx.set_library(lib)
+    sam = device_buffers(torch, p)
+    ctx.score(H.Detector.SAM, cube, h, w, b, sam["score"], sam["target"])
+    torch.cuda.synchronize()
+    got_s, got_t = sam["score"].cpu().numpy(), sam["target"].cpu().numpy()
+    want_s, want_t = oracle_lib.sam_best(host, lib, pixels=p, threads=8)
+    err = np.abs(got_s.astype(np.float64) - want_s.astype(np.float64))
+    assert err.max() <= ANGLE_TOL, f"max angle error {err.max()}"
+    diff = np.nonzero(got_t != want_t)[0]
+    if diff.size:
+        # a different winner only where its angle ties the oracle's best
+        allv = oracle_lib.sam_all(np.ascontiguousarray(host[:, diff]), lib)
+        gap = np.abs(allv[got_t[diff], np.arange(diff.size)].astype(np.float64) - want_s[diff])
+        assert gap.max() <= ANGLE_TOL
+    exact = int((got_s.view(np.uint32) == want_s.view(np.uint32)).sum())
+    print(f"full frame: SAM max |d angle| {err.max():.3g} rad, bit-identical {exact}/{p}, "
+          f"different winners {diff.size}")
+    assert exact >= 0.999 * p
+    ctx.stats_accumulate(cube, h, w, b)
+    ctx.stats_build(1e-6)
+    ctx.stats_check()
+    fused_s, fused_c = device_buffers(torch, p), device_buffers(torch, p)
+    ctx.score_sam_cem(cube, h, w, b, fused_s["score"], fused_s["target"], fused_c["score"], fused_c["target"])
+    cem = device_buffers(torch, p)
+    ctx.score(H.Detector.CEM, cube, h, w, b, cem["score"], cem["target"])
+    torch.cuda.synchronize()
+    for x, y in ((fused_s, sam), (fused_c, cem)):
+        assert same_bits(x["score"].cpu().numpy(), y["score"].cpu().numpy())
+        assert np.array_equal(x["target"].cpu().numpy(), y["target"].cpu().numpy())
+
+
 def test_sam_edge_cases(H, torch, ctx):
     g = golden("edge_cases")
     cube = H.to_device_cube(g["edge_cube"], 1, 8)
@@ -317,9 +359,10 @@ def test_detect_edge_maps(H, torch, ctx, case):
 
 @pytest.mark.parametrize("nms,higher", [(1, True), (2, False), (3, True), (5, False)])
 def test_detect_large_ragged_map(H, torch, ctx, oracle_lib, nms, higher):
-    """A 1000 x 1037 map: 1013 workgroup spans of 1024 pixels, the last one
-    ragged. Scores are quantised to 1/64 so plateaus of equal neighbours
-    exercise the raster-order tie rule; about 5% of pixels pass."""
+    """A 1000 x 1037 map: 4051 workgroups of 256 pixels, the last one
+    ragged (200 pixels), with neighbourhood runs that wrap across image rows.
+    Scores are quantised to 1/64 so plateaus of equal neighbours exercise the
+    raster-order tie rule; about 5% of pixels pass."""
     rng = np.random.default_rng(11 + nms)
     h, w = 1000, 1037
     score = (np.round(rng.random((h, w)) * 64) / 64).astype(np.float32)
