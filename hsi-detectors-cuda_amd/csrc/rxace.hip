@@ -250,8 +250,12 @@ __device__ __forceinline__ void rxace_stage(const float* __restrict__ white_bloc
   for (int e = threadIdx.x; e < NB * 16; e += blockDim.x) mean_lds[e] = e < bands ? mean[e] : 0.f;
 }
 
+// Up to 8 row blocks the registers are held to 256 so two waves share a
+// SIMD (the unconstrained allocation took 259 with accumulation registers).
+__host__ __device__ constexpr int plain_min_waves(int NB) { return NB <= 8 ? 2 : 1; }
+
 template <typename In, int NB, int TT, bool GATHER>
-__global__ __launch_bounds__(kRxWaves * 64) void rxace_plain_kernel(
+__global__ __launch_bounds__(kRxWaves * 64) __attribute__((amdgpu_waves_per_eu(plain_min_waves(NB)))) void rxace_plain_kernel(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
     int num_targets, const float* __restrict__ mean, const float* __restrict__ white_blocks,
     const float* __restrict__ ace_u, const float* __restrict__ target_norm2,
@@ -321,9 +325,10 @@ hipError_t launch_plain_t(const RxAceArgs& a, const Gather& gth, hipStream_t str
   // its capacity and let the kernel stop at the count.
   const std::size_t tiles = div_up(GATHER ? static_cast<std::size_t>(gth.cap) : a.pixels, 16);
   std::size_t wgs = div_up(tiles, kRxWaves);
-  // A candidate list is usually far shorter than its capacity: one
-  // workgroup per CU keeps the whitener staging per tile scored low.
-  const std::size_t max_wgs = GATHER ? 256 : 1024;
+  // A candidate list is usually far shorter than its capacity: as many
+  // workgroups as are resident at once (two per CU up to 8 row blocks, one
+  // above) keeps the whitener staging per tile scored low.
+  const std::size_t max_wgs = GATHER ? 256 * static_cast<std::size_t>(plain_min_waves(NB)) : 1024;
   if (wgs > max_wgs) wgs = max_wgs;
   if (wgs == 0) wgs = 1;
   hipLaunchKernelGGL((rxace_plain_kernel<In, NB, TT, GATHER>), dim3(static_cast<unsigned>(wgs)),
