@@ -70,8 +70,6 @@ struct hsi_ctx {
   // Screening scratch: per-wave regions of records and spectra, counters.
   void* d_screen_rec = nullptr;
   std::size_t screen_rec_bytes = 0;
-  void* d_screen_spec = nullptr;
-  std::size_t screen_spec_bytes = 0;
   void* d_screen_counts = nullptr;
   std::size_t screen_counts_bytes = 0;
 
@@ -193,7 +191,7 @@ void hsi_ctx_destroy(hsi_ctx* ctx) {
                   ctx->d_ace_hl, ctx->d_col_scale, ctx->d_row_inv, ctx->d_target_inv,
                   ctx->d_mom_scratch, ctx->d_det_scratch, ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v,
                   ctx->d_mlp_grad, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, ctx->d_mlp_act,
-                  ctx->d_positions, ctx->d_screen_rec, ctx->d_screen_spec, ctx->d_screen_counts,
+                  ctx->d_positions, ctx->d_screen_rec, ctx->d_screen_counts,
                   ctx->d_eval};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1038,13 +1036,9 @@ hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int wid
   const std::size_t cap = static_cast<std::size_t>(capacity);
   HSI_HIP(ctx, grow(&ctx->d_screen_rec, &ctx->screen_rec_bytes, (cap ? cap : 1) * sizeof(hsi_detection)),
           "screening scratch");
-  if (d_spectra)
-    HSI_HIP(ctx, grow(&ctx->d_screen_spec, &ctx->screen_spec_bytes, (cap ? cap : 1) * pb * sizeof(float)),
-            "screening scratch");
   HSI_HIP(ctx, grow(&ctx->d_screen_counts, &ctx->screen_counts_bytes, 2 * kScreenMaxWaves * sizeof(unsigned)),
           "screening scratch");
   a.scratch_rec = static_cast<hsi_detection*>(ctx->d_screen_rec);
-  a.scratch_spectra = static_cast<float*>(ctx->d_screen_spec);
   a.scratch_counts = static_cast<unsigned*>(ctx->d_screen_counts);
   HSI_HIP(ctx, launch_mlp_infer(a, s), "pre-screen screening");
   return HSI_OK;

@@ -129,8 +129,9 @@ hipError_t launch_rxace_plain(const RxAceArgs& args, hipStream_t stream);
 // The plain kernel over a candidate list (raster-order detections of a
 // pre-screen): scores only pixels (y * width + x) of the first
 // min(*count, capacity) records, written at their own positions. With
-// spectra non-null, record i's bands are read from row i of spectra
-// (pitch spectra_stride >= bands) instead of the cube.
+// spectra non-null, record i's bands are read from row cand[i].target of
+// spectra (pitch spectra_stride >= bands; a row outside [0, capacity) names
+// no pixel) instead of the cube.
 hipError_t launch_rxace_candidates(const RxAceArgs& args, const hsi_detection* cand, const unsigned* count,
                                    unsigned capacity, int width, const float* spectra,
                                    std::size_t spectra_stride, hipStream_t stream);
@@ -212,10 +213,11 @@ struct MlpInferArgs {
   // Screening (cand non-null, F32 cube): pixels with logit >= cut are
   // listed in cand[0 .. *cand_count) (x, y, 0, logit), in a fixed order
   // (region of the wave that scanned them, then raster order); with spectra
-  // non-null each one's bands go to row `slot` of spectra (pixel-major,
-  // pitch spectra_stride >= 32 * ceil(bands / 32)). A count above cand_cap
+  // non-null each one's bands go to a row of spectra (pixel-major, pitch
+  // spectra_stride >= 32 * ceil(bands / 32), a multiple of 4, cand_cap rows),
+  // the row a wave's region slot, named in the record's target. A count above cand_cap
   // means the list overflowed. The scratch holds the per-wave regions:
-  // cand_cap records, cand_cap spectra rows of 32 * ceil(bands / 32) floats,
+  // cand_cap records,
   // 2 * kScreenMaxWaves counters.
   float cut = 0.f;
   hsi_detection* cand = nullptr;
@@ -225,7 +227,6 @@ struct MlpInferArgs {
   std::size_t spectra_stride = 0;
   int width = 0;
   hsi_detection* scratch_rec = nullptr;
-  float* scratch_spectra = nullptr;
   unsigned* scratch_counts = nullptr;
 };
 constexpr int kScreenMaxWaves = 4096;

@@ -17,6 +17,7 @@
 // Adam kernel updates the masters. The loss is binary cross-entropy on the
 // logit with a weight on positives (recall matters more than precision for a
 // pre-screen).
+#include <climits>
 #include <type_traits>
 
 #include "common.hpp"
@@ -105,14 +106,16 @@ __device__ __forceinline__ f4 load4(__amdgpu_buffer_rsrc_t r, std::uint32_t off,
 // (FP32) per wave. The next K block's loads are in flight while the matrix
 // cores work on this one.
 // Screening outputs, per wave: wave w appends its candidates to its own
-// region (slots [w * rc, w * rc + rc)) of the record and spectra scratch and
-// leaves its count in counts[w] - no atomics, and the order is fixed (wave,
-// then tile, then pixel). A scan and a compaction pass make the list
-// contiguous afterwards.
+// region (slots [w * rc, w * rc + rc)) of the record scratch and of the
+// caller's spectra rows, and leaves its count in counts[w] - no atomics, and
+// the order is fixed (wave, then tile, then pixel). A scan and a compaction
+// pass make the record list contiguous afterwards; the spectra stay where
+// the pass wrote them, each record naming its row in `target`.
 struct ScreenOut {
   float cut;
   hsi_detection* cand;       // region layout
-  float* spectra;            // region layout, pixel-major, pitch 32 * NKB
+  float* spectra;            // nullable: rows in region layout, pitch `pitch`
+  std::size_t pitch;         // floats per spectra row (>= 32 * NKB, multiple of 4)
   unsigned* counts;          // [waves]
   unsigned rc;               // slots per wave
   int width;
@@ -127,7 +130,6 @@ template <int NKB>
 __device__ __forceinline__ void screen_tile(std::size_t pixels, std::size_t tile, const f4& res,
                                             const f4 (&keep)[NKB][8], const ScreenOut& so, std::size_t wave,
                                             unsigned& local) {
-  constexpr int PB = 32 * NKB;
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int n = lane & 15;
@@ -148,7 +150,7 @@ __device__ __forceinline__ void screen_tile(std::size_t pixels, std::size_t tile
       hsi_detection d;
       d.x = static_cast<int>(px % w);
       d.y = static_cast<int>(px / w);
-      d.target = 0;
+      d.target = so.spectra ? static_cast<int>(region + s) : 0;   // the spectrum's row
       d.score = lg;
       so.cand[region + s] = d;
     }
@@ -160,7 +162,7 @@ __device__ __forceinline__ void screen_tile(std::size_t pixels, std::size_t tile
     if ((m >> q) & 1ull) {
       const unsigned s = local + static_cast<unsigned>(__popcll(m & ((1ull << q) - 1ull)));
       if (s < so.rc) {
-        float* dst = so.spectra + (region + s) * PB + 8 * g;
+        float* dst = so.spectra + (region + s) * so.pitch + 8 * g;
 #pragma unroll
         for (int J = 0; J < NKB; ++J) {
           *reinterpret_cast<f4*>(dst + 32 * J) = f4{keep[J][0][u], keep[J][1][u], keep[J][2][u], keep[J][3][u]};
@@ -378,14 +380,12 @@ __global__ __launch_bounds__(kScanThreads) void screen_scan_kernel(const unsigne
 }
 
 // Region layout -> contiguous list: one wave per region copies its records
-// and spectra rows (pitch pb in, out_pitch out) to their final slots.
+// to their final slots (the spectra rows they name stay in place).
 __global__ __launch_bounds__(256) void screen_compact_kernel(const unsigned* __restrict__ counts,
                                                              const unsigned* __restrict__ prefix, int waves,
-                                                             unsigned rc, unsigned cap, int pb,
+                                                             unsigned rc, unsigned cap,
                                                              const hsi_detection* __restrict__ rec_in,
-                                                             const float* __restrict__ spec_in,
-                                                             hsi_detection* __restrict__ rec_out,
-                                                             float* __restrict__ spec_out, std::size_t out_pitch) {
+                                                             hsi_detection* __restrict__ rec_out) {
   const int w = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
   if (w >= waves) return;
   const int lane = threadIdx.x & 63;
@@ -394,14 +394,6 @@ __global__ __launch_bounds__(256) void screen_compact_kernel(const unsigned* __r
   const std::size_t region = static_cast<std::size_t>(w) * rc;
   for (unsigned j = lane; j < c; j += 64)
     if (base + j < cap) rec_out[base + j] = rec_in[region + j];
-  if (!spec_out) return;
-  const int q = pb / 4;   // float4 per row
-  for (unsigned e = lane; e < c * static_cast<unsigned>(q); e += 64) {
-    const unsigned j = e / q, t = e % q;
-    if (base + j < cap)
-      *reinterpret_cast<f4*>(spec_out + (base + j) * out_pitch + 4 * t) =
-          *reinterpret_cast<const f4*>(spec_in + (region + j) * pb + 4 * t);
-  }
 }
 
 template <typename In>
@@ -437,14 +429,15 @@ hipError_t launch_infer_typed(const MlpInferArgs& a, hipStream_t stream) {
   } else {
     const int pb = 32 * nkb;
     if (!a.cand_count || a.width <= 0 || !a.scratch_rec || !a.scratch_counts ||
-        (a.spectra && (a.spectra_stride < static_cast<std::size_t>(pb) || !a.scratch_spectra)))
+        (a.spectra && (a.spectra_stride < static_cast<std::size_t>(pb) || a.spectra_stride % 4 != 0 ||
+                       a.cand_cap > static_cast<unsigned>(INT_MAX))))
       return hipErrorInvalidValue;
     std::size_t wgs = div_up(tiles, static_cast<std::size_t>(4 * kInferWaves));
     if (wgs > kScreenMaxWaves / kInferWaves) wgs = kScreenMaxWaves / kInferWaves;
     if (wgs == 0) wgs = 1;
     const int waves = static_cast<int>(wgs) * kInferWaves;
     const unsigned rc = a.cand_cap / static_cast<unsigned>(waves);
-    ScreenOut so{a.cut, a.scratch_rec, a.spectra ? a.scratch_spectra : nullptr, a.scratch_counts, rc, a.width};
+    ScreenOut so{a.cut, a.scratch_rec, a.spectra, a.spectra_stride, a.scratch_counts, rc, a.width};
     switch (nkb) {
 #define HSI_K(N)                                                                                   \
   case N:                                                                                          \
@@ -464,8 +457,7 @@ hipError_t launch_infer_typed(const MlpInferArgs& a, hipStream_t stream) {
                        a.cand_cap, prefix, a.cand_count);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(screen_compact_kernel, dim3(static_cast<unsigned>(div_up(waves, 4))), dim3(256), 0, stream,
-                       a.scratch_counts, prefix, waves, rc, a.cand_cap, pb, a.scratch_rec, a.scratch_spectra,
-                       a.cand, a.spectra, a.spectra_stride);
+                       a.scratch_counts, prefix, waves, rc, a.cand_cap, a.scratch_rec, a.cand);
     return hipGetLastError();
   }
 }

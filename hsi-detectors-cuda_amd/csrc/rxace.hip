@@ -50,7 +50,7 @@ struct GatherDev {
   const unsigned* count = nullptr;
   unsigned cap = 0;
   int width = 0;
-  const float* spectra = nullptr;     // nullable: candidate spectra, row i = record i
+  const float* spectra = nullptr;     // nullable: candidate spectra, row = the record's target
   std::size_t spectra_stride = 0;
 };
 
@@ -87,6 +87,8 @@ __device__ __forceinline__ void rxace_tiles(
       // a record outside the frame names no pixel
       const std::size_t rows = pixels / static_cast<std::size_t>(gd.width);
       ok = d.x >= 0 && d.x < gd.width && d.y >= 0 && static_cast<std::size_t>(d.y) < rows;
+      // with spectra, neither does one whose row lies outside [0, cap)
+      if (from_spectra) ok = ok && d.target >= 0 && static_cast<unsigned>(d.target) < gd.cap;
       return ok ? static_cast<std::size_t>(d.y) * static_cast<std::size_t>(gd.width) + static_cast<std::size_t>(d.x)
                 : 0;
     } else {
@@ -110,8 +112,9 @@ __device__ __forceinline__ void rxace_tiles(
   // Raw inputs of a 16-pixel tile, all issued at once: band 16J + 4kk + g of
   // pixel tile*16 + n (the B-operand layout). The next tile's loads are in
   // flight while the matrix cores work on this one.
-  // With candidate spectra, element i of the list reads row i of spectra
-  // (its bands contiguous) instead of its pixel in the cube.
+  // With candidate spectra, element i of the list reads the spectra row its
+  // record names (target; bands contiguous) instead of its pixel in the cube;
+  // a row outside [0, cap) names no spectrum and the element is skipped.
   // The loads are unconditional (an out-of-range element reads element 0 and
   // is zeroed at use), so the whole tile is in flight at once and the raw
   // values are converted only where they are used.
@@ -122,9 +125,10 @@ __device__ __forceinline__ void rxace_tiles(
     std::size_t src_stride = stride, sp = pp;
     if constexpr (GATHER && std::is_same_v<In, float>) {
       if (from_spectra) {
+        const int row = pix_ok ? gd.cand[t * 16 + n].target : 0;   // checked by pixel_of
         src = gd.spectra;
         src_stride = 1;
-        sp = (t * 16 + n) * gd.spectra_stride;
+        sp = static_cast<std::size_t>(row) * gd.spectra_stride;
       }
     }
     ok = pix_ok;

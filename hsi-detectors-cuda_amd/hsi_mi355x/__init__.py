@@ -464,7 +464,8 @@ class Context:
     def mlp_screen(self, cube, height, width, bands, cut, candidates, count, spectra=None,
                    logit=None, stream=None):
         """One pass: candidates at the cut (fixed order, not raster) and with
-        `spectra` ([capacity, >= 32 * ceil(bands / 32)] float32) their bands."""
+        `spectra` ([capacity, >= 32 * ceil(bands / 32)] float32) their bands,
+        each in the row its record's target field names."""
         cap = candidates.shape[0]
         self._check(self._lib.hsi_mlp_screen(self.handle, _ptr(cube), height, width, bands,
                                              cube.stride(0), float(cut), _ptr(candidates), _ptr(count),

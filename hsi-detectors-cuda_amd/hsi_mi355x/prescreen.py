@@ -4,9 +4,9 @@ A small network (B -> 64 -> 32 -> 1) is trained on the GPU by the library's
 own forward/backward/Adam kernels to predict which pixels the plain RX/ACE
 detectors will put over threshold. At run time one pass over the frame
 (hsi_mlp_screen) computes the logits, lists the pixels at or above a
-conservative cut and copies their spectra to a compact buffer; only those
+conservative cut and writes their spectra to rows of a buffer; only those
 candidates are rescored exactly
-(hsi_score_candidate_spectra: the plain kernel on the copied spectra, so
+(hsi_score_candidate_spectra: the plain kernel on those rows, so
 their scores are bit-identical to the full-frame path), and the usual
 threshold + NMS runs on the result. The plain full-frame detectors are
 the judge (training labels, recall) and the fallback (a candidate list that

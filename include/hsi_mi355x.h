@@ -304,11 +304,13 @@ hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float
 
 /* Screening in one pass over the cube (F32, centred on the statistics'
  * mean, which must be built for this band count): every pixel whose logit
- * is >= cut is listed in d_candidates[0 .. *d_count) as {x, y, 0, logit}, in
- * a fixed order that is not raster order, and with d_spectra non-null its
- * bands are copied to row i of d_spectra ([capacity][spectra_stride],
+ * is >= cut is listed in d_candidates[0 .. *d_count) as {x, y, row, logit},
+ * in a fixed order that is not raster order. With d_spectra non-null its
+ * bands are written to row `row` of d_spectra ([capacity][spectra_stride],
  * spectra_stride >= 32 * ceil(bands / 32), a multiple of 4, 16-byte
- * aligned rows). The pass splits the capacity
+ * aligned rows; bands past the last read 0) - straight from the pass, with
+ * no copy: the rows are distinct but not in list order (each wave of the
+ * pass owns a block of them); with d_spectra NULL, row is 0. The pass splits the capacity
  * evenly over its waves; if any wave finds more candidates than its share,
  * or the total exceeds `capacity`, *d_count is set above `capacity` (the
  * list overflowed; hsi_score_candidate_spectra then scores the whole frame).
@@ -319,9 +321,10 @@ hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int wid
                           uint32_t* d_count, int capacity, float* d_spectra,
                           size_t spectra_stride, float* d_logit, hsi_stream stream);
 
-/* hsi_score_candidates over the rows that hsi_mlp_screen copied: record i
- * is scored from row i of d_spectra (pitch spectra_stride >= bands) and
- * written at its (x, y). The whole-frame fallback on overflow reads d_cube.
+/* hsi_score_candidates over the rows that hsi_mlp_screen wrote: record i
+ * is scored from row d_candidates[i].target of d_spectra (capacity rows,
+ * pitch spectra_stride >= bands; a record whose row is outside
+ * [0, capacity) is skipped) and written at its (x, y). The whole-frame fallback on overflow reads d_cube.
  * Same results as hsi_score_candidates. Asynchronous. */
 hsi_status hsi_score_candidate_spectra(hsi_ctx* ctx, hsi_detector detector, const float* d_cube,
                                        int height, int width, int bands, size_t plane_stride,

@@ -192,8 +192,9 @@ def test_candidate_rescoring_equals_full_frame(H, det):
 def test_one_pass_screen_matches_logits_and_full_frame(H, name):
     """hsi_mlp_screen's candidates are exactly the pixels whose logit (as
     hsi_mlp_infer computes it) reaches the cut, with those logits and their
-    cube spectra, listed the same way on a second run; rescoring from the
-    copied spectra equals the full-frame plain ACE bit for bit; an
+    cube spectra in the distinct rows their records name, listed the same way
+    on a second run; rescoring from those rows equals the full-frame plain
+    ACE bit for bit; an
     overflowing list falls back to the whole frame."""
     import torch
     from conftest import golden
@@ -228,14 +229,22 @@ def test_one_pass_screen_matches_logits_and_full_frame(H, name):
     rec = cand[:n].cpu().numpy()
     got_px = rec[:, 1].astype(np.int64) * w + rec[:, 0]
     assert np.array_equal(np.sort(got_px), want)
-    assert np.all(rec[:, 2] == 0)
+    rows = rec[:, 2]
+    assert np.all((rows >= 0) & (rows < cap)) and np.unique(rows).size == n
     assert np.array_equal(rec[:, 3].view(np.float32), lg[got_px])
     host = g["cube"].reshape(b, p)
     again = torch.full((cap, 4), -9, **i32)
     ctx.mlp_screen(cube, h, w, b, cut, again, count, spectra)
     torch.cuda.synchronize()
     assert torch.equal(again[:n], cand[:n])
-    assert np.array_equal(spectra[:n, :b].cpu().numpy(), host[:, got_px].T)
+    sp = spectra.cpu().numpy()
+    assert np.array_equal(sp[rows, :b], host[:, got_px].T)
+    assert np.all(sp[rows, b:] == 0)
+    # without spectra the records carry row 0
+    ctx.mlp_screen(cube, h, w, b, cut, again, count)
+    torch.cuda.synchronize()
+    assert np.array_equal(again[:n].cpu().numpy()[:, [0, 1, 3]], rec[:, [0, 1, 3]])
+    assert np.all(again[:n].cpu().numpy()[:, 2] == 0)
 
     full_s, full_a = torch.empty(p, **f32), torch.empty(p, **f32)
     full_t = torch.empty(p, **i32)
