@@ -9,8 +9,8 @@
 //   L_S L_S^T = S, whitener W = L_S^-1   (linalg.cpp:105-122, 150-162)
 //   u_t = W (d_t - mean), |u_t|^2        (detector.cpp:111-128)
 //   L_R L_R^T = R, w_t = R^-1 d_t / (d_t^T R^-1 d_t)  (detector.cpp:133-147)
-// and narrows to float. A pivot that is not positive (or is below 4*n*eps of
-// its diagonal, see block_cholesky) sets the device status word instead.
+// and narrows to float. A pivot that is not positive (or is below 4*256*eps
+// of its diagonal, see diag_factor) sets the device status word instead.
 //
 // The factorisations are blocked over 16x16 tiles so that everything but
 // the 16x16 diagonal tiles is a tile product on v_mfma_f64_16x16x4:
@@ -116,22 +116,31 @@ __device__ __forceinline__ double rsqrt_refined(double p) {
 //   from one refined reciprocal square root (d = p / sqrt(p) = p * r).
 //   Inverse: lane c < 16 computes column c of L^-1 by forward substitution,
 //   reading L's rows as LDS broadcasts, so no lane waits on another.
-// Returns false if a pivot fails. Writes L_KK into `lt` and L_KK^-1 into
-// `xt` (both swizzled, upper triangle zero).
+// Returns 0, or the first failing pivot's kind: kPivotNotPositive (not
+// > 0, the reference's own rule, linalg.cpp:115-117) or kPivotBelowTolerance
+// (positive, but at most 4*256*eps of its original diagonal: rounding noise
+// of a rank-deficient matrix, which the blocked order would otherwise accept
+// where the reference's sequential order lands on exactly zero). Writes L_KK
+// into `lt` and L_KK^-1 into `xt` (both swizzled, upper triangle zero).
+constexpr int kPivotNotPositive = 1;
+constexpr int kPivotBelowTolerance = 2;
 template <typename P>
-__device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_double* diag0, int lane) {
+__device__ __attribute__((noinline)) int diag_factor(P* lt, P* xt, const lds_double* diag0, int lane) {
   double row[kTile];
   double rinv[kTile];
   const int i = lane & 15;
 #pragma unroll
   for (int c = 0; c < kTile; ++c) row[c] = (lane < 16 && c <= i) ? lt[sw(i, c)] : 0.0;
   bool ok = true;
+  int why = 0;
   const double tol = 4.0 * 256 * 2.220446049250313e-16;
 #pragma unroll
   for (int j = 0; j < kTile; ++j) {
     const double pivot = lane_value(row[j], j);
     const double d0 = diag0[j];
-    if (!(pivot > 0.0) || !(pivot > tol * d0)) ok = false;
+    if (ok && !(pivot > 0.0)) why = kPivotNotPositive;
+    else if (ok && !(pivot > tol * d0)) why = kPivotBelowTolerance;
+    if (why != 0) ok = false;
     const double r = ok ? rsqrt_refined(pivot) : 1.0;
     rinv[j] = r;
     if (i == j) row[j] = ok ? pivot * r : 1.0;
@@ -162,23 +171,24 @@ __device__ __attribute__((noinline)) bool diag_factor(P* lt, P* xt, const lds_do
 #pragma unroll
     for (int r = 0; r < kTile; ++r) xt[sw(r, c)] = x[r];
   }
-  return ok;
+  return why;
 }
 
 // Blocked Cholesky of A (nt x nt tiles, lower) in place; the inverses of the
-// diagonal tiles go to X's diagonal. Returns false uniformly on failure.
+// diagonal tiles go to X's diagonal. Returns 0, or (uniformly across the
+// workgroup) the kind of the first failing pivot.
 // Lookahead: in panel K's trailing update wave 0 updates the next diagonal
 // tile first and factors it at once, while the other waves finish the rest
 // of the trailing update, so the serial diagonal factor of K + 1 is off the
 // critical path.
 template <typename P>
-__device__ __attribute__((noinline)) bool block_cholesky(Mat<P> A, Mat<P> X, int nt, const lds_double* diag0,
-                                                         __attribute__((address_space(3))) int* flag) {
+__device__ __attribute__((noinline)) int block_cholesky(Mat<P> A, Mat<P> X, int nt, const lds_double* diag0,
+                                                        __attribute__((address_space(3))) int* flag) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   auto factor = [&](int K) {
-    const bool ok = diag_factor(A.tile(K, K), X.tile(K, K), diag0 + K * kTile, lane);
-    if (lane == 0 && !ok) *flag = 1;
+    const int why = diag_factor(A.tile(K, K), X.tile(K, K), diag0 + K * kTile, lane);
+    if (lane == 0 && why != 0) *flag = why;
   };
   auto update = [&](int K, int q) {
     int r = 0;
@@ -195,7 +205,7 @@ __device__ __attribute__((noinline)) bool block_cholesky(Mat<P> A, Mat<P> X, int
   };
   if (wave == 0) factor(0);
   __syncthreads();
-  if (*flag) return false;
+  if (*flag) return *flag;
   for (int K = 0; K < nt; ++K) {
     // Panel: L_IK = A_IK * Linv_KK^T.
     const P* xkk = X.tile(K, K);
@@ -221,9 +231,9 @@ __device__ __attribute__((noinline)) bool block_cholesky(Mat<P> A, Mat<P> X, int
       for (int q = wave; q < count; q += kWaves - 1) update(K, q);
     }
     __syncthreads();
-    if (*flag) return false;
+    if (*flag) return *flag;
   }
-  return true;
+  return 0;
 }
 
 // X = L^-1 given L (tiles) and the diagonal tiles of X already inverted.
@@ -386,8 +396,11 @@ __device__ __forceinline__ void build_path(const BuildArgs& a, int path, Mat<P> 
   __syncthreads();
   fill(A, n, nt, a.sum_outer, count, mu, a.loading, path == 0, diag0);
   __syncthreads();
-  if (!block_cholesky(A, X, nt, diag0, flag)) {
-    if (tid == 0) *status = path == 0 ? HSI_STATUS_COV_NOT_PD : HSI_STATUS_CORR_NOT_PD;
+  const int why = block_cholesky(A, X, nt, diag0, flag);
+  if (why != 0) {
+    if (tid == 0)
+      *status = (path == 0 ? HSI_STATUS_COV_NOT_PD : HSI_STATUS_CORR_NOT_PD) |
+                (why == kPivotBelowTolerance ? HSI_STATUS_BELOW_TOLERANCE : 0u);
     return;
   }
   block_inverse(A, X, nt);

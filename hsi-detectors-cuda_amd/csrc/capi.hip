@@ -421,13 +421,20 @@ hsi_status hsi_stats_check(hsi_ctx* ctx, hsi_stream stream) {
   if (status == 0) return HSI_OK;
   char loading[64];
   std::snprintf(loading, sizeof loading, "%f", ctx->last_loading);
+  // The reference's messages (detector.cpp:72-77, 97-100), then the
+  // device's own reason when its stricter pivot rule fired.
+  const unsigned word = (words[0] & HSI_STATUS_COV_NOT_PD) ? words[0] : words[1];
+  const std::string tail = (word & HSI_STATUS_BELOW_TOLERANCE)
+                               ? " (a Cholesky pivot was positive but below 4*256*eps of its diagonal: "
+                                 "numerically singular)"
+                               : "";
   if (status & HSI_STATUS_COV_NOT_PD)
     return fail(ctx, HSI_ERR_NOT_POSITIVE_DEFINITE,
                 std::string("scene covariance is not positive definite at loading ") + loading +
-                    "; hyperspectral bands are strongly correlated, so try a larger --loading");
+                    "; hyperspectral bands are strongly correlated, so try a larger --loading" + tail);
   return fail(ctx, HSI_ERR_NOT_POSITIVE_DEFINITE,
               std::string("scene autocorrelation is not positive definite at loading ") + loading +
-                  "; try a larger --loading");
+                  "; try a larger --loading" + tail);
 }
 
 hsi_status hsi_stats_status_copy(hsi_ctx* ctx, uint32_t* dst, hsi_stream stream) {
@@ -642,6 +649,9 @@ hsi_status hsi_detect(hsi_ctx* ctx, const float* d_score, const int32_t* d_targe
 
 hsi_status hsi_narrow_to_half(const float* d_src, void* d_dst, size_t count, hsi_stream stream) {
   if (!d_src || !d_dst) return HSI_ERR_INVALID_ARG;
+  // The kernel moves eight elements per lane with 16-byte accesses.
+  if (reinterpret_cast<std::uintptr_t>(d_src) % 16 != 0 || reinterpret_cast<std::uintptr_t>(d_dst) % 16 != 0)
+    return HSI_ERR_INVALID_ARG;
   if (count == 0) return HSI_OK;
   if (launch_narrow_to_half(d_src, static_cast<_Float16*>(d_dst), count, static_cast<hipStream_t>(stream)) !=
       hipSuccess)
@@ -1006,6 +1016,9 @@ hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int wid
   if (d_spectra && (spectra_stride < pb || spectra_stride % 4 != 0))
     return fail(ctx, HSI_ERR_INVALID_ARG,
                 "candidate spectra need a pitch >= 32 * ceil(bands / 32), a multiple of 4");
+  // The pass writes each row with 16-byte stores.
+  if (d_spectra && reinterpret_cast<std::uintptr_t>(d_spectra) % 16 != 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "candidate spectra must be 16-byte aligned");
   if (ctx->stats_bands != bands)
     return fail(ctx, HSI_ERR_NO_STATISTICS, "screening centres on the scene mean: build statistics first");
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;

@@ -51,6 +51,4 @@ constexpr double kMaxSamAngle = static_cast<double>(1.57079632679489661923f);
 
 }  // namespace hsi_amd
 
-// Device status word bits written by the statistics build.
-#define HSI_STATUS_COV_NOT_PD 1u
-#define HSI_STATUS_CORR_NOT_PD 2u
+// The device status word bits (HSI_STATUS_*) are public: include/hsi_mi355x.h.

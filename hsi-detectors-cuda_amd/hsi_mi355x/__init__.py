@@ -36,6 +36,8 @@ class HsiError(RuntimeError):
 
 OK, ERR_INVALID_ARG, ERR_CAPACITY, ERR_NOT_POSITIVE_DEFINITE, ERR_DEVICE, ERR_NO_STATISTICS, \
     ERR_NO_LIBRARY = range(7)
+# Statistics status-word bits (HSI_STATUS_*).
+STATUS_COV_NOT_PD, STATUS_CORR_NOT_PD, STATUS_BELOW_TOLERANCE = 1, 2, 4
 F32, F16 = 0, 1
 PLAIN, MIXED = 0, 1
 
@@ -357,6 +359,15 @@ class Context:
     def stats_check(self, stream=None):
         self._check(self._lib.hsi_stats_check(self.handle, _stream(stream)))
 
+    def stats_status(self, stream=None):
+        """The two status words of the last build (covariance, autocorrelation
+        path) as ints: 0, or STATUS_* bits."""
+        torch = _torch()
+        words = torch.zeros(2, dtype=torch.int32, device="cuda")
+        self._check(self._lib.hsi_stats_status_copy(self.handle, _ptr(words), _stream(stream)))
+        torch.cuda.synchronize()
+        return [int(v) for v in words.cpu().tolist()]
+
     def stats_read(self, bands, num_targets):
         out = {
             "mean": np.zeros(bands, np.float32),
@@ -467,6 +478,8 @@ class Context:
         `spectra` ([capacity, >= 32 * ceil(bands / 32)] float32) their bands,
         each in the row its record's target field names."""
         cap = candidates.shape[0]
+        if spectra is not None and spectra.shape[0] < cap:
+            raise ValueError(f"spectra hold {spectra.shape[0]} rows; the list's capacity is {cap}")
         self._check(self._lib.hsi_mlp_screen(self.handle, _ptr(cube), height, width, bands,
                                              cube.stride(0), float(cut), _ptr(candidates), _ptr(count),
                                              cap, _ptr(spectra), spectra.stride(0) if spectra is not None else 0,
@@ -475,6 +488,8 @@ class Context:
     def score_candidate_spectra(self, detector, cube, height, width, bands, candidates, count, spectra,
                                 score, target=None, aux=None, stream=None):
         cap = candidates.shape[0]
+        if spectra.shape[0] < cap:
+            raise ValueError(f"spectra hold {spectra.shape[0]} rows; the list's capacity is {cap}")
         self._check(self._lib.hsi_score_candidate_spectra(
             self.handle, int(detector), _ptr(cube), height, width, bands, cube.stride(0),
             _ptr(candidates), _ptr(count), cap, _ptr(spectra), spectra.stride(0), _ptr(score),
@@ -585,6 +600,24 @@ def demosaic(ctx, raw, raw_type, raw_height, raw_width, period, bands, cube, ban
     ctx._check(ctx._lib.hsi_demosaic(ctx.handle, _ptr(raw), raw_type, raw_height, raw_width, period, bands,
                                      None if order is None else order.ctypes.data, _ptr(cube),
                                      cube.stride(0), _stream(stream)))
+
+
+def narrow_to_half(src, dst, count=None, stream=None):
+    """hsi_narrow_to_half: `count` float32 elements of device tensor `src`
+    to float16 in `dst` (both 16-byte aligned)."""
+    count = src.numel() if count is None else count
+    st = lib().hsi_narrow_to_half(_ptr(src), _ptr(dst), count, _stream(stream))
+    if st != OK:
+        raise HsiError(st, "narrow_to_half: null or misaligned buffer")
+
+
+def transpose_bip_to_bsq(src, dst, height, width, bands, stream=None):
+    """hsi_transpose_bip_to_bsq: device BIP [pixel][band] float32 -> BSQ
+    (bands, stride) tensor `dst`."""
+    st = lib().hsi_transpose_bip_to_bsq(_ptr(src), _ptr(dst), height, width, bands,
+                                        dst.stride(0), _stream(stream))
+    if st != OK:
+        raise HsiError(st, "transpose_bip_to_bsq: bad shape or stride")
 
 
 def mlp_param_count(bands):

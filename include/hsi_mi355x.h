@@ -154,12 +154,25 @@ hsi_status hsi_stats_build(hsi_ctx* ctx, double diagonal_loading, hsi_stream str
 
 /* Synchronous: wait for `stream`, then report HSI_OK or
  * HSI_ERR_NOT_POSITIVE_DEFINITE with the reference's message (which names
- * --loading) in hsi_ctx_last_error. */
+ * --loading) in hsi_ctx_last_error; when the failing pivot was positive but
+ * below the device tolerance (HSI_STATUS_BELOW_TOLERANCE) the message says
+ * so after the reference's text. */
 hsi_status hsi_stats_check(hsi_ctx* ctx, hsi_stream stream);
 
+/* Bits of the statistics status words. A Cholesky pivot fails when it is not
+ * positive - the reference's rule (linalg.cpp:115-117) - or when it is
+ * positive but at most 4*256*eps of its diagonal entry before factorisation:
+ * the device rejects such rounding-level pivots, which a rank-deficient
+ * matrix leaves in a blocked factorisation where the reference's sequential
+ * one lands on exactly zero. BELOW_TOLERANCE is set beside COV/CORR_NOT_PD
+ * in that second case; the reference would have built statistics there. */
+#define HSI_STATUS_COV_NOT_PD 1u
+#define HSI_STATUS_CORR_NOT_PD 2u
+#define HSI_STATUS_BELOW_TOLERANCE 4u
+
 /* Asynchronous: copy the two status words of the last build (covariance
- * path, autocorrelation path; 0 = factorised, else the factorisation failed)
- * to `dst` (device or pinned host memory, 8 bytes) on `stream`. The
+ * path, autocorrelation path; 0 = factorised, else HSI_STATUS_* bits) to
+ * `dst` (device or pinned host memory, 8 bytes) on `stream`. The
  * non-blocking form of hsi_stats_check, for pipelines and captured graphs.
  * No reference counterpart: build_statistics is synchronous there
  * (detector.cpp:47-152). */
@@ -322,7 +335,10 @@ hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int wid
                           size_t spectra_stride, float* d_logit, hsi_stream stream);
 
 /* hsi_score_candidates over the rows that hsi_mlp_screen wrote: record i
- * is scored from row d_candidates[i].target of d_spectra (capacity rows,
+ * is scored from row d_candidates[i].target of d_spectra. Only a list that
+ * hsi_mlp_screen wrote (with these spectra) is valid input: the target field
+ * of any other list (hsi_detect's target index) names no spectra row. Rows:
+ * (capacity rows,
  * pitch spectra_stride >= bands; a record whose row is outside
  * [0, capacity) is skipped) and written at its (x, y). The whole-frame fallback on overflow reads d_cube.
  * Same results as hsi_score_candidates. Asynchronous. */
@@ -420,7 +436,8 @@ hsi_ctx* hsi_batch_context(hsi_batch* batch, int scene);
 
 /* ---- ingest helpers ---------------------------------------------------------- */
 
-/* FP32 -> FP16 narrowing of `count` elements (round to nearest even).
+/* FP32 -> FP16 narrowing of `count` elements (round to nearest even). Both
+ * buffers must be 16-byte aligned (INVALID_ARG otherwise); any count.
  * Replaces launch_narrow_to_half (sam_cuda.hpp:107-109, sam.cu:311-316). */
 hsi_status hsi_narrow_to_half(const float* d_src, void* d_dst, size_t count,
                               hsi_stream stream);

@@ -4,13 +4,14 @@
 // the reference's own sources (src/types.cpp, linalg.cpp, detector.cpp,
 // detectors_cpu.cpp, sam_cpu.cpp), read in place from the reference checkout,
 // into oracle/_ref/libhsiref.so. tests/golden/make_golden.py calls it to write
-// the golden fixtures, and tests/test_oracle_vs_ref.py compares the oracle
-// against it. The product never loads it.
+// the golden fixtures that tests/test_oracle_golden.py holds the oracle to,
+// bit for bit. The product never loads it.
 //
 // Each wrapper builds the reference's own types and calls the reference
 // function named in its comment; nothing here re-implements arithmetic.
 #include <cstdint>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -188,6 +189,37 @@ void ref_triangular_inverse_lower(const double* l, int n, double* inv) {
 }
 void ref_cholesky_solve(const double* l, int n, const double* b, double* x) {
   hsi::cholesky_solve(l, n, b, x);
+}
+
+// cholesky_inverse (linalg.cpp:139-148)
+void ref_cholesky_inverse(const double* l, int n, double* inv) { hsi::cholesky_inverse(l, n, inv); }
+
+// quadratic_form (linalg.cpp:172-180)
+double ref_quadratic_form(const double* m, const double* y, int n) {
+  return hsi::quadratic_form(m, y, n);
+}
+
+// moments_from_shifted (linalg.cpp:81-103): the rebuilt sum[B] and the full
+// sum_outer[B*B] as SpectralMoments holds it (upper triangle zero).
+void ref_moments_from_shifted(const double* s1, const double* s2_lower, double count,
+                              const double* reference, int bands, double* out_sum,
+                              double* out_sum_outer) {
+  hsi::SpectralMoments m;
+  hsi::moments_from_shifted(s1, s2_lower, count, reference, bands, &m);
+  std::memcpy(out_sum, m.sum.data(), m.sum.size() * sizeof(double));
+  std::memcpy(out_sum_outer, m.sum_outer.data(), m.sum_outer.size() * sizeof(double));
+}
+
+// mean_spectrum (sam_cpu.cpp:140-164). Returns 1, or 0 when the mask selects
+// no pixel (the reference throws).
+int ref_mean_spectrum(const float* cube, int h, int w, int b, const uint8_t* mask, float* out) {
+  try {
+    const hsi::Spectrum s = hsi::mean_spectrum(cube, shape_of(h, w, b), mask, "mean");
+    std::memcpy(out, s.values.data(), s.values.size() * sizeof(float));
+    return 1;
+  } catch (const std::invalid_argument&) {
+    return 0;
+  }
 }
 
 }  // extern "C"

@@ -13,7 +13,8 @@ targets planted at high abundance), plus hand-built edge cases. Outputs come
 from the reference's own functions (sam_best_cpu, sam_cpu, detect_cpu,
 accumulate_moments_cpu, build_statistics, rx_cpu, ace_best_cpu, cem_best_cpu,
 rx_threshold_for_false_alarm_rate, cholesky, triangular_inverse_lower,
-cholesky_solve) called through oracle/ref_shim.cpp. Each fixture is an .npz
+cholesky_solve, cholesky_inverse, quadratic_form, moments_from_shifted,
+mean_spectrum) called through oracle/ref_shim.cpp. Each fixture is an .npz
 holding inputs and expected outputs; tests load them with
 numpy.load(allow_pickle=False).
 """
@@ -60,6 +61,13 @@ def load_ref():
     lib.ref_cholesky.argtypes = [f64p, C.c_int]
     lib.ref_triangular_inverse_lower.argtypes = [f64p, C.c_int, f64p]
     lib.ref_cholesky_solve.argtypes = [f64p, C.c_int, f64p, f64p]
+    lib.ref_cholesky_inverse.argtypes = [f64p, C.c_int, f64p]
+    lib.ref_quadratic_form.restype = C.c_double
+    lib.ref_quadratic_form.argtypes = [f64p, f64p, C.c_int]
+    lib.ref_moments_from_shifted.argtypes = [f64p, f64p, C.c_double, f64p, C.c_int, f64p, f64p]
+    lib.ref_mean_spectrum.restype = C.c_int
+    lib.ref_mean_spectrum.argtypes = [f32p, C.c_int, C.c_int, C.c_int,
+                                      np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"), f32p]
     return lib
 
 
@@ -250,6 +258,89 @@ def edge_cases(ref):
     return out
 
 
+def helper_cases(ref):
+    """mean_spectrum, moments_from_shifted, cholesky_inverse and quadratic_form."""
+    out = {}
+    rng = np.random.default_rng(23)
+    # mean_spectrum over masks of a 9x11x7 cube: a sparse mask, a full mask,
+    # a single pixel, and an empty mask (the reference throws).
+    cube = np.ascontiguousarray(rng.normal(0.4, 0.2, size=(7, 9, 11)).astype(np.float32))
+    masks = np.zeros((4, 99), np.uint8)
+    masks[0] = rng.random(99) < 0.3
+    masks[1] = 1
+    masks[2, 57] = 1
+    means = np.zeros((4, 7), np.float32)
+    oks = np.zeros(4, np.int32)
+    for i in range(4):
+        oks[i] = ref.ref_mean_spectrum(cube, 9, 11, 7, np.ascontiguousarray(masks[i]), means[i])
+    out.update(ms_cube=cube, ms_masks=masks, ms_mean=means, ms_ok=oks)
+    # moments_from_shifted: the shifted moments of a small cube about a
+    # reference near its mean (computed here in float64), rebuilt by the
+    # reference; and a zero count.
+    bands = 9
+    x = rng.normal(0.5, 0.1, size=(bands, 200))
+    r = (x.mean(axis=1) + rng.normal(0, 0.01, bands)).astype(np.float64)
+    d = x - r[:, None]
+    s1 = d.sum(axis=1)
+    s2 = np.tril(d @ d.T)
+    s2[np.triu_indices(bands, 1)] = 7.0  # the upper triangle is ignored
+    osum = np.zeros(bands)
+    oouter = np.zeros(bands * bands)
+    ref.ref_moments_from_shifted(s1, np.ascontiguousarray(s2), 200.0, r, bands, osum, oouter)
+    zsum = np.zeros(bands)
+    zouter = np.zeros(bands * bands)
+    ref.ref_moments_from_shifted(s1, np.ascontiguousarray(s2), 0.0, r, bands, zsum, zouter)
+    out.update(mfs_s1=s1, mfs_s2=s2, mfs_ref=r, mfs_sum=osum, mfs_outer=oouter.reshape(bands, bands),
+               mfs_zero_sum=zsum, mfs_zero_outer=zouter.reshape(bands, bands))
+    # cholesky_inverse and quadratic_form on SPD matrices.
+    for n in (1, 3, 12, 33):
+        bm = rng.normal(size=(n, n))
+        a = bm @ bm.T + n * np.eye(n)
+        l = np.ascontiguousarray(a.copy())
+        assert ref.ref_cholesky(l, n) == 1
+        inv = np.empty((n, n))
+        ref.ref_cholesky_inverse(l, n, inv)
+        y = rng.normal(size=n)
+        q = ref.ref_quadratic_form(np.ascontiguousarray(a), y, n)
+        out.update({f"ci{n}_a": a, f"ci{n}_l": l, f"ci{n}_inv": inv, f"ci{n}_y": y,
+                    f"ci{n}_q": np.float64(q)})
+    return out
+
+
+def near_singular_case(ref):
+    """A scene whose covariance the reference factorises although one pivot is
+    below 4*256*eps of its diagonal: band 1 is band 0 moved by one float ulp on
+    part of the pixels, so the two are collinear to about 1e-15."""
+    rng = np.random.default_rng(77)
+    h, w, b = 16, 16, 6
+    tol = 4.0 * 256 * np.finfo(np.float64).eps
+    for _ in range(100):
+        base = rng.normal(0.0, 0.2, size=(b, h * w)).astype(np.float32)
+        sel = rng.random(h * w) < rng.uniform(0.05, 0.9)
+        up = rng.random(int(sel.sum())) < 0.5
+        base[1] = base[0]
+        base[1, sel] = np.nextafter(base[0, sel], np.where(up, np.inf, -np.inf).astype(np.float32))
+        xd = base.astype(np.float64)
+        cov = xd @ xd.T / (h * w) - np.outer(xd.mean(axis=1), xd.mean(axis=1))
+        piv = np.linalg.eigvalsh(cov).min() / np.diag(cov).max()
+        if not (0.005 * tol < piv < 0.1 * tol):
+            continue
+        cube = np.ascontiguousarray(base.reshape(b, h, w))
+        out = {"nsing_cube": cube}
+        for key, loading in (("nsing_l0", 0.0), ("nsing_l3", 1e-3)):
+            mean = np.zeros(b, np.float32)
+            white = np.zeros(b * b, np.float32)
+            z1 = np.zeros(1, np.float32)
+            err = C.create_string_buffer(512)
+            ok = ref.ref_build_statistics(cube, h, w, b, None, 0, b, loading, mean, white, z1, z1,
+                                          z1, err, 512)
+            assert ok == 1, err.value
+            out[f"{key}_whitener"] = white.reshape(b, b)
+            out[f"{key}_mean"] = mean
+        return out
+    raise RuntimeError("no near-singular scene found")
+
+
 def singular_case(ref):
     """Duplicated bands: build_statistics must fail at loading 0 and name --loading."""
     rng = np.random.default_rng(5)
@@ -293,6 +384,8 @@ def main():
         print(f"wrote {path} ({os.path.getsize(path) // 1024} KiB)")
     out = edge_cases(ref)
     out.update(singular_case(ref))
+    out.update(helper_cases(ref))
+    out.update(near_singular_case(ref))
     path = os.path.join(HERE, "edge_cases.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path} ({os.path.getsize(path) // 1024} KiB)")

@@ -374,6 +374,30 @@ def test_singular_covariance_is_reported_with_loading(H, torch, ctx):
     assert rel_err(st["whitener"], want, float(np.abs(want).max()) * 1e-3).max() <= 1e-5
 
 
+def test_near_singular_covariance_is_rejected_with_its_own_status(H, torch, ctx):
+    """The reference factorises this covariance (its smallest pivot is ~1e-15
+    of its diagonal, positive); the device rejects pivots at or below
+    4*256*eps of their diagonal and says so in the status word and message.
+    With loading both factorise and agree."""
+    g = golden("edge_cases")
+    cube = H.to_device_cube(g["nsing_cube"], 16, 16)
+    ctx.stats_accumulate(cube, 16, 16, 6)
+    ctx.stats_build(0.0)
+    cov, corr = ctx.stats_status()
+    assert cov == H.STATUS_COV_NOT_PD | H.STATUS_BELOW_TOLERANCE, cov
+    with pytest.raises(H.HsiError) as info:
+        ctx.stats_check()
+    assert info.value.status == H.ERR_NOT_POSITIVE_DEFINITE
+    assert "loading" in str(info.value) and "4*256*eps" in str(info.value)
+    ctx.stats_build(1e-3)
+    ctx.stats_check()
+    assert ctx.stats_status() == [0, 0]
+    st = ctx.stats_read(6, 0)
+    want = g["nsing_l3_whitener"]
+    assert rel_err(st["whitener"], want, float(np.abs(want).max()) * 1e-3).max() <= 1e-5
+    assert rel_err(st["mean"], g["nsing_l3_mean"], 1.0).max() <= 1e-6
+
+
 def run_detect(H, torch, ctx, score, target, h, w, thr, nms, higher, cap=None):
     p = h * w
     cap = p if cap is None else cap

@@ -64,3 +64,48 @@ def test_adam_first_step_is_lr_times_sign():
     w = np.zeros(3)
     w1, m, v = mlp_ref.adam_step(w, np.zeros(3), np.zeros(3), g, 1, 0.01, 0.9, 0.999, 0.0)
     assert np.allclose(w1, -0.01 * np.sign(g))
+
+
+def test_forward_bf16_equals_forward_on_bf16_exact_inputs():
+    """Weights, inputs, biases and layer-1 activations that bfloat16 holds
+    exactly (multiples of 1/16 below 8): every rounding step is the identity,
+    so the bf16 forward pass must equal the float64 one."""
+    bands, n = 3, 40
+    rng = np.random.default_rng(5)
+    P = mlp_ref.param_count(bands)
+    w = rng.integers(-4, 5, P) / 4.0
+    W1, b1, W2, b2, w3, b3 = mlp_ref.unpack(w, bands)
+    b1[:] = rng.integers(-8, 9, b1.shape) / 16.0
+    w = mlp_ref.pack(W1, b1, W2, b2, w3, b3)
+    x = rng.integers(-4, 5, (bands, n)) / 4.0
+    z, h1, _ = mlp_ref.forward(w, x, bands)
+    assert np.array_equal(mlp_ref.bf16(h1), h1)
+    assert np.abs(h1).max() > 0
+    assert np.array_equal(mlp_ref.forward_bf16(w, x, bands), z)
+
+
+def test_forward_bf16_hand_worked():
+    """One live unit per layer, worked by hand. bf16 keeps 8 significant bits:
+    bf16(1.01) = 1.0078125 (129/128), bf16(0.3) = 0.30078125 (154/512),
+    bf16(0.51) = 0.51171875 (131/256).
+    Sample 0, x = (3, 0.3): W1.x = 1.0078125*3 + 2*0.30078125 = 3.625;
+      + b1 = 3.725 -> bf16 3.71875 (238/64); W2 h1 = 0.51171875*3.71875 =
+      1.9029541015625; + b2 = 2.1529541015625; logit = 2*that - 1 = 3.305908203125.
+    Sample 1, x = (-1, 0): W1.x + b1 = -0.9078125 -> relu 0; h2 = b2 = 0.25;
+      logit = -0.5."""
+    bands = 2
+    w = np.zeros(mlp_ref.param_count(bands))
+    W1, b1, W2, b2, w3, b3 = mlp_ref.unpack(w, bands)
+    W1 = W1.copy(); b1 = b1.copy(); W2 = W2.copy(); b2 = b2.copy(); w3 = w3.copy()
+    W1[0] = [1.01, 2.0]
+    b1[0] = 0.1
+    W2[0, 0] = 0.51
+    b2[0] = 0.25
+    w3[0] = 2.0
+    w3[5] = 9.0  # a dead unit: its input is zero
+    w = mlp_ref.pack(W1, b1, W2, b2, w3, -1.0)
+    x = np.array([[3.0, -1.0], [0.3, 0.0]])
+    got = mlp_ref.forward_bf16(w, x, bands)
+    assert got[0] == 3.305908203125 and got[1] == -0.5
+    # the float64 network differs on sample 0: every rounding step matters
+    assert mlp_ref.forward(w, x, bands)[0][0] != got[0]

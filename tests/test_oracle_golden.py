@@ -129,3 +129,54 @@ def test_singular_and_chi2_match_reference(oracle_lib):
             assert same_bits(st.mean, g[f"{key}_mean"])
         else:
             assert "loading" in st.error
+
+
+def test_mean_spectrum_matches_reference(oracle_lib):
+    g = golden("edge_cases")
+    for i, mask in enumerate(g["ms_masks"]):
+        if g["ms_ok"][i]:
+            assert same_bits(oracle_lib.mean_spectrum(g["ms_cube"], mask), g["ms_mean"][i])
+        else:
+            with pytest.raises(ValueError, match="no pixels"):
+                oracle_lib.mean_spectrum(g["ms_cube"], mask)
+    assert list(g["ms_ok"]) == [1, 1, 1, 0]
+
+
+def test_moments_from_shifted_matches_reference(oracle_lib):
+    g = golden("edge_cases")
+    s, so = oracle_lib.moments_from_shifted(g["mfs_s1"], g["mfs_s2"], 200.0, g["mfs_ref"])
+    assert same_bits(s, g["mfs_sum"])
+    assert same_bits(so, g["mfs_outer"])
+    z, zo = oracle_lib.moments_from_shifted(g["mfs_s1"], g["mfs_s2"], 0.0, g["mfs_ref"])
+    assert same_bits(z, g["mfs_zero_sum"]) and same_bits(zo, g["mfs_zero_outer"])
+    assert not z.any() and not zo.any()
+
+
+@pytest.mark.parametrize("n", [1, 3, 12, 33])
+def test_cholesky_inverse_and_quadratic_form_match_reference(oracle_lib, n):
+    g = golden("edge_cases")
+    ok, l = oracle_lib.cholesky(g[f"ci{n}_a"])
+    assert ok and same_bits(l, g[f"ci{n}_l"])
+    assert same_bits(oracle_lib.cholesky_inverse(l), g[f"ci{n}_inv"])
+    q = oracle_lib.quadratic_form(g[f"ci{n}_a"], g[f"ci{n}_y"])
+    assert same_bits(np.array([q], np.float64), np.array([g[f"ci{n}_q"]], np.float64))
+
+
+def test_near_singular_scene_matches_reference(oracle_lib):
+    """The reference factorises this covariance although its smallest pivot is
+    a rounding-level fraction of its diagonal; the oracle must agree bit for
+    bit, and the pivot must lie in (0, 4*256*eps] of its diagonal entry - the
+    band the device rejects where the reference does not."""
+    g = golden("edge_cases")
+    cube = g["nsing_cube"]
+    for key, loading in (("nsing_l0", 0.0), ("nsing_l3", 1e-3)):
+        st = oracle_lib.statistics_of_cube(cube, None, loading)
+        assert st.valid, st.error
+        assert same_bits(st.whitener, g[f"{key}_whitener"])
+        assert same_bits(st.mean, g[f"{key}_mean"])
+    count, s, so = oracle_lib.moments(cube)
+    cov = oracle_lib.covariance(count, s, so, 0.0)
+    ok, l = oracle_lib.cholesky(cov)
+    rel = np.diag(l) ** 2 / np.diag(cov)
+    tol = 4.0 * 256 * np.finfo(np.float64).eps
+    assert ok and 0.0 < rel.min() <= tol
