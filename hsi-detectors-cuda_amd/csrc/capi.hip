@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -142,6 +143,29 @@ hsi_status set_device(hsi_ctx* ctx) {
   return HSI_OK;
 }
 
+// Device address of hsi_detect_no_alloc's static scratch, per device: looked
+// up once, by the first context created on that device (with the device
+// current), and kept for the life of the process.
+constexpr int kMaxDevices = 64;
+std::mutex g_no_alloc_mutex;
+unsigned* g_no_alloc_scratch[kMaxDevices] = {};
+bool g_no_alloc_looked_up[kMaxDevices] = {};
+
+void resolve_no_alloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_no_alloc_mutex);
+  if (g_no_alloc_looked_up[device]) return;
+  g_no_alloc_looked_up[device] = true;
+  unsigned* ptr = nullptr;
+  if (detect_no_alloc_scratch(&ptr) == hipSuccess) g_no_alloc_scratch[device] = ptr;
+}
+
+unsigned* no_alloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_no_alloc_mutex);
+  return g_no_alloc_scratch[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -237,6 +261,7 @@ hsi_status hsi_ctx_create(int device, hsi_ctx** out) {
     hsi_ctx_destroy(ctx);
     return HSI_ERR_DEVICE;
   }
+  resolve_no_alloc_scratch(device);
   *out = ctx;
   return HSI_OK;
 }
@@ -643,6 +668,29 @@ hsi_status hsi_detect(hsi_ctx* ctx, const float* d_score, const int32_t* d_targe
           "allocate detect scratch");
   HSI_HIP(ctx, launch_detect(d_score, d_target, height, width, params, d_out, d_count,
                              ctx->d_det_scratch, static_cast<hipStream_t>(stream)),
+          "detect");
+  return HSI_OK;
+}
+
+hsi_status hsi_detect_no_alloc(hsi_ctx* ctx, const float* d_score, const int32_t* d_target, int height,
+                               int width, hsi_detection_params params, hsi_detection* d_out, uint32_t* d_count,
+                               hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (!d_score || !d_out || !d_count || height <= 0 || width <= 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "detect: null buffer or bad shape");
+  if (params.max_detections < 0 || params.nms_radius < 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "detect: negative capacity or radius");
+  const std::size_t pixels = static_cast<std::size_t>(height) * width;
+  if (pixels > kDetectNoAllocMaxPixels)
+    return fail(ctx, HSI_ERR_CAPACITY,
+                "detect without allocation: " + std::to_string(pixels) + " pixels exceed the static scratch's " +
+                    std::to_string(kDetectNoAllocMaxPixels) + " (use hsi_detect)");
+  unsigned* scratch = no_alloc_scratch(ctx->device);
+  if (!scratch)
+    return fail(ctx, HSI_ERR_DEVICE, "detect without allocation: the static scratch has no address on this device");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, launch_detect(d_score, d_target, height, width, params, d_out, d_count, scratch,
+                             static_cast<hipStream_t>(stream)),
           "detect");
   return HSI_OK;
 }

@@ -223,6 +223,22 @@ __global__ __launch_bounds__(kDetBlock) void detect_write_kernel(
 
 }  // namespace
 
+// Scratch of hsi_detect_no_alloc: one static array per GPU, sized for frames
+// of up to kDetectNoAllocMaxPixels. Host code never names it directly (its
+// host-side name is not a device address): detect_no_alloc_scratch looks up
+// its address on the current device with hipGetSymbolAddress.
+constexpr std::size_t kDetectNoAllocWords =
+    (kDetectNoAllocMaxPixels / kDetBlock) * (kDetBlock / kWave) * 2 + 2 * (kDetectNoAllocMaxPixels / kDetBlock);
+static_assert(kDetectNoAllocMaxPixels % kDetBlock == 0, "the cap must be whole detect blocks");
+static __device__ __align__(256) unsigned scratch[kDetectNoAllocWords];
+
+hipError_t detect_no_alloc_scratch(unsigned** out) {
+  void* ptr = nullptr;
+  const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  *out = err == hipSuccess ? static_cast<unsigned*>(ptr) : nullptr;
+  return err;
+}
+
 std::size_t detect_scratch_words(std::size_t pixels) {
   const std::size_t blocks = div_up(pixels, kDetBlock);
   // masks (2 words each), counts, offsets

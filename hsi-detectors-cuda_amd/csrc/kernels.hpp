@@ -252,6 +252,11 @@ hipError_t launch_demosaic(const void* raw, int raw_type, int raw_height, int ra
 
 // ---- detect ------------------------------------------------------------------
 std::size_t detect_scratch_words(std::size_t pixels);
+// The static scratch of hsi_detect_no_alloc covers frames up to this size
+// (4096 x 4096 pixels: 2.5 MiB of device memory per GPU, held until the
+// process ends). Address on the current device, looked up once per device.
+constexpr std::size_t kDetectNoAllocMaxPixels = std::size_t(1) << 24;
+hipError_t detect_no_alloc_scratch(unsigned** out);
 hipError_t launch_detect(const float* score, const int* target, int height, int width,
                          hsi_detection_params params, hsi_detection* out, unsigned* count,
                          unsigned* scratch, hipStream_t stream);

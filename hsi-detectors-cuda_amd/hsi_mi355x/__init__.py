@@ -36,6 +36,7 @@ class HsiError(RuntimeError):
 
 OK, ERR_INVALID_ARG, ERR_CAPACITY, ERR_NOT_POSITIVE_DEFINITE, ERR_DEVICE, ERR_NO_STATISTICS, \
     ERR_NO_LIBRARY = range(7)
+DETECT_NO_ALLOC_MAX_PIXELS = 1 << 24
 # Statistics status-word bits (HSI_STATUS_*).
 STATUS_COV_NOT_PD, STATUS_CORR_NOT_PD, STATUS_BELOW_TOLERANCE = 1, 2, 4
 F32, F16 = 0, 1
@@ -168,6 +169,8 @@ def lib():
         "hsi_score_sam_cem": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t,
                                         _vp, _vp, _vp, _vp, _vp]),
         "hsi_detect": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, DetectionParams, _vp, _vp, _vp]),
+        "hsi_detect_no_alloc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, DetectionParams, _vp, _vp,
+                                          _vp]),
         "hsi_narrow_to_half": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
         "hsi_transpose_bip_to_bsq": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t,
                                                _vp]),
@@ -422,6 +425,13 @@ class Context:
     def detect(self, score, target, height, width, params, out, count, stream=None):
         self._check(self._lib.hsi_detect(self.handle, _ptr(score), _ptr(target), height, width,
                                          params, _ptr(out), _ptr(count), _stream(stream)))
+
+    def detect_no_alloc(self, score, target, height, width, params, out, count, stream=None):
+        """hsi_detect_no_alloc: hsi_detect over a static per-GPU scratch
+        (frames up to DETECT_NO_ALLOC_MAX_PIXELS; overlapping calls race)."""
+        self._check(self._lib.hsi_detect_no_alloc(self.handle, _ptr(score), _ptr(target), height,
+                                                  width, params, _ptr(out), _ptr(count),
+                                                  _stream(stream)))
 
     # learned pre-screen
     def mlp_init(self, bands, seed=1):

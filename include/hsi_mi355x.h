@@ -241,6 +241,20 @@ hsi_status hsi_detect(hsi_ctx* ctx, const float* d_score, const int32_t* d_targe
                       int height, int width, hsi_detection_params params,
                       hsi_detection* d_out, uint32_t* d_count, hsi_stream stream);
 
+/* hsi_detect without any allocation. No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, results and
+ * errors as hsi_detect, but its scratch is one static device array per GPU
+ * (2.5 MiB, held until the process ends) instead of the context's, so it
+ * calls no hipMalloc/hipFree. Frames of at most HSI_DETECT_NO_ALLOC_MAX_PIXELS
+ * pixels (4096 x 4096); larger ones return HSI_ERR_CAPACITY. Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams or
+ * threads) race and must be serialised by the caller. */
+#define HSI_DETECT_NO_ALLOC_MAX_PIXELS 16777216u
+hsi_status hsi_detect_no_alloc(hsi_ctx* ctx, const float* d_score, const int32_t* d_target,
+                               int height, int width, hsi_detection_params params,
+                               hsi_detection* d_out, uint32_t* d_count, hsi_stream stream);
+
 /* ---- learned pre-screen --------------------------------------------------------
  *
  * No counterpart in the reference. A B -> 64 -> 32 -> 1 ReLU network (FP32
