@@ -3,15 +3,18 @@
 // Counterpart of the reference's hsi_detect (main.cpp:125-292): frames from a
 // synthetic or ENVI-replay source go through the Pipeline, and the run reports
 // throughput, latency percentiles and per-stage device times. Any of the four
-// detectors (or several) score each frame; flags keep the reference's names
-// where the meaning is the same (--source, --envi, --width/--height/--bands,
-// --targets, --noise, --library, --threshold, --nms, --max-detections,
-// --streams, --frames, --warmup, --prefill, --csv, --print-detections,
-// --dump-score, --quiet).
+// detectors (or several) score each frame, each against its own threshold.
+// The reference's flags keep their names and meanings (--source, --envi,
+// --resident, --width/--height/--bands, --targets, --noise, --library,
+// --threshold, --nms, --max-detections, --variant, --memory, --streams,
+// --frames, --warmup, --prefill, --csv, --print-detections, --dump-score,
+// --quiet); an unknown flag or value is an error (exit 2), never ignored.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -24,21 +27,38 @@ void usage() {
 
   hsi_detect [options]
 
+Source
   --source=synthetic|envi       where frames come from               [synthetic]
   --envi=A.hdr,B.hdr            ENVI cubes to replay (cycled)
+  --resident=N                  ENVI cubes held in RAM (the first N)       [4]
   --width=N --height=N          synthetic frame size                 [512x512]
   --bands=N                     synthetic band count                     [113]
   --targets=N                   synthetic library size                     [4]
   --noise=F                     synthetic noise sigma                  [0.005]
+  --drift=F                     synthetic per-frame drift of the mix       [0]
   --seed=N                      synthetic seed                      [20261203]
+
+Detection
   --library=FILE.csv            target spectra; the synthetic source supplies
                                 its own when this is absent
   --detector=LIST               sam, rx, ace, cem, comma-separated       [sam]
-  --precision=plain|mixed       RX/ACE arithmetic form                 [plain]
-  --loading=F                   diagonal loading of the statistics      [1e-6]
-  --threshold=F                 detection threshold (angle for SAM)     [0.10]
+  --threshold=RAD               SAM spectral angle threshold            [0.10]
+  --rx-threshold=F              RX threshold (Mahalanobis squared); default:
+                                the chi-squared quantile of --rx-false-alarm
+  --rx-false-alarm=RATE         per-pixel false-alarm rate for RX       [1e-4]
+  --ace-threshold=F             ACE threshold                            [0.5]
+  --cem-threshold=F             CEM threshold                            [0.5]
   --nms=N                       non-maximum suppression radius, 0 off      [2]
   --max-detections=N            detection list capacity per detector   [4096]
+  --loading=F                   diagonal loading of the statistics      [1e-6]
+
+Execution
+  --variant=baseline|optimized|half   baseline and optimized score the FP32
+                                cube; half narrows each frame to FP16 on the
+                                device and scores that        [optimized]
+  --memory=copy                 frames are copied to HBM (the only mode on a
+                                discrete GPU; zerocopy is refused)      [copy]
+  --precision=plain|mixed       RX/ACE arithmetic form                 [plain]
   --streams=N                   frames in flight                           [3]
   --frames=N                    frames to process                        [300]
   --warmup=N                    unmeasured frames first                   [30]
@@ -46,13 +66,25 @@ void usage() {
   --no-graph                    launch kernels one by one instead of replaying
                                 a captured hipGraph per frame
   --device=N                    HIP device                                 [0]
+
+Output
   --csv=FILE                    write per-frame timings
   --print-detections=N          print the first N detections of the last frame
   --dump-score=FILE             write the last frame's first score map (raw f32)
+  --dump-detections=FILE        write every measured frame's detection lists
+                                (CSV: frame,detector,x,y,target,score)
   --json                        print one JSON summary line
   --quiet
   --help)");
 }
+
+// Every flag this program reads; anything else is refused.
+const std::set<std::string> kFlags = {
+    "source", "envi", "resident", "width", "height", "bands", "targets", "noise", "drift", "seed",
+    "library", "detector", "threshold", "rx-threshold", "rx-false-alarm", "ace-threshold",
+    "cem-threshold", "nms", "max-detections", "loading", "variant", "memory", "precision", "streams",
+    "frames", "warmup", "prefill", "no-graph", "device", "csv", "print-detections", "dump-score",
+    "dump-detections", "json", "quiet", "help", "h"};
 
 struct Args {
   std::map<std::string, std::string> v;
@@ -71,15 +103,22 @@ struct Args {
   }
 };
 
-Args parse(int argc, char** argv) {
+// --key=value and --flag tokens; a stray positional argument or an unknown
+// key is reported in `bad`.
+Args parse(int argc, char** argv, std::string* bad) {
   Args a;
   for (int i = 1; i < argc; ++i) {
     std::string t = argv[i];
-    if (t.rfind("--", 0) != 0) continue;
+    if (t.rfind("--", 0) != 0) {
+      if (bad->empty()) *bad = "unexpected argument '" + t + "'";
+      continue;
+    }
     t = t.substr(2);
     const std::size_t eq = t.find('=');
-    if (eq == std::string::npos) a.v[t] = "";
-    else a.v[t.substr(0, eq)] = t.substr(eq + 1);
+    const std::string key = eq == std::string::npos ? t : t.substr(0, eq);
+    if (!kFlags.count(key) && bad->empty()) *bad = "unknown option --" + key;
+    if (eq == std::string::npos) a.v[key] = "";
+    else a.v[key] = t.substr(eq + 1);
   }
   return a;
 }
@@ -102,12 +141,33 @@ std::vector<std::string> commas(const std::string& s) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  const Args args = parse(argc, argv);
-  if (args.has("help")) {
+  std::string bad;
+  const Args args = parse(argc, argv, &bad);
+  if (args.has("help") || args.has("h")) {
     usage();
     return 0;
   }
+  if (!bad.empty()) {
+    std::fprintf(stderr, "hsi_detect: %s (see --help)\n", bad.c_str());
+    return 2;
+  }
   try {
+    // ---- execution flags that can be refused before any work --------------
+    hsi::SamVariant variant = hsi::SamVariant::Optimized;
+    if (!hsi::parse_sam_variant(args.str("variant", "optimized"), &variant)) {
+      std::fputs("--variant must be baseline, optimized or half\n", stderr);
+      return 2;
+    }
+    const std::string memory = args.str("memory", "copy");
+    if (memory == "zerocopy") {
+      std::fputs("--memory=zerocopy maps host memory into every kernel read; it is meant for integrated "
+                 "GPUs, and the MI355X reads frames from its own HBM: use --memory=copy\n", stderr);
+      return 2;
+    }
+    if (memory != "copy") {
+      std::fputs("--memory must be copy or zerocopy\n", stderr);
+      return 2;
+    }
     const bool quiet = args.has("quiet");
     // ---- source -----------------------------------------------------------
     std::unique_ptr<hsi::FrameSource> source;
@@ -129,7 +189,12 @@ int main(int argc, char** argv) {
         std::fputs("--source=envi needs --envi=<header>[,<header>...]\n", stderr);
         return 2;
       }
-      source = hsi::make_envi_replay_source(commas(args.str("envi", "")), true);
+      source = hsi::make_envi_replay_source(commas(args.str("envi", "")), true,
+                                            static_cast<int>(args.integer("resident", 4)));
+    } else if (kind == "hot") {
+      std::fputs("--source=hot (folders of snapshot-mosaic PNG frames) is not part of this build; "
+                 "decode the frames and demosaic them on the device with hsi_demosaic\n", stderr);
+      return 2;
     } else {
       std::fprintf(stderr, "unknown --source=%s\n", kind.c_str());
       return 2;
@@ -150,8 +215,13 @@ int main(int argc, char** argv) {
       return 2;
     }
     opt.precision = prec == "mixed" ? HSI_MIXED : HSI_PLAIN;
+    opt.variant = variant;
     opt.loading = args.real("loading", 1e-6);
     opt.detection.threshold = static_cast<float>(args.real("threshold", 0.10));
+    opt.thresholds.ace = static_cast<float>(args.real("ace-threshold", 0.5));
+    opt.thresholds.cem = static_cast<float>(args.real("cem-threshold", 0.5));
+    opt.thresholds.rx = static_cast<float>(args.real("rx-threshold", std::nan("")));
+    opt.thresholds.rx_false_alarm_rate = args.real("rx-false-alarm", 1e-4);
     opt.detection.nms_radius = static_cast<int>(args.integer("nms", 2));
     opt.detection.max_detections = static_cast<int>(args.integer("max-detections", 4096));
     opt.num_streams = static_cast<int>(args.integer("streams", 3));
@@ -179,6 +249,12 @@ int main(int argc, char** argv) {
       for (hsi::Detector d : opt.detectors) csv << ",found_" << hsi::to_string(d);
       csv << "\n";
     }
+    std::ofstream dets_out;
+    if (args.has("dump-detections")) {
+      dets_out.open(args.str("dump-detections", ""));
+      if (!dets_out) throw std::runtime_error("cannot write " + args.str("dump-detections", ""));
+      dets_out << "frame,detector,x,y,target,score\n";
+    }
     hsi::FrameResult last;
     bool stats_failed = false;
     const bool ok = pipeline->run(
@@ -190,6 +266,16 @@ int main(int argc, char** argv) {
                 << (r.statistics_ok ? 1 : 0);
             for (const auto& dr : r.results) csv << "," << dr.detections_found;
             csv << "\n";
+          }
+          if (dets_out.is_open()) {
+            char line[160];
+            for (const auto& dr : r.results)
+              for (const hsi::Detection& d : dr.detections) {
+                std::snprintf(line, sizeof line, "%llu,%s,%d,%d,%d,%.9g\n",
+                              static_cast<unsigned long long>(r.meta.index), hsi::to_string(dr.detector), d.x, d.y,
+                              d.target, static_cast<double>(d.score));
+                dets_out << line;
+              }
           }
           stats_failed |= !r.statistics_ok;
           last = r;

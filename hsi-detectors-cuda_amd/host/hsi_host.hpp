@@ -13,6 +13,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -143,15 +144,39 @@ struct SyntheticOptions {
 std::unique_ptr<FrameSource> make_synthetic_source(const SyntheticOptions& options);
 
 // Replays ENVI cubes (all of one shape) from memory, cycling when `loop`.
+// At most `max_resident` of the headers (the first ones) are loaded, as the
+// reference's EnviReplayOptions::max_resident (frame_source.hpp:60-70).
 std::unique_ptr<FrameSource> make_envi_replay_source(const std::vector<std::string>& hdr_paths,
-                                                     bool loop);
+                                                     bool loop, int max_resident = 4);
 
 // ---- pipeline (pipeline.hpp:101-120) ----------------------------------------
+
+// The reference's SamVariant (sam_cuda.hpp:19-36) as the pipeline selects it:
+// Baseline and Optimized both score the FP32 cube (this build has one FP32
+// kernel per detector, the tuned one); Half narrows each uploaded frame to
+// FP16 on the device and scores that, as pipeline.cu:290-296 does for SAM.
+enum class SamVariant { Baseline, Optimized, Half };
+bool parse_sam_variant(const std::string& name, SamVariant* out);
+const char* to_string(SamVariant variant);
+
+// Detection threshold per detector. SAM uses DetectionParams::threshold (an
+// angle in radians, as in the reference). RX uses `rx` when it is a number,
+// else the chi-squared threshold of the frame's band count at
+// `rx_false_alarm_rate` (rx_threshold_for_false_alarm_rate,
+// detectors_cpu.cpp:189-206). ACE and CEM responses are unit-scaled.
+struct DetectorThresholds {
+  float ace = 0.5f;
+  float cem = 0.5f;
+  float rx = std::numeric_limits<float>::quiet_NaN();
+  double rx_false_alarm_rate = 1e-4;
+};
 
 struct PipelineOptions {
   std::vector<Detector> detectors{Detector::Sam};
   hsi_precision precision = HSI_PLAIN;  // RX/ACE arithmetic form
-  DetectionParams detection;            // polarity is taken per detector
+  SamVariant variant = SamVariant::Optimized;
+  DetectionParams detection;            // SAM threshold, NMS, capacity; polarity is per detector
+  DetectorThresholds thresholds;        // RX, ACE, CEM thresholds
   double loading = 1e-6;                // diagonal loading for the statistics
   // Frames in flight; each slot owns a stream, pinned staging, a device cube,
   // score maps, a detection list and a context.
@@ -199,6 +224,11 @@ struct PipelineStats {
         mean_download = 0, mean_gpu = 0;
   std::uint64_t total_detections = 0;
 };
+
+// The threshold `options` give detector `detector` on frames of `bands`
+// bands; throws std::invalid_argument for an RX false-alarm rate outside
+// (0, 1).
+float detection_threshold(const PipelineOptions& options, Detector detector, int bands);
 
 // Frame-rate detector over a FrameSource. Slots run ahead of each other and
 // results come back in frame order.

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <fstream>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -28,7 +29,9 @@ void usage() {
   --target=NAME          score against this one target only
   --precision=plain|mixed  RX/ACE arithmetic form                   [plain]
   --loading=F            diagonal loading of the statistics          [1e-6]
-  --threshold=F          detection threshold                         [0.10]
+  --threshold=F          detection threshold of the selected detector
+                         [SAM 0.10 rad; RX from --rx-false-alarm; ACE, CEM 0.5]
+  --rx-false-alarm=RATE  RX threshold as a chi-squared false-alarm rate [1e-4]
   --nms=N                non-maximum suppression radius, 0 off          [2]
   --max-detections=N     detection list capacity                     [4096]
   --dump-score=FILE      write the score map as raw float32
@@ -38,15 +41,25 @@ void usage() {
   --help)");
 }
 
-std::map<std::string, std::string> parse(int argc, char** argv) {
+// Every flag this program reads; anything else is refused (exit 2).
+const std::set<std::string> kFlags = {"cube", "library", "detector", "target", "precision", "loading",
+                                      "threshold", "rx-false-alarm", "nms", "max-detections", "dump-score", "dump-target",
+                                      "print", "device", "help", "h"};
+
+std::map<std::string, std::string> parse(int argc, char** argv, std::string* bad) {
   std::map<std::string, std::string> a;
   for (int i = 1; i < argc; ++i) {
     std::string t = argv[i];
-    if (t.rfind("--", 0) != 0) continue;
+    if (t.rfind("--", 0) != 0) {
+      if (bad->empty()) *bad = "unexpected argument '" + t + "'";
+      continue;
+    }
     t = t.substr(2);
     const std::size_t eq = t.find('=');
-    if (eq == std::string::npos) a[t] = "";
-    else a[t.substr(0, eq)] = t.substr(eq + 1);
+    const std::string key = eq == std::string::npos ? t : t.substr(0, eq);
+    if (!kFlags.count(key) && bad->empty()) *bad = "unknown option --" + key;
+    if (eq == std::string::npos) a[key] = "";
+    else a[key] = t.substr(eq + 1);
   }
   return a;
 }
@@ -73,7 +86,12 @@ void dump(const std::string& path, const std::vector<T>& v) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  const auto args = parse(argc, argv);
+  std::string bad;
+  const auto args = parse(argc, argv, &bad);
+  if (!bad.empty() && !args.count("help") && !args.count("h")) {
+    std::fprintf(stderr, "hsi_score: %s (see --help)\n", bad.c_str());
+    return 2;
+  }
   auto get = [&](const char* k, const std::string& d) {
     auto it = args.find(k);
     return it == args.end() || it->second.empty() ? d : it->second;
@@ -84,9 +102,9 @@ int main(int argc, char** argv) {
     return 2;
   }
   const bool need_lib = hsi::traits_of(det).needs_targets;
-  if (args.count("help") || !args.count("cube") || (need_lib && !args.count("library"))) {
+  if (args.count("help") || args.count("h") || !args.count("cube") || (need_lib && !args.count("library"))) {
     usage();
-    return args.count("help") ? 0 : 2;
+    return args.count("help") || args.count("h") ? 0 : 2;
   }
   hsi_ctx* ctx = nullptr;
   try {
@@ -147,7 +165,13 @@ int main(int argc, char** argv) {
                      nullptr),
            "score");
     hsi_detection_params p;
-    p.threshold = std::stof(get("threshold", "0.10"));
+    // --threshold applies to the selected detector; without it each detector
+    // takes the pipeline's default (SAM 0.10 rad, RX from --rx-false-alarm,
+    // ACE and CEM 0.5).
+    hsi::PipelineOptions defaults;
+    defaults.thresholds.rx_false_alarm_rate = std::stod(get("rx-false-alarm", "1e-4"));
+    p.threshold = args.count("threshold") ? std::stof(get("threshold", "0.10"))
+                                          : hsi::detection_threshold(defaults, det, shape.bands);
     p.nms_radius = std::stoi(get("nms", "2"));
     p.max_detections = cap;
     p.higher_is_better = hsi::traits_of(det).polarity == hsi::ScorePolarity::HigherIsBetter ? 1 : 0;

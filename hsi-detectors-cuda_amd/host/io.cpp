@@ -207,6 +207,41 @@ bool parse_detector(const std::string& name, Detector* out) {
   return false;
 }
 
+bool parse_sam_variant(const std::string& name, SamVariant* out) {
+  const std::string n = lowercase(strip(name));
+  if (n == "baseline") *out = SamVariant::Baseline;
+  else if (n == "optimized") *out = SamVariant::Optimized;
+  else if (n == "half") *out = SamVariant::Half;
+  else return false;
+  return true;
+}
+
+const char* to_string(SamVariant v) {
+  switch (v) {
+    case SamVariant::Baseline: return "baseline";
+    case SamVariant::Optimized: return "optimized";
+    case SamVariant::Half: return "half";
+  }
+  return "?";
+}
+
+float detection_threshold(const PipelineOptions& o, Detector d, int bands) {
+  switch (d) {
+    case Detector::Sam: return o.detection.threshold;
+    case Detector::Ace: return o.thresholds.ace;
+    case Detector::Cem: return o.thresholds.cem;
+    case Detector::Rx: {
+      if (!std::isnan(o.thresholds.rx)) return o.thresholds.rx;
+      const double t = hsi_rx_threshold(bands, o.thresholds.rx_false_alarm_rate);
+      if (std::isnan(t))
+        throw std::invalid_argument("the RX false-alarm rate must lie in (0, 1), got " +
+                                    std::to_string(o.thresholds.rx_false_alarm_rate));
+      return static_cast<float>(t);
+    }
+  }
+  throw std::invalid_argument("unknown detector");
+}
+
 bool parse_detector_list(const std::string& names, std::vector<Detector>* out) {
   std::vector<Detector> list;
   for (const std::string& f : comma_fields(names)) {

@@ -3,7 +3,7 @@
 // Role of the reference's CudaPipeline (pipeline.cu:60-438): frames flow from
 // a FrameSource through pinned staging to the device, are scored and
 // compacted there, and come back as detection lists in frame order. The
-// MI355X design differs in three ways:
+// MI355X design differs in these ways:
 //   - every slot owns a whole hsi_ctx (library, statistics, scratch), so any
 //     of the four detectors runs per frame, RX/ACE/CEM with statistics
 //     recomputed from that frame on the device (moments -> FP64 build);
@@ -11,13 +11,20 @@
 //     download - is captured once as a hipGraph and replayed per frame, so a
 //     frame costs one launch call on the host;
 //   - SAM and CEM share one read of the cube (hsi_score_sam_cem) and ACE
-//     produces the RX map as a side output when both are requested.
+//     produces the RX map as a side output when both are requested;
+//   - each detector thresholds with its own value (DetectorThresholds; RX
+//     from a false-alarm rate by default), where the reference pipeline only
+//     ever thresholds SAM angles.
+// With SamVariant::Half each uploaded frame is narrowed to FP16 on the device
+// and every detector scores the FP16 cube, as pipeline.cu:290-296 does for
+// SAM; the statistics still accumulate from the FP32 upload.
 // Host code only: device work goes through the hsi_mi355x C ABI; this file
 // uses the HIP runtime for streams, events, pinned memory and graphs.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -53,6 +60,7 @@ struct Slot {
   hsi_ctx* ctx = nullptr;
   float* staging = nullptr;  // pinned, stride * bands
   float* d_cube = nullptr;
+  void* d_half = nullptr;  // FP16 copy of the frame (SamVariant::Half)
   std::vector<float*> d_score;
   std::vector<std::int32_t*> d_target;
   hsi_detection* d_dets = nullptr;   // [detectors][capacity]
@@ -93,6 +101,8 @@ class HipPipeline final : public Pipeline {
     lib_values_ = library.flatten();
     lib_norms_ = library.norms();
     lib_targets_ = library.size();
+    half_ = opt_.variant == SamVariant::Half;
+    for (Detector d : opt_.detectors) thresholds_.push_back(detection_threshold(opt_, d, shape_.bands));
     stride_ = hsi_plane_stride(shape_.height, shape_.width);
     cap_ = std::max(opt_.detection.max_detections, 1);
     hip_check(hipSetDevice(opt_.device), "select device");
@@ -124,6 +134,7 @@ class HipPipeline final : public Pipeline {
       for (float* p : s.d_score) (void)hipFree(p);
       for (std::int32_t* p : s.d_target) (void)hipFree(p);
       (void)hipFree(s.d_cube);
+      if (s.d_half) (void)hipFree(s.d_half);
       (void)hipFree(s.d_dets);
       (void)hipFree(s.d_counts);
       (void)hipHostFree(s.staging);
@@ -208,6 +219,13 @@ class HipPipeline final : public Pipeline {
     d += ", " + std::to_string(lib_targets_) + " targets, " + std::to_string(slots_.size()) + " streams, ";
     d += opt_.use_graph ? "hipGraph frames" : "eager launches";
     d += opt_.precision == HSI_MIXED ? ", mixed precision" : ", plain precision";
+    d += std::string(", ") + (half_ ? "FP16" : "FP32") + " cube (variant " + to_string(opt_.variant) + ")";
+    d += ", thresholds";
+    for (std::size_t i = 0; i < opt_.detectors.size(); ++i) {
+      char buf[64];
+      std::snprintf(buf, sizeof buf, " %s=%.6g", to_string(opt_.detectors[i]), static_cast<double>(thresholds_[i]));
+      d += buf;
+    }
     if (opt_.prefill) d += ", prefilled ring";
     d += "; source: " + source_->describe();
     return d;
@@ -228,6 +246,7 @@ class HipPipeline final : public Pipeline {
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&s.staging), bytes, hipHostMallocDefault), "pinned staging");
     std::memset(s.staging, 0, bytes);
     hip_check(hipMalloc(reinterpret_cast<void**>(&s.d_cube), bytes), "device cube");
+    if (half_) hip_check(hipMalloc(&s.d_half, stride_ * shape_.bands * sizeof(std::uint16_t)), "device FP16 cube");
     for (std::size_t i = 0; i < nd; ++i) {
       float* sc = nullptr;
       std::int32_t* tg = nullptr;
@@ -279,6 +298,11 @@ class HipPipeline final : public Pipeline {
     if (events) hip_check(hipEventRecord(s.ev[kStart], s.stream), "event");
     hip_check(hipMemcpyAsync(s.d_cube, s.staging, stride_ * B * sizeof(float), hipMemcpyHostToDevice, s.stream),
               "upload frame");
+    if (half_)
+      hsi_check(s.ctx, hsi_narrow_to_half(s.d_cube, s.d_half, stride_ * static_cast<std::size_t>(B), st),
+                "narrow to FP16");
+    const void* cube = half_ ? s.d_half : static_cast<const void*>(s.d_cube);
+    const hsi_dtype dtype = half_ ? HSI_F16 : HSI_F32;
     if (events) hip_check(hipEventRecord(s.ev[kUploaded], s.stream), "event");
     if (need_stats_) {
       hsi_check(s.ctx, hsi_stats_accumulate(s.ctx, s.d_cube, H, W, B, stride_, opt_.precision, st), "moments");
@@ -290,25 +314,25 @@ class HipPipeline final : public Pipeline {
     const int i_rx = index_of(Detector::Rx), i_ace = index_of(Detector::Ace);
     if (i_sam >= 0 || i_cem >= 0)
       hsi_check(s.ctx,
-                hsi_score_sam_cem(s.ctx, s.d_cube, HSI_F32, H, W, B, stride_, i_sam >= 0 ? s.d_score[i_sam] : nullptr,
+                hsi_score_sam_cem(s.ctx, cube, dtype, H, W, B, stride_, i_sam >= 0 ? s.d_score[i_sam] : nullptr,
                                   i_sam >= 0 ? s.d_target[i_sam] : nullptr, i_cem >= 0 ? s.d_score[i_cem] : nullptr,
                                   i_cem >= 0 ? s.d_target[i_cem] : nullptr, st),
                 "score SAM/CEM");
     if (i_ace >= 0)
       hsi_check(s.ctx,
-                hsi_score(s.ctx, HSI_DET_ACE, opt_.precision, s.d_cube, HSI_F32, H, W, B, stride_, s.d_score[i_ace],
+                hsi_score(s.ctx, HSI_DET_ACE, opt_.precision, cube, dtype, H, W, B, stride_, s.d_score[i_ace],
                           s.d_target[i_ace], i_rx >= 0 ? s.d_score[i_rx] : nullptr, st),
                 "score ACE");
     else if (i_rx >= 0)
       hsi_check(s.ctx,
-                hsi_score(s.ctx, HSI_DET_RX, opt_.precision, s.d_cube, HSI_F32, H, W, B, stride_, s.d_score[i_rx],
+                hsi_score(s.ctx, HSI_DET_RX, opt_.precision, cube, dtype, H, W, B, stride_, s.d_score[i_rx],
                           nullptr, nullptr, st),
                 "score RX");
     if (events) hip_check(hipEventRecord(s.ev[kScored], s.stream), "event");
     for (std::size_t i = 0; i < nd; ++i) {
       const Detector d = opt_.detectors[i];
       hsi_detection_params p;
-      p.threshold = opt_.detection.threshold;
+      p.threshold = thresholds_[i];
       p.nms_radius = opt_.detection.nms_radius;
       p.max_detections = opt_.detection.max_detections;
       p.higher_is_better = traits_of(d).polarity == ScorePolarity::HigherIsBetter ? 1 : 0;
@@ -409,6 +433,8 @@ class HipPipeline final : public Pipeline {
   std::size_t stride_ = 0;
   std::size_t cap_ = 1;
   bool need_stats_ = false;
+  bool half_ = false;
+  std::vector<float> thresholds_;  // per detector, in PipelineOptions::detectors order
   std::vector<float> lib_values_, lib_norms_;
   int lib_targets_ = 0;
   std::string device_name_;

@@ -61,14 +61,16 @@ class SyntheticSource final : public FrameSource {
 
 class EnviReplaySource final : public FrameSource {
  public:
-  EnviReplaySource(const std::vector<std::string>& paths, bool loop) : loop_(loop) {
+  EnviReplaySource(const std::vector<std::string>& paths, bool loop, int max_resident) : loop_(loop) {
     if (paths.empty()) throw std::invalid_argument("ENVI replay needs at least one header");
+    const std::size_t keep = static_cast<std::size_t>(max_resident < 1 ? 1 : max_resident);
     for (const std::string& p : paths) {
+      if (frames_.size() >= keep) break;
       frames_.push_back(read_envi(p));
       if (frames_.back().shape != frames_.front().shape)
         throw std::invalid_argument("ENVI replay: " + p + " has a different shape from " + paths.front());
     }
-    names_ = paths;
+    names_.assign(paths.begin(), paths.begin() + static_cast<std::ptrdiff_t>(frames_.size()));
   }
   CubeShape shape() const override { return frames_.front().shape; }
   std::uint64_t length() const override { return loop_ ? 0 : frames_.size(); }
@@ -107,8 +109,9 @@ std::unique_ptr<FrameSource> make_synthetic_source(const SyntheticOptions& optio
   return std::unique_ptr<FrameSource>(new SyntheticSource(options));
 }
 
-std::unique_ptr<FrameSource> make_envi_replay_source(const std::vector<std::string>& hdr_paths, bool loop) {
-  return std::unique_ptr<FrameSource>(new EnviReplaySource(hdr_paths, loop));
+std::unique_ptr<FrameSource> make_envi_replay_source(const std::vector<std::string>& hdr_paths, bool loop,
+                                                     int max_resident) {
+  return std::unique_ptr<FrameSource>(new EnviReplaySource(hdr_paths, loop, max_resident));
 }
 
 }  // namespace hsi

@@ -39,3 +39,18 @@ def same_bits(a, b):
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b)
     return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def near_cut(score, h, w, thr, nms, tol):
+    """Pixels whose score is within tol of the threshold or of a neighbour's
+    score inside the NMS window: where a `tol` relative score difference may
+    flip whether the pixel is listed."""
+    sc = score.reshape(h, w).astype(np.float64)
+    near = np.abs(sc - thr) <= tol * max(abs(thr), 1.0)
+    pad = np.pad(sc, nms, constant_values=np.nan)
+    for dy in range(-nms, nms + 1):
+        for dx in range(-nms, nms + 1):
+            if dy or dx:
+                nb = pad[nms + dy:nms + dy + h, nms + dx:nms + dx + w]
+                near |= np.abs(nb - sc) <= tol * np.maximum(np.abs(sc), 1.0)
+    return near.reshape(-1)

@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 from conftest import SCENES, golden
-from gpu_util import ANGLE_TOL, FLOOR, REL, device_buffers, rel_err, same_bits, upload_golden_stats
+from gpu_util import (ANGLE_TOL, FLOOR, REL, device_buffers, near_cut, rel_err, same_bits,
+                      upload_golden_stats)
 
 pytestmark = pytest.mark.gpu
 
@@ -297,21 +298,6 @@ def test_end_to_end_scores_with_device_statistics(H, torch, ctx, name):
         e = rel_err(got.cpu().numpy(), g[f"{key}_score"], FLOOR[key])
         print(f"{name}: end-to-end {key} max rel {e.max():.3g}")
         assert e.max() <= REL, key
-
-
-def near_cut(score, h, w, thr, nms, tol):
-    """Pixels whose score is within tol of the threshold or of a neighbour's
-    score inside the NMS window: where a 1e-5 score difference may flip
-    whether the pixel is listed."""
-    sc = score.reshape(h, w).astype(np.float64)
-    near = np.abs(sc - thr) <= tol * max(abs(thr), 1.0)
-    pad = np.pad(sc, nms, constant_values=np.nan)
-    for dy in range(-nms, nms + 1):
-        for dx in range(-nms, nms + 1):
-            if dy or dx:
-                nb = pad[nms + dy:nms + dy + h, nms + dx:nms + dx + w]
-                near |= np.abs(nb - sc) <= tol * np.maximum(np.abs(sc), 1.0)
-    return near.reshape(-1)
 
 
 def test_end_to_end_detection_lists(H, torch, ctx, oracle_lib):

@@ -54,6 +54,30 @@ def write_csv(path, library, names=None):
 
 # ---- CPU --------------------------------------------------------------------
 
+@pytest.mark.parametrize("args,needle", [
+    (["hsi_detect", "--bogus=1"], "unknown option --bogus"),
+    (["hsi_detect", "stray"], "unexpected argument"),
+    (["hsi_detect", "--variant=fast"], "--variant must be"),
+    (["hsi_detect", "--memory=zerocopy"], "--memory=copy"),
+    (["hsi_detect", "--memory=pinned"], "--memory must be"),
+    (["hsi_detect", "--source=hot", "--hot-dir=x"], "unknown option --hot-dir"),
+    (["hsi_detect", "--source=hot"], "hsi_demosaic"),
+    (["hsi_score", "--cube=x.hdr", "--frobnicate"], "unknown option --frobnicate"),
+])
+def test_cli_refuses_unknown_flags_and_unsupported_modes(args, needle):
+    """The reference's flags are honoured or refused with exit 2, never
+    silently ignored (main.cpp:181-188 refuses bad --variant/--memory values)."""
+    r = run(args)
+    assert r.returncode == 2, (r.returncode, r.stdout, r.stderr)
+    assert needle in r.stderr
+
+
+def test_cli_refuses_an_invalid_rx_false_alarm_rate():
+    r = run(["hsi_detect", "--detector=rx", "--width=16", "--height=16", "--rx-false-alarm=2"])
+    assert r.returncode == 1
+    assert "false-alarm rate must lie in (0, 1)" in r.stderr
+
+
 def test_cli_help_and_usage():
     for tool in ("hsi_score", "hsi_detect"):
         r = run([tool, "--help"])
@@ -209,7 +233,8 @@ def test_hsi_detect_graph_replay_equals_eager_and_python(tmp_path):
         ctx.stats_accumulate(cube, 96, 160, 113)
         ctx.stats_build(1e-6)
         ctx.score(H.Detector.CEM, cube, 96, 160, 113, s, t)
-        ctx.detect(s, t, 96, 160, H.DetectionParams(0.1, 2, 4096, True), out, cnt)
+        # CEM's own threshold: the pipeline default of 0.5
+        ctx.detect(s, t, 96, 160, H.DetectionParams(0.5, 2, 4096, True), out, cnt)
         torch.cuda.synchronize()
         assert int(cnt.item()) == int(row["found_cem"]), k
     ctx.close()
@@ -221,3 +246,95 @@ def test_hsi_detect_prefill_and_mixed(tmp_path):
     assert summ["frames"] == 6 and summ["statistics_ok"]
     # a prefilled ring of 2 slots replays frames 0 and 1
     assert np.array_equal(rows["found_sam"][0::2], rows["found_sam"][0::2][:1].repeat(3))
+
+
+def read_dumped_lists(path):
+    rows = np.genfromtxt(path, delimiter=",", names=True, dtype=None, encoding=None)
+    rows = np.atleast_1d(rows)
+    out = {}
+    for r in rows:
+        out.setdefault((int(r["frame"]), str(r["detector"])), []).append(
+            (int(r["x"]), int(r["y"]), int(r["target"]), float(r["score"])))
+    return out
+
+
+def oracle_frame_lists(H, oracle_lib, cfg, frame, thresholds, nms, half=False):
+    """The oracle's four lists for one seeded frame: statistics from the FP32
+    frame; with half=True the scorers read the frame rounded to FP16 (the
+    pipeline's --variant=half)."""
+    h, w, b = cfg.height, cfg.width, cfg.bands
+    p = h * w
+    lib = H.synth_library(cfg)
+    host = H.synth_frame(cfg, frame)[:, :p]
+    st = oracle_lib.statistics_of_cube(np.ascontiguousarray(host).reshape(b, h, w), lib, 1e-6)
+    assert st.valid, st.error
+    cube = host.astype(np.float16).astype(np.float32) if half else host
+    cube = np.ascontiguousarray(cube).reshape(b, h, w)
+    scores = {"sam": oracle_lib.sam_best(cube, lib),
+              "rx": (oracle_lib.rx(cube, st), None),
+              "ace": oracle_lib.ace_best(cube, st)[:2],
+              "cem": oracle_lib.cem_best(cube, st)}
+    out = {}
+    for det, (sc, tg) in scores.items():
+        higher = det != "sam"
+        thr = thresholds[det]
+        dets, found = oracle_lib.detect(sc, tg, h, w, thr, nms, higher)
+        out[det] = (sc, [(int(d["x"]), int(d["y"]), int(d["target"]), float(d["score"])) for d in dets], found)
+    return out
+
+
+def compare_pipeline_lists(got_lists, want, h, w, nms, frames, tol):
+    from gpu_util import near_cut
+    report = {}
+    for k in frames:
+        for det, (sc, wl, wfound) in want[k].items():
+            gl = got_lists.get((k, det), [])
+            gset = {(y * w + x, t) for x, y, t, _ in gl}
+            wset = {(y * w + x, t) for x, y, t, _ in wl}
+            if det == "sam":
+                # SAM scores are the reference's to the bit: identical lists
+                assert gset == wset, (k, det, sorted(gset ^ wset)[:5])
+                assert [g[:3] for g in gl] == [v[:3] for v in wl], (k, det)
+                continue
+            near = near_cut(sc, h, w, thr_of(det, want, k), nms, tol)
+            off = [(q, t) for q, t in gset ^ wset if not near[q]]
+            assert not off, (k, det, off[:10])
+            report[(k, det)] = (len(gset), len(wset), len(gset ^ wset))
+    return report
+
+
+def thr_of(det, want, k):
+    return want["_thr"][det]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["optimized", "half"])
+def test_hsi_detect_lists_match_the_oracle_per_detector(tmp_path, oracle_lib, variant):
+    """hsi_detect --detector=sam,rx,ace,cem over four seeded frames (two
+    eager, two replayed from captured graphs per slot), each detector with its
+    own threshold (RX from a false-alarm rate): every frame's four lists equal
+    detect_cpu on the oracle's scores - SAM element for element, RX/ACE/CEM as
+    (pixel, target) sets apart from pixels within 1e-5 of the threshold or of
+    an NMS neighbour. With --variant=half the oracle scores the FP16-rounded
+    frame (statistics from the FP32 frame, as the pipeline does)."""
+    import hsi_mi355x as H
+    h, w, b, t, seed = 96, 160, 113, 8, 77
+    rate, ace_thr, cem_thr = 1e-3, 0.3, 0.5
+    dump = str(tmp_path / "lists.csv")
+    args = ["hsi_detect", f"--width={w}", f"--height={h}", f"--bands={b}", f"--targets={t}", f"--seed={seed}",
+            "--frames=4", "--warmup=0", "--streams=2", "--detector=sam,rx,ace,cem", "--threshold=0.1",
+            f"--rx-false-alarm={rate}", f"--ace-threshold={ace_thr}", f"--cem-threshold={cem_thr}",
+            "--max-detections=20000", f"--variant={variant}", f"--dump-detections={dump}", "--quiet"]
+    print(" ".join(args), flush=True)
+    r = run(args, timeout=180)
+    assert r.returncode == 0, r.stderr
+    got = read_dumped_lists(dump)
+    cfg = H.synth_config(h, w, b, t, seed=seed)
+    thresholds = {"sam": 0.1, "rx": float(np.float32(H.rx_threshold(b, rate))), "ace": ace_thr, "cem": cem_thr}
+    want = {k: oracle_frame_lists(H, oracle_lib, cfg, k, thresholds, 2, half=variant == "half")
+            for k in range(4)}
+    want["_thr"] = thresholds
+    report = compare_pipeline_lists(got, want, h, w, 2, range(4), 1e-5)
+    for (k, det), (ng, nw, nd) in sorted(report.items()):
+        print(f"frame {k} {det}: {ng} device / {nw} oracle, {nd} differ at near-threshold or near-tie pixels")
+        assert nw > 0
