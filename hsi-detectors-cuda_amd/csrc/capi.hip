@@ -74,6 +74,12 @@ struct hsi_ctx {
   void* d_screen_counts = nullptr;
   std::size_t screen_counts_bytes = 0;
 
+  // Q16 code (hsi_encode_q16 / hsi_set_q16_code): per-band offset and scale.
+  int q16_bands = 0;
+  float* d_q16_offset = nullptr;
+  float* d_q16_scale = nullptr;
+  unsigned* d_q16_extrema = nullptr;
+
   // Demosaic band positions (uploaded when they change).
   int* d_positions = nullptr;
   std::vector<int> positions;
@@ -120,8 +126,10 @@ hipError_t grow(void** ptr, std::size_t* have, std::size_t need) {
   return err;
 }
 
+// Q16 cubes are accepted only where allow_q16 (the mixed statistics and the
+// mixed RX/ACE kernels).
 hsi_status check_cube(hsi_ctx* ctx, const void* d_cube, int height, int width, int bands,
-                      std::size_t stride, hsi_dtype dtype) {
+                      std::size_t stride, hsi_dtype dtype, bool allow_q16 = false) {
   if (!d_cube) return fail(ctx, HSI_ERR_INVALID_ARG, "null cube");
   if (height <= 0 || width <= 0 || bands <= 0)
     return fail(ctx, HSI_ERR_INVALID_ARG, "cube shape must be positive");
@@ -132,7 +140,10 @@ hsi_status check_cube(hsi_ctx* ctx, const void* d_cube, int height, int width, i
   if (stride < pixels || stride % 8 != 0)
     return fail(ctx, HSI_ERR_INVALID_ARG,
                 "plane stride must be at least height*width and a multiple of 8 (use hsi_plane_stride)");
-  if (dtype != HSI_F32 && dtype != HSI_F16) return fail(ctx, HSI_ERR_INVALID_ARG, "unknown cube dtype");
+  if (dtype == HSI_Q16 && !allow_q16)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "Q16 cubes are read by the MIXED statistics and MIXED RX/ACE only");
+  if (dtype != HSI_F32 && dtype != HSI_F16 && dtype != HSI_Q16)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "unknown cube dtype");
   if (reinterpret_cast<std::uintptr_t>(d_cube) % 16 != 0)
     return fail(ctx, HSI_ERR_INVALID_ARG, "cube must be 16-byte aligned");
   return HSI_OK;
@@ -216,7 +227,7 @@ void hsi_ctx_destroy(hsi_ctx* ctx) {
                   ctx->d_mom_scratch, ctx->d_det_scratch, ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v,
                   ctx->d_mlp_grad, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, ctx->d_mlp_act,
                   ctx->d_positions, ctx->d_screen_rec, ctx->d_screen_counts,
-                  ctx->d_eval};
+                  ctx->d_eval, ctx->d_q16_offset, ctx->d_q16_scale, ctx->d_q16_extrema};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete ctx;
@@ -254,6 +265,9 @@ hsi_status hsi_ctx_create(int device, hsi_ctx** out) {
   if (err == hipSuccess) err = alloc(&ctx->d_col_scale, B);
   if (err == hipSuccess) err = alloc(&ctx->d_row_inv, B);
   if (err == hipSuccess) err = alloc(&ctx->d_target_inv, T);
+  if (err == hipSuccess) err = alloc(&ctx->d_q16_offset, B);
+  if (err == hipSuccess) err = alloc(&ctx->d_q16_scale, B);
+  if (err == hipSuccess) err = alloc(&ctx->d_q16_extrema, q16_extrema_words(kMaxBands));
   if (err == hipSuccess) err = hipMemset(ctx->d_sum_outer, 0, B * B * sizeof(double));
   if (err == hipSuccess) err = hipMemset(ctx->d_status, 0, 2 * sizeof(unsigned));
   if (err == hipSuccess) err = hipDeviceSynchronize();
@@ -336,23 +350,81 @@ hsi_status hsi_set_library(hsi_ctx* ctx, const float* values, const float* norms
   return HSI_OK;
 }
 
-hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const float* d_cube, int height, int width, int bands,
-                                size_t plane_stride, hsi_precision precision, hsi_stream stream) {
+hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype, int height, int width,
+                                int bands, size_t plane_stride, hsi_precision precision,
+                                const float* d_shift_or_null, hsi_stream stream) {
   if (!ctx) return HSI_ERR_INVALID_ARG;
-  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, HSI_F32);
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, dtype, true);
   if (st != HSI_OK) return st;
   if (precision != HSI_PLAIN && precision != HSI_MIXED)
     return fail(ctx, HSI_ERR_INVALID_ARG, "unknown precision");
+  if (precision == HSI_PLAIN && dtype != HSI_F32)
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "the plain (FP64) moments read F32 cubes; use HSI_MIXED for F16 and Q16 cubes");
+  if (dtype == HSI_Q16 && ctx->q16_bands != bands)
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "no Q16 code for " + std::to_string(bands) + " bands: encode with hsi_encode_q16 or install one "
+                "with hsi_set_q16_code");
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
   const std::size_t pixels = static_cast<std::size_t>(height) * width;
   HSI_HIP(ctx, grow(reinterpret_cast<void**>(&ctx->d_mom_scratch), &ctx->mom_scratch_bytes,
                     moments_scratch_doubles(pixels, bands) * sizeof(double)),
           "allocate moment scratch");
-  HSI_HIP(ctx, launch_moments(d_cube, pixels, bands, plane_stride, precision, ctx->d_mom_scratch,
-                              ctx->d_count, ctx->d_sum, ctx->d_sum_outer,
-                              static_cast<hipStream_t>(stream)),
-          "accumulate moments");
+  MomentArgs m;
+  m.cube = d_cube;
+  m.dtype = dtype;
+  m.pixels = pixels;
+  m.bands = bands;
+  m.stride = plane_stride;
+  m.precision = precision;
+  m.shift = precision == HSI_MIXED ? d_shift_or_null : nullptr;
+  m.q_offset = ctx->d_q16_offset;
+  m.q_scale = ctx->d_q16_scale;
+  m.count = ctx->d_count;
+  m.sum = ctx->d_sum;
+  m.sum_outer = ctx->d_sum_outer;
+  HSI_HIP(ctx, launch_moments(m, ctx->d_mom_scratch, static_cast<hipStream_t>(stream)), "accumulate moments");
   ctx->moment_bands = bands;
+  return HSI_OK;
+}
+
+hsi_status hsi_encode_q16(hsi_ctx* ctx, const float* d_src, int height, int width, int bands, size_t plane_stride,
+                          uint16_t* d_dst, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  hsi_status st = check_cube(ctx, d_src, height, width, bands, plane_stride, HSI_F32);
+  if (st != HSI_OK) return st;
+  if (!d_dst || reinterpret_cast<std::uintptr_t>(d_dst) % 16 != 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "Q16 destination must be a 16-byte aligned device buffer");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, launch_encode_q16(d_src, static_cast<std::size_t>(height) * width, bands, plane_stride,
+                                 ctx->d_q16_extrema, d_dst, ctx->d_q16_offset, ctx->d_q16_scale,
+                                 static_cast<hipStream_t>(stream)),
+          "encode Q16");
+  ctx->q16_bands = bands;
+  return HSI_OK;
+}
+
+hsi_status hsi_set_q16_code(hsi_ctx* ctx, int bands, const float* offset, const float* scale) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (bands <= 0 || bands > kMaxBands || !offset || !scale)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "Q16 code needs 1.." + std::to_string(kMaxBands) + " bands of offsets and scales");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, hipMemcpy(ctx->d_q16_offset, offset, bands * sizeof(float), hipMemcpyHostToDevice), "upload Q16 code");
+  HSI_HIP(ctx, hipMemcpy(ctx->d_q16_scale, scale, bands * sizeof(float), hipMemcpyHostToDevice), "upload Q16 code");
+  ctx->q16_bands = bands;
+  return HSI_OK;
+}
+
+hsi_status hsi_get_q16_code(hsi_ctx* ctx, int bands, float* offset, float* scale) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  if (ctx->q16_bands <= 0 || bands != ctx->q16_bands)
+    return fail(ctx, HSI_ERR_NO_STATISTICS, "no Q16 code for this band count");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, hipDeviceSynchronize(), "wait for device");
+  if (offset)
+    HSI_HIP(ctx, hipMemcpy(offset, ctx->d_q16_offset, bands * sizeof(float), hipMemcpyDeviceToHost), "read Q16 code");
+  if (scale)
+    HSI_HIP(ctx, hipMemcpy(scale, ctx->d_q16_scale, bands * sizeof(float), hipMemcpyDeviceToHost), "read Q16 code");
   return HSI_OK;
 }
 
@@ -609,8 +681,12 @@ hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precisio
     default:
       return fail(ctx, HSI_ERR_INVALID_ARG, "unknown detector");
   }
-  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, dtype);
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, dtype, precision == HSI_MIXED);
   if (st != HSI_OK) return st;
+  if (dtype == HSI_Q16 && ctx->q16_bands != bands)
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "no Q16 code for " + std::to_string(bands) + " bands: encode with hsi_encode_q16 or install one "
+                "with hsi_set_q16_code");
   if (bands > kMaxRxBands)
     return fail(ctx, HSI_ERR_CAPACITY, "RX/ACE support at most " + std::to_string(kMaxRxBands) +
                                            " bands (the whitener tile must fit in LDS)");
@@ -640,6 +716,8 @@ hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precisio
   a.col_scale = ctx->d_col_scale;
   a.row_inv = ctx->d_row_inv;
   a.target_inv = ctx->d_target_inv;
+  a.q_offset = ctx->d_q16_offset;
+  a.q_scale = ctx->d_q16_scale;
   if (detector == HSI_DET_RX) {
     a.rx = d_score;
   } else {
@@ -1403,7 +1481,8 @@ hsi_status hsi_batch_statistics(hsi_batch* b, double loading, hsi_stream stream)
     const hsi_scene& sc = b->scenes[i];
     if (!sc.d_rx_score && !sc.d_ace_score && !sc.d_cem_score) continue;
     hsi_ctx* c = b->ctx[i];
-    hsi_status st = hsi_stats_accumulate(c, sc.d_cube, sc.height, sc.width, sc.bands, sc.plane_stride, HSI_PLAIN, stream);
+    hsi_status st = hsi_stats_accumulate(c, sc.d_cube, HSI_F32, sc.height, sc.width, sc.bands, sc.plane_stride,
+                                         HSI_PLAIN, nullptr, stream);
     if (st == HSI_OK) st = hsi_stats_build(c, loading, stream);
     if (st == HSI_OK) st = ensure_layouts(c, static_cast<hipStream_t>(stream));
     if (st != HSI_OK) return bfail(b, st, "scene " + std::to_string(i) + ": " + c->error);

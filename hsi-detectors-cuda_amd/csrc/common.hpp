@@ -40,6 +40,36 @@ __device__ __forceinline__ float buffer_elem(__amdgpu_buffer_rsrc_t r, std::uint
 __device__ __forceinline__ float buffer_elem(__amdgpu_buffer_rsrc_t r, std::uint32_t off, _Float16) {
   return static_cast<float>(__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)));
 }
+
+// Element tag of an HSI_Q16 cube: a 16-bit unsigned code q of a per-band
+// affine map, decoded as fmaf(scale[b], q, offset[b]) (q16_decode). The
+// buffer loads below return the code itself as a float (exact).
+struct q16_t {
+  std::uint16_t code;
+};
+__device__ __forceinline__ float buffer_elem(__amdgpu_buffer_rsrc_t r, std::uint32_t off, q16_t) {
+  return static_cast<float>(static_cast<std::uint16_t>(__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)));
+}
+__device__ __forceinline__ float q16_decode(float code, float scale, float offset) {
+  return __builtin_fmaf(scale, code, offset);
+}
+template <typename In> struct is_q16 { static constexpr bool value = false; };
+template <> struct is_q16<q16_t> { static constexpr bool value = true; };
+
+// Two adjacent elements of one band plane (the first at byte `off`, which
+// must be a multiple of 2 * sizeof(element)), as floats.
+__device__ __forceinline__ float2 buffer_pair(__amdgpu_buffer_rsrc_t r, std::uint32_t off, float) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float2 buffer_pair(__amdgpu_buffer_rsrc_t r, std::uint32_t off, _Float16) {
+  const unsigned w = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+  return make_float2(static_cast<float>(__builtin_bit_cast(_Float16, static_cast<std::uint16_t>(w & 0xffffu))),
+                     static_cast<float>(__builtin_bit_cast(_Float16, static_cast<std::uint16_t>(w >> 16))));
+}
+__device__ __forceinline__ float2 buffer_pair(__amdgpu_buffer_rsrc_t r, std::uint32_t off, q16_t) {
+  const unsigned w = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+  return make_float2(static_cast<float>(w & 0xffffu), static_cast<float>(w >> 16));
+}
 // Whether a cube of `bands` planes of `stride` elements fits one buffer view.
 __host__ __device__ inline bool fits_buffer(int bands, std::size_t stride, std::size_t elem_bytes) {
   return static_cast<std::size_t>(bands) * stride * elem_bytes < (static_cast<std::size_t>(1) << 32) - 256;

@@ -42,9 +42,21 @@ inline int moment_pairs(int bands) {
 }
 // Scratch doubles for the per-workgroup partials.
 std::size_t moments_scratch_doubles(std::size_t pixels, int bands);
-hipError_t launch_moments(const float* cube, std::size_t pixels, int bands, std::size_t stride,
-                          hsi_precision precision, double* scratch, double* count, double* sum,
-                          double* sum_outer, hipStream_t stream);
+struct MomentArgs {
+  const void* cube = nullptr;
+  hsi_dtype dtype = HSI_F32;       // PLAIN: F32 only; MIXED: F32, F16, Q16
+  std::size_t pixels = 0;
+  int bands = 0;
+  std::size_t stride = 0;
+  hsi_precision precision = HSI_PLAIN;
+  const float* shift = nullptr;    // MIXED: caller's reference [bands], or null (sample mean)
+  const float* q_offset = nullptr; // Q16 code [bands]
+  const float* q_scale = nullptr;
+  double* count = nullptr;
+  double* sum = nullptr;
+  double* sum_outer = nullptr;
+};
+hipError_t launch_moments(const MomentArgs& args, double* scratch, hipStream_t stream);
 
 struct BuildArgs {
   int bands = 0;
@@ -121,6 +133,8 @@ struct RxAceArgs {
   const float* col_scale = nullptr;
   const float* row_inv = nullptr;
   const float* target_inv = nullptr;
+  const float* q_offset = nullptr;  // Q16 code [bands] (dtype HSI_Q16)
+  const float* q_scale = nullptr;
   float* rx = nullptr;
   float* ace_score = nullptr;
   int* ace_target = nullptr;
@@ -246,6 +260,14 @@ hipError_t launch_gather(const float* cube, std::size_t stride, std::size_t pixe
                          int n, const float* center, float* out, std::size_t out_stride, hipStream_t stream);
 
 // ---- ingest (ingest.hip) -------------------------------------------------------
+// Q16 encoding of an F32 cube (16-byte aligned planes, stride % 8 == 0):
+// per-band extrema into `extrema` (q16_extrema_words(bands) words of
+// scratch), then the codes into dst and the code itself into q_offset /
+// q_scale [bands].
+std::size_t q16_extrema_words(int bands);
+hipError_t launch_encode_q16(const float* src, std::size_t pixels, int bands, std::size_t stride,
+                             unsigned* extrema, std::uint16_t* dst, float* q_offset, float* q_scale,
+                             hipStream_t stream);
 // raw_type: 0 = u8, 1 = u16, 2 = f32; d_position[bands] = mosaic positions.
 hipError_t launch_demosaic(const void* raw, int raw_type, int raw_height, int raw_width, int period, int bands,
                            const int* d_position, float* cube, std::size_t stride, hipStream_t stream);

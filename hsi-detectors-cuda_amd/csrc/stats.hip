@@ -17,8 +17,9 @@
 //  PLAIN  v_mfma_f64_16x16x4 on the widened floats: every product of two
 //         floats is exact in double, so the only difference from the CPU
 //         oracle is the order of the FP64 sums.
-//  MIXED  the shifted moments of d = x - r (r: a per-band reference near the
-//         mean, from a sparse sample of the frame) on v_mfma_f32_16x16x32_f16
+//  MIXED  the shifted moments of d = x - r (r: the caller's per-band
+//         reference, e.g. the previous frame's mean, or the mean of a sparse
+//         sample of the frame) over an F32, F16 or Q16 cube on v_mfma_f32_16x16x32_f16
 //         with d split into FP16 hi + lo parts (hi*hi + hi*lo + lo*hi), FP32
 //         accumulation over 128-pixel tiles, and FP64 across tiles - the
 //         hierarchy the reference designed for its device accumulator
@@ -116,6 +117,43 @@ __device__ __forceinline__ float4 stage_fix(float4 v, int bands, int b, std::siz
   if (px + 2 >= p_end) v.z = 0.f;
   if (px + 3 >= p_end) v.w = 0.f;
   return v;
+}
+
+// The mixed form also stages 16-bit cubes (F16, Q16): four elements per
+// 8-byte load, widened to floats (Q16 decoded with its band's code) before
+// the shift.
+template <typename In> struct Stage4 { typedef uint2 type; };
+template <> struct Stage4<float> { typedef float4 type; };
+
+template <typename In>
+__device__ __forceinline__ typename Stage4<In>::type stage_raw_t(const void* __restrict__ cube, std::size_t stride,
+                                                                 int bands, int b, std::size_t px,
+                                                                 std::size_t p_begin, std::size_t p_end) {
+  if constexpr (sizeof(In) == 4) {
+    return stage_raw(static_cast<const float*>(cube), stride, bands, b, px, p_begin, p_end);
+  } else {
+    const int bc = b < bands ? b : bands - 1;
+    const std::size_t pc = px < p_end ? px : p_begin;
+    return *reinterpret_cast<const uint2*>(static_cast<const std::uint16_t*>(cube) +
+                                           static_cast<std::size_t>(bc) * stride + pc);
+  }
+}
+
+// Raw staged elements as floats; qo/qs: the Q16 code of this band row.
+template <typename In>
+__device__ __forceinline__ float4 stage_widen(typename Stage4<In>::type r, float qo, float qs) {
+  if constexpr (sizeof(In) == 4) {
+    return r;
+  } else {
+    const std::uint16_t e[4] = {static_cast<std::uint16_t>(r.x & 0xffffu), static_cast<std::uint16_t>(r.x >> 16),
+                                static_cast<std::uint16_t>(r.y & 0xffffu), static_cast<std::uint16_t>(r.y >> 16)};
+    float v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      v[i] = is_q16<In>::value ? q16_decode(static_cast<float>(e[i]), qs, qo)
+                               : static_cast<float>(__builtin_bit_cast(_Float16, e[i]));
+    return make_float4(v[0], v[1], v[2], v[3]);
+  }
 }
 
 // Launch shape of each form, per block count NB. Eight waves per workgroup
@@ -380,11 +418,11 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const flo
 // pitch of CH + 8 halves (16 lanes reading 16 bytes each at rows m = 0..15
 // land on 16 disjoint bank quads). Each owner position is its own code path,
 // holding only its run's accumulators and operand blocks.
-template <int NB, int POS>
-__device__ __forceinline__ void moments_mixed_owner(const float* __restrict__ cube, std::size_t p_begin,
+template <typename In, int NB, int POS>
+__device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cube, std::size_t p_begin,
                                                     std::size_t p_end, int bands, std::size_t stride,
-                                                    const float* sh, _Float16* hi_t, _Float16* lo_t,
-                                                    double* __restrict__ out) {
+                                                    const float* sh, const float* qo, const float* qs,
+                                                    _Float16* hi_t, _Float16* lo_t, double* __restrict__ out) {
   constexpr int CH = f16_chunk<NB>();
   constexpr int NP = NB * (NB + 1) / 2;
   constexpr int OWNERS = kWavesPerWg * f16_splits<NB>();
@@ -407,13 +445,13 @@ __device__ __forceinline__ void moments_mixed_owner(const float* __restrict__ cu
   }
   const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
 
-  float4 pre[PER];
+  typename Stage4<In>::type pre[PER];
   auto load = [&](std::size_t p) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + THREADS * k;
       const int b = e / (CH / 4);
-      pre[k] = stage_raw(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_begin, p_end);
+      pre[k] = stage_raw_t<In>(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_begin, p_end);
     }
   };
   auto store = [&](int buf, std::size_t p) {
@@ -426,7 +464,9 @@ __device__ __forceinline__ void moments_mixed_owner(const float* __restrict__ cu
         const std::size_t px = p + k4;
         // Shift real bands only, and only real pixels (padding stays zero).
         const float s = b < bands ? sh[b] : 0.f;
-        const float4 f = stage_fix(pre[k], bands, b, px, p_end, true);
+        // (b < ROWS here; the Q16 code arrays exist only for Q16 cubes)
+        const float bqo = is_q16<In>::value ? qo[b] : 0.f, bqs = is_q16<In>::value ? qs[b] : 0.f;
+        const float4 f = stage_fix(stage_widen<In>(pre[k], bqo, bqs), bands, b, px, p_end, true);
         float v[4] = {f.x, f.y, f.z, f.w};
         h4 hv, lv;
 #pragma unroll
@@ -496,12 +536,14 @@ __device__ __forceinline__ void moments_mixed_owner(const float* __restrict__ cu
   }
 }
 
-template <int NB>
-__global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_eu(4))) void moments_mixed_kernel(const float* __restrict__ cube,
+template <typename In, int NB>
+__global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_eu(4))) void moments_mixed_kernel(const void* __restrict__ cube,
                                                                         std::size_t pixels, int bands,
                                                                         std::size_t stride,
                                                                         std::size_t px_per_wg,
                                                                         const float* __restrict__ shift,
+                                                                        const float* __restrict__ q_offset,
+                                                                        const float* __restrict__ q_scale,
                                                                         double* __restrict__ partial) {
   constexpr int CH = f16_chunk<NB>();
   constexpr int NP = NB * (NB + 1) / 2;
@@ -512,8 +554,18 @@ __global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_e
   __shared__ _Float16 hi_t[2 * ROWS * PITCH];
   __shared__ _Float16 lo_t[2 * ROWS * PITCH];
   __shared__ float sh[ROWS];
+  __shared__ float qo[is_q16<In>::value ? ROWS : 1];
+  __shared__ float qs[is_q16<In>::value ? ROWS : 1];
 
-  for (int b = threadIdx.x; b < ROWS; b += THREADS) sh[b] = b < bands ? shift[b] : 0.f;
+  for (int b = threadIdx.x; b < ROWS; b += THREADS) {
+    sh[b] = b < bands ? shift[b] : 0.f;
+    if constexpr (is_q16<In>::value) {
+      qo[b] = b < bands ? q_offset[b] : 0.f;
+      qs[b] = b < bands ? q_scale[b] : 0.f;
+    }
+  }
+  if constexpr (!is_q16<In>::value)
+    if (threadIdx.x == 0) qo[0] = qs[0] = 0.f;
   __syncthreads();
 
   // Workgroup j: pixel range (j / (8 * SPLITS)) * 8 + j % 8, pair half (j / 8) % SPLITS.
@@ -529,7 +581,7 @@ __global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_e
 #define HSI_POS(K)                                                                                 \
   case K:                                                                                          \
     if constexpr (K < kWavesPerWg * SPLITS)                                                        \
-      moments_mixed_owner<NB, K>(cube, p_begin, p_end, bands, stride, sh, hi_t, lo_t, out);        \
+      moments_mixed_owner<In, NB, K>(cube, p_begin, p_end, bands, stride, sh, qo, qs, hi_t, lo_t, out); \
     break;
     HSI_POS(0) HSI_POS(1) HSI_POS(2) HSI_POS(3) HSI_POS(4) HSI_POS(5) HSI_POS(6) HSI_POS(7)
     HSI_POS(8) HSI_POS(9) HSI_POS(10) HSI_POS(11) HSI_POS(12) HSI_POS(13) HSI_POS(14) HSI_POS(15)
@@ -621,24 +673,40 @@ __global__ __launch_bounds__(kRedElems * kRedGroups) void moments_reduce_kernel(
   }
 }
 
-// Reference point for the mixed form: the FP64 mean of a sparse sample of the
-// frame (at most 64 runs of 64 contiguous pixels), one block per band. Any
-// point near the mean protects the covariance from cancellation; the exact
-// moments are rebuilt from it afterwards.
-__global__ __launch_bounds__(256) void shift_kernel(const float* __restrict__ cube, std::size_t pixels,
+// Reference point for the mixed form, one block per band: the caller's shift
+// when `given` is non-null, else the FP64 mean of a sparse sample of the
+// frame (at most 64 runs of 64 contiguous pixels). Any point near the mean
+// protects the covariance from cancellation; the exact moments are rebuilt
+// from it afterwards.
+template <typename In>
+__global__ __launch_bounds__(256) void shift_kernel(const void* __restrict__ cube, std::size_t pixels,
                                                     std::size_t stride, std::size_t run_stride,
+                                                    const float* __restrict__ given,
+                                                    const float* __restrict__ q_offset,
+                                                    const float* __restrict__ q_scale,
                                                     float* __restrict__ shift,
                                                     double* __restrict__ shift_d) {
   __shared__ double red[256];
   __shared__ double cnt[256];
   const int b = blockIdx.x;
+  if (given) {
+    if (threadIdx.x == 0) {
+      shift[b] = given[b];
+      shift_d[b] = static_cast<double>(given[b]);
+    }
+    return;
+  }
+  const In* plane = static_cast<const In*>(cube) + static_cast<std::size_t>(b) * stride;
   const std::size_t runs = div_up(pixels, run_stride);
   const std::size_t samples = runs * 64;
   double s = 0.0, c = 0.0;
   for (std::size_t i = threadIdx.x; i < samples; i += 256) {
     const std::size_t p = (i / 64) * run_stride + (i % 64);
     if (p < pixels) {
-      s += cube[static_cast<std::size_t>(b) * stride + p];
+      float v;
+      if constexpr (is_q16<In>::value) v = q16_decode(static_cast<float>(plane[p].code), q_scale[b], q_offset[b]);
+      else v = static_cast<float>(plane[p]);
+      s += v;
       c += 1.0;
     }
   }
@@ -686,10 +754,10 @@ std::size_t mom_wgs(std::size_t pixels, int max_wgs = kMixedMaxWgs) {
   return chunks < static_cast<std::size_t>(max_wgs) ? chunks : static_cast<std::size_t>(max_wgs);
 }
 
-template <int NB>
-hipError_t launch_mixed_nb(const float* cube, std::size_t pixels, int bands, std::size_t stride,
-                           double* scratch, double* count, double* sum, double* sum_outer,
-                           hipStream_t stream) {
+template <typename In, int NB>
+hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t stream) {
+  const std::size_t pixels = m.pixels;
+  const int bands = m.bands;
   const std::size_t chunks = div_up(pixels, kChunk);
   const std::size_t wgs = mom_wgs(pixels, f16_max_wgs<NB>());
   const std::size_t px_per_wg = div_up(chunks, wgs) * kChunk;
@@ -701,28 +769,41 @@ hipError_t launch_mixed_nb(const float* cube, std::size_t pixels, int bands, std
   double* shift_d = scratch + wgs * np * 256;
   float* shift = reinterpret_cast<float*>(shift_d + kMaxBands);
   const std::size_t run_stride = pixels > 64 * 64 ? div_up(pixels, 64) : 64;
-  hipLaunchKernelGGL(shift_kernel, dim3(bands), dim3(256), 0, stream, cube, pixels, stride,
-                     run_stride, shift, shift_d);
+  hipLaunchKernelGGL((shift_kernel<In>), dim3(bands), dim3(256), 0, stream, m.cube, pixels, m.stride, run_stride,
+                     m.shift, m.q_offset, m.q_scale, shift, shift_d);
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // Ranges rounded up to whole groups of 8 so the range/split decode is
   // exact; ranges past the pixel count return at once and write nothing.
   const std::size_t ranges = f16_splits<NB>() > 1 ? round_up(used, kXcds) : used;
-  hipLaunchKernelGGL(moments_mixed_kernel<NB>, dim3(static_cast<unsigned>(ranges * f16_splits<NB>())),
-                     dim3(kWavesPerWg * 64), 0,
-                     stream, cube, pixels, bands, stride, px_per_wg, shift, partial);
+  hipLaunchKernelGGL((moments_mixed_kernel<In, NB>), dim3(static_cast<unsigned>(ranges * f16_splits<NB>())),
+                     dim3(kWavesPerWg * 64), 0, stream, m.cube, pixels, bands, m.stride, px_per_wg, shift,
+                     m.q_offset, m.q_scale, partial);
   if ((err = hipGetLastError()) != hipSuccess) return err;
   ReduceArgs ra{partial, static_cast<int>(used), static_cast<std::size_t>(np) * 256, NB, bands, 0, 1,
-                0, 0, 0.0, count, sum, sum_outer};
+                0, 0, 0.0, m.count, m.sum, m.sum_outer};
   hipLaunchKernelGGL(moments_reduce_kernel, dim3(static_cast<unsigned>(np * 256 / kRedElems)),
                      dim3(kRedElems * kRedGroups), 0, stream, ra);
   if ((err = hipGetLastError()) != hipSuccess) return err;
   const int tri = bands * (bands + 1) / 2;
   hipLaunchKernelGGL(unshift_kernel, dim3(static_cast<unsigned>(div_up(tri, 256))), dim3(256), 0,
-                     stream, bands, shift_d, count, sum, sum_outer);
+                     stream, bands, shift_d, m.count, m.sum, m.sum_outer);
   if ((err = hipGetLastError()) != hipSuccess) return err;
   hipLaunchKernelGGL(unshift_sum_kernel, dim3(static_cast<unsigned>(div_up(bands, 256))), dim3(256),
-                     0, stream, bands, shift_d, count, sum);
+                     0, stream, bands, shift_d, m.count, m.sum);
   return hipGetLastError();
+}
+
+template <typename In>
+hipError_t launch_mixed_typed(const MomentArgs& m, double* scratch, hipStream_t stream) {
+  switch (moment_blocks(m.bands)) {
+#define HSI_NB(N) \
+  case N:         \
+    return launch_mixed_nb<In, N>(m, scratch, stream);
+    HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8) HSI_NB(9)
+    HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14) HSI_NB(15) HSI_NB(16) HSI_NB(17)
+#undef HSI_NB
+  }
+  return hipErrorInvalidValue;
 }
 
 template <int NBM>
@@ -771,19 +852,17 @@ std::size_t moments_scratch_doubles(std::size_t pixels, int bands) {
   return mixed > plain ? mixed : plain;
 }
 
-hipError_t launch_moments(const float* cube, std::size_t pixels, int bands, std::size_t stride,
-                          hsi_precision precision, double* scratch, double* count, double* sum,
-                          double* sum_outer, hipStream_t stream) {
-  if (precision == HSI_PLAIN)
-    return launch_moments_f64(cube, pixels, bands, stride, scratch, count, sum, sum_outer, stream);
-  if (precision != HSI_MIXED) return hipErrorInvalidValue;
-  switch (moment_blocks(bands)) {
-#define HSI_NB(N)                                                                                  \
-  case N:                                                                                          \
-    return launch_mixed_nb<N>(cube, pixels, bands, stride, scratch, count, sum, sum_outer, stream);
-    HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8) HSI_NB(9)
-    HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14) HSI_NB(15) HSI_NB(16) HSI_NB(17)
-#undef HSI_NB
+hipError_t launch_moments(const MomentArgs& m, double* scratch, hipStream_t stream) {
+  if (m.precision == HSI_PLAIN) {
+    if (m.dtype != HSI_F32) return hipErrorInvalidValue;
+    return launch_moments_f64(static_cast<const float*>(m.cube), m.pixels, m.bands, m.stride, scratch, m.count,
+                              m.sum, m.sum_outer, stream);
+  }
+  if (m.precision != HSI_MIXED) return hipErrorInvalidValue;
+  switch (m.dtype) {
+    case HSI_F32: return launch_mixed_typed<float>(m, scratch, stream);
+    case HSI_F16: return launch_mixed_typed<_Float16>(m, scratch, stream);
+    case HSI_Q16: return launch_mixed_typed<q16_t>(m, scratch, stream);
   }
   return hipErrorInvalidValue;
 }

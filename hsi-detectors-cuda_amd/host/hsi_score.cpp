@@ -152,7 +152,8 @@ int main(int argc, char** argv) {
     }
     const hsi_precision prec = get("precision", "plain") == "mixed" ? HSI_MIXED : HSI_PLAIN;
     if (hsi::traits_of(det).needs_statistics) {
-      hsi_ok(ctx, hsi_stats_accumulate(ctx, d_cube, shape.height, shape.width, shape.bands, stride, prec, nullptr),
+      hsi_ok(ctx, hsi_stats_accumulate(ctx, d_cube, HSI_F32, shape.height, shape.width, shape.bands, stride, prec,
+                                       nullptr, nullptr),
              "moments");
       hsi_ok(ctx, hsi_stats_build(ctx, std::stod(get("loading", "1e-6")), nullptr), "statistics");
       hsi_ok(ctx, hsi_stats_check(ctx, nullptr), "statistics");

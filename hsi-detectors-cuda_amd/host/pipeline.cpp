@@ -305,7 +305,7 @@ class HipPipeline final : public Pipeline {
     const hsi_dtype dtype = half_ ? HSI_F16 : HSI_F32;
     if (events) hip_check(hipEventRecord(s.ev[kUploaded], s.stream), "event");
     if (need_stats_) {
-      hsi_check(s.ctx, hsi_stats_accumulate(s.ctx, s.d_cube, H, W, B, stride_, opt_.precision, st), "moments");
+      hsi_check(s.ctx, hsi_stats_accumulate(s.ctx, s.d_cube, HSI_F32, H, W, B, stride_, opt_.precision, nullptr, st), "moments");
       hsi_check(s.ctx, hsi_stats_build(s.ctx, opt_.loading, st), "statistics");
       hsi_check(s.ctx, hsi_stats_status_copy(s.ctx, s.h_counts + nd, st), "statistics status");
     }

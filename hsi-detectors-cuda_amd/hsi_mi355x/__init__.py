@@ -39,7 +39,7 @@ OK, ERR_INVALID_ARG, ERR_CAPACITY, ERR_NOT_POSITIVE_DEFINITE, ERR_DEVICE, ERR_NO
 DETECT_NO_ALLOC_MAX_PIXELS = 1 << 24
 # Statistics status-word bits (HSI_STATUS_*).
 STATUS_COV_NOT_PD, STATUS_CORR_NOT_PD, STATUS_BELOW_TOLERANCE = 1, 2, 4
-F32, F16 = 0, 1
+F32, F16, Q16 = 0, 1, 2
 PLAIN, MIXED = 0, 1
 
 
@@ -158,8 +158,11 @@ def lib():
         "hsi_max_targets": (C.c_int, []),
         "hsi_max_bands": (C.c_int, []),
         "hsi_set_library": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int]),
-        "hsi_stats_accumulate": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t,
-                                           C.c_int, _vp]),
+        "hsi_stats_accumulate": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_size_t, C.c_int, _vp, _vp]),
+        "hsi_encode_q16": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t, _vp, _vp]),
+        "hsi_set_q16_code": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+        "hsi_get_q16_code": (C.c_int, [_vp, C.c_int, _vp, _vp]),
         "hsi_stats_build": (C.c_int, [_vp, C.c_double, _vp]),
         "hsi_stats_check": (C.c_int, [_vp, _vp]),
         "hsi_stats_read": (C.c_int, [_vp] + [_vp] * 8),
@@ -294,6 +297,28 @@ def _stream(stream):
     return C.c_void_p(getattr(stream, "cuda_stream", stream))
 
 
+def _dtype_of(cube):
+    """hsi_dtype of a device tensor: float32 -> F32, float16 -> F16, a 16-bit
+    integer tensor -> Q16 (the codes of hsi_encode_q16)."""
+    d = str(cube.dtype)
+    if d == "torch.float16":
+        return F16
+    if d in ("torch.int16", "torch.uint16"):
+        return Q16
+    return F32
+
+
+def q16_decode(codes, offset, scale):
+    """The device's Q16 decode, fmaf(scale[b], q, offset[b]) rounded once to
+    float32, on the host: codes (bands, n) uint16 -> float32. The product of a
+    float and a 16-bit integer and its sum with a float are exact in float64
+    here, so one rounding to float32 matches the fused multiply-add."""
+    q = np.asarray(codes).view(np.uint16).astype(np.float64)
+    off = np.asarray(offset, np.float32).astype(np.float64)[:, None]
+    sc = np.asarray(scale, np.float32).astype(np.float64)[:, None]
+    return (sc * q + off).astype(np.float32)
+
+
 def to_device_cube(cube, height, width, dtype="f32", device="cuda"):
     """Host BSQ cube (bands, H, W) or (bands, >=H*W) -> device (bands, stride) tensor."""
     torch = _torch()
@@ -352,9 +377,33 @@ class Context:
         self.num_targets, self.bands = t, b
 
     # statistics
-    def stats_accumulate(self, cube, height, width, bands, precision=PLAIN, stream=None):
-        self._check(self._lib.hsi_stats_accumulate(self.handle, _ptr(cube), height, width, bands,
-                                                   cube.stride(0), precision, _stream(stream)))
+    def stats_accumulate(self, cube, height, width, bands, precision=PLAIN, shift=None, stream=None):
+        """Moments of a device cube (float32, float16, or a Q16 code as
+        int16/uint16); `shift` (device float32 [bands]) is the MIXED form's
+        reference, the frame's sampled mean when None."""
+        self._check(self._lib.hsi_stats_accumulate(self.handle, _ptr(cube), _dtype_of(cube), height, width,
+                                                   bands, cube.stride(0), precision, _ptr(shift),
+                                                   _stream(stream)))
+
+    def encode_q16(self, src, height, width, bands, dst, stream=None):
+        """hsi_encode_q16: device float32 (bands, stride) cube -> Q16 codes in
+        `dst` (int16 or uint16 tensor of the same shape); the context keeps
+        the code."""
+        self._check(self._lib.hsi_encode_q16(self.handle, _ptr(src), height, width, bands, src.stride(0),
+                                             _ptr(dst), _stream(stream)))
+
+    def set_q16_code(self, offset, scale):
+        offset = np.ascontiguousarray(offset, np.float32)
+        scale = np.ascontiguousarray(scale, np.float32)
+        self._check(self._lib.hsi_set_q16_code(self.handle, offset.size, offset.ctypes.data,
+                                               scale.ctypes.data))
+
+    def q16_code(self, bands):
+        """(offset, scale) float32 arrays of the context's Q16 code."""
+        off = np.zeros(bands, np.float32)
+        sc = np.zeros(bands, np.float32)
+        self._check(self._lib.hsi_get_q16_code(self.handle, bands, off.ctypes.data, sc.ctypes.data))
+        return off, sc
 
     def stats_build(self, loading, stream=None):
         self._check(self._lib.hsi_stats_build(self.handle, loading, _stream(stream)))
@@ -409,14 +458,14 @@ class Context:
     # scoring
     def score(self, detector, cube, height, width, bands, score, target=None, aux=None,
               precision=PLAIN, stream=None):
-        dtype = F16 if str(cube.dtype) == "torch.float16" else F32
+        dtype = _dtype_of(cube)
         self._check(self._lib.hsi_score(self.handle, int(detector), precision, _ptr(cube), dtype,
                                         height, width, bands, cube.stride(0), _ptr(score),
                                         _ptr(target), _ptr(aux), _stream(stream)))
 
     def score_sam_cem(self, cube, height, width, bands, sam_score=None, sam_target=None,
                       cem_score=None, cem_target=None, stream=None):
-        dtype = F16 if str(cube.dtype) == "torch.float16" else F32
+        dtype = _dtype_of(cube)
         self._check(self._lib.hsi_score_sam_cem(self.handle, _ptr(cube), dtype, height, width,
                                                 bands, cube.stride(0), _ptr(sam_score),
                                                 _ptr(sam_target), _ptr(cem_score),
@@ -462,7 +511,7 @@ class Context:
                                                  C.byref(params), _ptr(loss), _stream(stream)))
 
     def mlp_infer(self, cube, height, width, bands, logit, center=True, stream=None):
-        dtype = F16 if str(cube.dtype) == "torch.float16" else F32
+        dtype = _dtype_of(cube)
         self._check(self._lib.hsi_mlp_infer(self.handle, _ptr(cube), dtype, height, width, bands,
                                             cube.stride(0), int(bool(center)), _ptr(logit),
                                             _stream(stream)))

@@ -55,8 +55,15 @@ typedef enum hsi_detector {
 } hsi_detector;
 
 /* Element type of a device cube. F16 replaces the reference's SamVariant::Half
- * cube (sam_cuda.hpp:31, sam.cu:210-274). */
-typedef enum hsi_dtype { HSI_F32 = 0, HSI_F16 = 1 } hsi_dtype;
+ * cube (sam_cuda.hpp:31, sam.cu:210-274). Q16 is a 16-bit per-band affine
+ * code (no reference counterpart): element q of band b stands for
+ * fmaf(scale[b], q, offset[b]) in FP32, with offset[b] the band's minimum and
+ * scale[b] its range / 65535, so the code spends all 16 bits on the band's
+ * own range where FP16 keeps 11 significant bits of each value. The code's
+ * offsets and scales belong to the context (hsi_encode_q16,
+ * hsi_set_q16_code); Q16 cubes are read by the MIXED statistics and the MIXED
+ * RX/ACE kernels. */
+typedef enum hsi_dtype { HSI_F32 = 0, HSI_F16 = 1, HSI_Q16 = 2 } hsi_dtype;
 
 /* Arithmetic form. PLAIN accumulates in FP64 (the oracle's precision) on an
  * FP32 or FP16 cube. MIXED runs the RX/ACE whitening and the covariance on
@@ -131,15 +138,22 @@ hsi_status hsi_set_library(hsi_ctx* ctx, const float* values, const float* norms
 
 /* Accumulate the scene's raw first and second moments over a device cube
  * (count, sum[B], lower triangle of sum x x^T) in FP64, replacing any
- * previous accumulation. PLAIN forms each product exactly in FP64; MIXED
- * shifts by the frame mean, forms products on FP16 matrix cores from a
- * hi/lo split with FP32 tile accumulators, and reduces tiles in FP64.
- * Replaces launch_reset_moments + launch_accumulate_mean +
- * launch_accumulate_moments (stats_cuda.hpp:141-165, stats.cu:174-214) and
+ * previous accumulation. PLAIN forms each product exactly in FP64 and reads
+ * F32 cubes. MIXED reads F32, F16 or Q16 cubes: it accumulates the moments of
+ * d = x - r, r = d_shift_or_null[B] (device floats, e.g. the previous frame's
+ * mean: the reference's lagged mode, linalg.hpp:78-83) or, when NULL, the
+ * mean of a sparse sample of the frame; products are formed on FP16 matrix
+ * cores from a hi/lo split of d with FP32 tile accumulators, tiles are
+ * reduced in FP64, and the raw moments are rebuilt from the shifted ones in
+ * FP64 as moments_from_shifted does (linalg.cpp:81-103). PLAIN ignores the
+ * shift (its products are exact). Replaces launch_reset_moments +
+ * launch_accumulate_mean + launch_accumulate_moments (stats_cuda.hpp:45-72,
+ * stats.cu:174-214; their `reference` argument is d_shift_or_null) and
  * SpectralMoments::add over a frame (detectors_cpu.cpp:77-93). */
-hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const float* d_cube, int height,
-                                int width, int bands, size_t plane_stride,
-                                hsi_precision precision, hsi_stream stream);
+hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype,
+                                int height, int width, int bands, size_t plane_stride,
+                                hsi_precision precision, const float* d_shift_or_null,
+                                hsi_stream stream);
 
 /* Derive DetectorStatistics on the device in FP64 from the accumulated
  * moments and the installed library: covariance and autocorrelation with
@@ -449,6 +463,25 @@ hsi_status hsi_batch_autotune(hsi_batch* batch, hsi_stream stream, char* report,
 hsi_ctx* hsi_batch_context(hsi_batch* batch, int scene);
 
 /* ---- ingest helpers ---------------------------------------------------------- */
+
+/* Encode an F32 device cube as Q16 on the device: one pass finds each band's
+ * minimum and maximum over the frame's pixels, a second writes
+ * q = round((x - offset[b]) / scale[b]) with offset[b] = minimum and
+ * scale[b] = (maximum - minimum) / 65535 (0 for a constant band, coded 0),
+ * and the context keeps offset/scale as its Q16 code for `bands` bands.
+ * Decoding is within scale[b] / 2 of x (plus FP32 rounding). Elements must
+ * be finite. d_dst has the same plane stride as d_src (uint16 elements);
+ * plane padding is coded 0. Asynchronous. The storage form of the precision
+ * path: no reference counterpart (the reference narrows to FP16,
+ * sam.cu:311-316). */
+hsi_status hsi_encode_q16(hsi_ctx* ctx, const float* d_src, int height, int width,
+                          int bands, size_t plane_stride, uint16_t* d_dst, hsi_stream stream);
+
+/* Synchronous: install a Q16 code (host offset[bands], scale[bands]) on the
+ * context, e.g. the one another context encoded with; and read the current
+ * code back (HSI_ERR_NO_STATISTICS if there is none). */
+hsi_status hsi_set_q16_code(hsi_ctx* ctx, int bands, const float* offset, const float* scale);
+hsi_status hsi_get_q16_code(hsi_ctx* ctx, int bands, float* offset, float* scale);
 
 /* FP32 -> FP16 narrowing of `count` elements (round to nearest even). Both
  * buffers must be 16-byte aligned (INVALID_ARG otherwise); any count.
