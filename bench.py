@@ -527,45 +527,28 @@ def batched_form(H, torch, repeats, seed=20261203 + 4000):
     """64 uneven scenes in one batch: heights U{128..2048}, widths U{128..1024},
     bands from {16, 113, 224}, libraries of U{1..32} targets; 16 of them are
     4x4 snapshot-mosaic frames of 2048x1088 (u16) demosaiced on the device to
-    512x272x16. All four detectors run. Timed: the batch's persistent
-    launches (one per shape class, shapes from the autotuner) against the
-    same scoring issued scene by scene through the single-scene entry points."""
-    rng = np.random.default_rng(seed)
-    f32 = dict(dtype=torch.float32, device="cuda")
-    i32 = dict(dtype=torch.int32, device="cuda")
+    512x272x16 (hsi_mi355x.workloads). All four detectors run. Timed: the
+    batch's persistent launches (one per shape class, shapes from the
+    autotuner) against the same scoring issued scene by scene through the
+    single-scene entry points; and the whole path with statistics (moments
+    per scene, all builds in one launch per build form), eager and replayed
+    from the batch's captured hipGraph, against statistics + scoring scene by
+    scene."""
+    from hsi_mi355x.workloads import make_scene, uneven_scene_specs
     scenes = []
     total_px = 0
     ctx0 = H.Context(0)
     demosaic_us = 0.0
-    for i in range(64):
-        t = int(rng.integers(1, 33))
-        if i % 4 == 3:   # a snapshot-mosaic frame
-            h, w, b = 272, 512, 16
-        else:
-            h, w = int(rng.integers(128, 2049)), int(rng.integers(128, 1025))
-            b = int(rng.choice([16, 113, 224]))
-        cfg = H.synth_config(h, w, b, min(t, 8), seed=seed + i, anomaly_fraction=0.005)
-        lib = H.synth_library(H.synth_config(h, w, b, t, seed=seed + i))
-        stride = H.plane_stride(h, w)
-        if i % 4 == 3:
-            host = H.synth_frame(cfg, 0, threads=16)[:, :h * w].reshape(b, h, w)
-            raw = np.zeros((h * 4, w * 4), np.uint16)
-            for band in range(b):
-                oy, ox = divmod(band, 4)
-                raw[oy::4, ox::4] = np.clip(np.round(host[band] * 65535.0), 0, 65535).astype(np.uint16)
-            raw_d = torch.from_numpy(raw.view(np.int16)).cuda()
-            cube = torch.zeros((b, stride), **f32)
-            demosaic_us += time_on_stream(torch, lambda: H.demosaic(ctx0, raw_d, H.RAW_U16, h * 4, w * 4, 4, b, cube), 3) * 1e3
-            del raw_d
-        else:
-            cube = torch.from_numpy(H.synth_frame(cfg, 0, threads=16)).cuda()
-        p = h * w
-        total_px += p
-        scenes.append(dict(cube=cube, height=h, width=w, library=lib,
-                           sam_score=torch.empty(p, **f32), sam_target=torch.empty(p, **i32),
-                           cem_score=torch.empty(p, **f32), cem_target=torch.empty(p, **i32),
-                           rx=torch.empty(p, **f32), ace_score=torch.empty(p, **f32),
-                           ace_target=torch.empty(p, **i32)))
+    for spec in uneven_scene_specs(seed):
+        sc = make_scene(H, torch, ctx0, spec)
+        if spec["mosaic"]:
+            h, w, b = spec["height"], spec["width"], spec["bands"]
+            raw_d = torch.zeros((h * 4, w * 4), dtype=torch.int16, device="cuda")
+            scratch = torch.zeros_like(sc["cube"])
+            demosaic_us += time_on_stream(torch, lambda: H.demosaic(ctx0, raw_d, H.RAW_U16, h * 4, w * 4, 4, b, scratch), 3) * 1e3
+            del raw_d, scratch
+        total_px += spec["height"] * spec["width"]
+        scenes.append(sc)
     batch = H.Batch(0)
     batch.set_scenes(scenes)
     batch.statistics(LOADING)
@@ -585,6 +568,19 @@ def batched_form(H, torch, repeats, seed=20261203 + 4000):
 
     ms_single = time_on_stream(torch, per_scene, repeats)
     ms_stats = time_on_stream(torch, lambda: batch.statistics(LOADING), repeats)
+
+    def per_scene_path():
+        for c, sc in zip(ctxs, scenes):
+            c.stats_accumulate(sc["cube"], sc["height"], sc["width"], sc["cube"].shape[0])
+            c.stats_build(LOADING)
+        per_scene()
+
+    # the whole path (statistics + scoring): scene by scene through the
+    # single-scene entry points, the batch eagerly, and the batch replayed
+    # from its captured graph
+    ms_path_single = time_on_stream(torch, per_scene_path, repeats)
+    ms_path_eager = time_on_stream(torch, lambda: batch.run(LOADING, use_graph=False), repeats)
+    ms_path_graph = time_on_stream(torch, lambda: batch.run(LOADING, use_graph=True), repeats)
     classes = report.strip().count("\n") + 1
     ctx0.close()
     batch.close()
@@ -599,6 +595,11 @@ def batched_form(H, torch, repeats, seed=20261203 + 4000):
         "batched_mpix_s": round(total_px / (ms_batch / 1e3) / 1e6, 1),
         "speedup_vs_scene_by_scene": round(ms_single / ms_batch, 3),
         "statistics_us": round(ms_stats * 1e3, 1),
+        "path_scene_by_scene_us": round(ms_path_single * 1e3, 1),
+        "path_batched_us": round(ms_path_eager * 1e3, 1),
+        "path_batched_graph_us": round(ms_path_graph * 1e3, 1),
+        "speedup_path_graph_vs_scene_by_scene": round(ms_path_single / ms_path_graph, 3),
+        "path_graph_mpix_s": round(total_px / (ms_path_graph / 1e3) / 1e6, 1),
         "demosaic_us_total": round(demosaic_us, 1),
         "autotuner": report.strip().split("\n"),
     }

@@ -447,12 +447,8 @@ __device__ __forceinline__ void build_path(const BuildArgs& a, int path, Mat<P> 
 }
 
 template <bool IN_LDS>
-__global__ __launch_bounds__(kThreads) void build_kernel(BuildArgs a) {
-  __shared__ double lds[IN_LDS ? 2 * kLdsBlocks * kTileElems : 1];
-  __shared__ double mu[kMaxBands];
-  __shared__ double diag0[kMaxBands];
-  __shared__ int flag;
-  const int path = blockIdx.x;
+__device__ __forceinline__ void build_body(const BuildArgs& a, int path, double* lds, double* mu, double* diag0,
+                                           int* flag) {
   const int nt = (a.bands + kTile - 1) / kTile;
   const std::size_t mat = static_cast<std::size_t>(nt) * (nt + 1) / 2 * kTileElems;
   // work: [path 0: A, X][path 1: A, X][vec 0][vec 1][solved]
@@ -460,15 +456,40 @@ __global__ __launch_bounds__(kThreads) void build_kernel(BuildArgs a) {
   double* solved = a.work + 4 * mat + 2 * static_cast<std::size_t>(kMaxTargets) * kMaxBands;
   lds_double* l_mu = (lds_double*)mu;
   lds_double* l_diag0 = (lds_double*)diag0;
-  lds_int* l_flag = (lds_int*)&flag;
+  lds_int* l_flag = (lds_int*)flag;
   if constexpr (IN_LDS) {
     lds_double* base = (lds_double*)lds;
     build_path(a, path, Mat<lds_double>{base}, Mat<lds_double>{base + kLdsBlocks * kTileElems}, l_mu,
                l_diag0, l_flag, vec, solved);
   } else {
+    (void)lds;
     double* base = a.work + 2 * mat * path;
     build_path(a, path, Mat<double>{base}, Mat<double>{base + mat}, l_mu, l_diag0, l_flag, vec, solved);
   }
+}
+
+// One scene: workgroup 0 takes the covariance path, workgroup 1 the
+// autocorrelation path.
+template <bool IN_LDS>
+__global__ __launch_bounds__(kThreads) void build_kernel(BuildArgs a) {
+  __shared__ double lds[IN_LDS ? 2 * kLdsBlocks * kTileElems : 1];
+  __shared__ double mu[kMaxBands];
+  __shared__ double diag0[kMaxBands];
+  __shared__ int flag;
+  build_body<IN_LDS>(a, blockIdx.x, lds, mu, diag0, &flag);
+}
+
+// Many scenes in one launch: workgroups 2i and 2i + 1 build scene i of the
+// table (its two paths), with the loading given here.
+template <bool IN_LDS>
+__global__ __launch_bounds__(kThreads) void build_batch_kernel(const BuildArgs* __restrict__ table, double loading) {
+  __shared__ double lds[IN_LDS ? 2 * kLdsBlocks * kTileElems : 1];
+  __shared__ double mu[kMaxBands];
+  __shared__ double diag0[kMaxBands];
+  __shared__ int flag;
+  BuildArgs a = table[blockIdx.x / 2];
+  a.loading = loading;
+  build_body<IN_LDS>(a, blockIdx.x % 2, lds, mu, diag0, &flag);
 }
 
 }  // namespace
@@ -477,6 +498,17 @@ std::size_t build_work_doubles(int bands, int num_targets) {
   (void)num_targets;
   const std::size_t nt = (static_cast<std::size_t>(bands) + kTile - 1) / kTile;
   return 4 * nt * (nt + 1) / 2 * kTileElems + 3 * static_cast<std::size_t>(kMaxTargets) * kMaxBands;
+}
+
+bool build_in_lds(int bands) { return bands <= kLdsTiles * kTile; }
+
+hipError_t launch_build_batch(const BuildArgs* d_table, int count, bool in_lds, double loading, hipStream_t stream) {
+  if (count <= 0) return hipSuccess;
+  if (in_lds)
+    hipLaunchKernelGGL(build_batch_kernel<true>, dim3(2 * count), dim3(kThreads), 0, stream, d_table, loading);
+  else
+    hipLaunchKernelGGL(build_batch_kernel<false>, dim3(2 * count), dim3(kThreads), 0, stream, d_table, loading);
+  return hipGetLastError();
 }
 
 hipError_t launch_build(const BuildArgs& a, hipStream_t stream) {

@@ -469,22 +469,14 @@ static hsi_status ensure_mixed_layouts(hsi_ctx* ctx, hipStream_t stream) {
   return HSI_OK;
 }
 
-hsi_status hsi_stats_build(hsi_ctx* ctx, double diagonal_loading, hsi_stream stream) {
-  if (!ctx) return HSI_ERR_INVALID_ARG;
-  if (ctx->moment_bands <= 0) return fail(ctx, HSI_ERR_NO_STATISTICS, "no scene statistics accumulated yet");
-  const int bands = ctx->moment_bands;
-  const int targets = ctx->num_targets;
-  if (targets > 0 && ctx->lib_bands != bands)
-    return fail(ctx, HSI_ERR_INVALID_ARG,
-                "library has " + std::to_string(ctx->lib_bands) + " bands but the statistics cover " +
-                    std::to_string(bands));
-  if (!(diagonal_loading >= 0.0) || !std::isfinite(diagonal_loading))
-    return fail(ctx, HSI_ERR_INVALID_ARG, "diagonal loading must be finite and non-negative");
-  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  BuildArgs a;
+}  // extern "C"
+
+// The build of a context's accumulated moments: checks, then its arguments
+// (device pointers only, so a batch can keep them in a table).
+static void fill_build_args(hsi_ctx* ctx, int bands, double diagonal_loading, BuildArgs* out) {
+  BuildArgs& a = *out;
   a.bands = bands;
-  a.num_targets = targets;
+  a.num_targets = ctx->num_targets;
   a.loading = diagonal_loading;
   a.count = ctx->d_count;
   a.sum = ctx->d_sum;
@@ -497,12 +489,40 @@ hsi_status hsi_stats_build(hsi_ctx* ctx, double diagonal_loading, hsi_stream str
   a.target_norm2 = ctx->d_target_norm2;
   a.cem_weight = ctx->d_cem_weight;
   a.status = ctx->d_status;
-  HSI_HIP(ctx, launch_build(a, s), "build statistics");
+}
+
+static hsi_status build_args_of(hsi_ctx* ctx, double diagonal_loading, BuildArgs* out) {
+  if (ctx->moment_bands <= 0) return fail(ctx, HSI_ERR_NO_STATISTICS, "no scene statistics accumulated yet");
+  const int bands = ctx->moment_bands;
+  if (ctx->num_targets > 0 && ctx->lib_bands != bands)
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "library has " + std::to_string(ctx->lib_bands) + " bands but the statistics cover " +
+                    std::to_string(bands));
+  if (!(diagonal_loading >= 0.0) || !std::isfinite(diagonal_loading))
+    return fail(ctx, HSI_ERR_INVALID_ARG, "diagonal loading must be finite and non-negative");
+  fill_build_args(ctx, bands, diagonal_loading, out);
+  return HSI_OK;
+}
+
+// What a launched build leaves the context with.
+static void mark_built(hsi_ctx* ctx, double diagonal_loading) {
   ctx->layouts_dirty = true;
   ctx->mixed_dirty = true;
-  ctx->stats_bands = bands;
-  ctx->stats_targets = targets;
+  ctx->stats_bands = ctx->moment_bands;
+  ctx->stats_targets = ctx->num_targets;
   ctx->last_loading = diagonal_loading;
+}
+
+extern "C" {
+
+hsi_status hsi_stats_build(hsi_ctx* ctx, double diagonal_loading, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  BuildArgs a;
+  const hsi_status st = build_args_of(ctx, diagonal_loading, &a);
+  if (st != HSI_OK) return st;
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  HSI_HIP(ctx, launch_build(a, static_cast<hipStream_t>(stream)), "build statistics");
+  mark_built(ctx, diagonal_loading);
   return HSI_OK;
 }
 
@@ -1221,6 +1241,19 @@ struct hsi_batch {
   std::vector<SamClass> sam;
   std::vector<RxClass> rx;
   bool tuned = false;
+  // Batched statistics: the scenes that need them and their build arguments
+  // in two device tables, one per build form (matrices in LDS / in global
+  // memory), each built by one launch.
+  std::vector<int> stats_scenes;
+  BuildArgs* d_build[2] = {nullptr, nullptr};
+  int build_count[2] = {0, 0};
+  // Statistics + scoring captured once as a graph (hsi_batch_run), on a
+  // stream of the batch's own, ordered against the caller's by events.
+  hipStream_t graph_stream = nullptr;
+  hipEvent_t graph_in = nullptr, graph_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double graph_loading = -1.0;
 };
 
 namespace {
@@ -1232,7 +1265,18 @@ hsi_status bfail(hsi_batch* b, hsi_status st, const std::string& msg) {
 
 const int kRxChunks[3] = {16, 64, 256};
 
+void batch_release_graph(hsi_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec), b->exec = nullptr;
+  if (b->graph) (void)hipGraphDestroy(b->graph), b->graph = nullptr;
+  b->graph_loading = -1.0;
+}
+
 void batch_release_tables(hsi_batch* b) {
+  batch_release_graph(b);
+  for (auto*& p : b->d_build)
+    if (p) (void)hipFree(p), p = nullptr;
+  b->build_count[0] = b->build_count[1] = 0;
+  b->stats_scenes.clear();
   for (auto& c : b->sam) {
     for (auto*& p : c.d_table)
       if (p) (void)hipFree(p), p = nullptr;
@@ -1331,7 +1375,11 @@ hsi_status hsi_batch_create(int device, hsi_batch** out) {
 void hsi_batch_destroy(hsi_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
+  if (b->graph_stream) (void)hipStreamSynchronize(b->graph_stream);
   batch_release_tables(b);
+  if (b->graph_in) (void)hipEventDestroy(b->graph_in);
+  if (b->graph_out) (void)hipEventDestroy(b->graph_out);
+  if (b->graph_stream) (void)hipStreamDestroy(b->graph_stream);
   for (hsi_ctx* c : b->ctx) hsi_ctx_destroy(c);
   delete b;
 }
@@ -1472,19 +1520,46 @@ hsi_status hsi_batch_set_scenes(hsi_batch* b, const hsi_scene* scenes, int count
     for (int k = 0; k < 3; ++k)
       if (upload_vec(&cls.d_items[k], cls.items[k]) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "upload work items");
   }
+  // Build tables: every scene whose detectors need statistics.
+  std::vector<BuildArgs> forms[2];
+  for (int i = 0; i < count; ++i) {
+    const hsi_scene& sc = scenes[i];
+    if (!sc.d_rx_score && !sc.d_ace_score && !sc.d_cem_score) continue;
+    BuildArgs a;
+    fill_build_args(b->ctx[static_cast<std::size_t>(i)], sc.bands, 0.0, &a);
+    forms[build_in_lds(sc.bands) ? 0 : 1].push_back(a);
+    b->stats_scenes.push_back(i);
+  }
+  for (int k = 0; k < 2; ++k) {
+    b->build_count[k] = static_cast<int>(forms[k].size());
+    if (upload_vec(&b->d_build[k], forms[k]) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "upload build table");
+  }
   return HSI_OK;
 }
 
 hsi_status hsi_batch_statistics(hsi_batch* b, double loading, hsi_stream stream) {
   if (!b) return HSI_ERR_INVALID_ARG;
-  for (std::size_t i = 0; i < b->scenes.size(); ++i) {
-    const hsi_scene& sc = b->scenes[i];
-    if (!sc.d_rx_score && !sc.d_ace_score && !sc.d_cem_score) continue;
-    hsi_ctx* c = b->ctx[i];
-    hsi_status st = hsi_stats_accumulate(c, sc.d_cube, HSI_F32, sc.height, sc.width, sc.bands, sc.plane_stride,
-                                         HSI_PLAIN, nullptr, stream);
-    if (st == HSI_OK) st = hsi_stats_build(c, loading, stream);
-    if (st == HSI_OK) st = ensure_layouts(c, static_cast<hipStream_t>(stream));
+  if (!(loading >= 0.0) || !std::isfinite(loading))
+    return bfail(b, HSI_ERR_INVALID_ARG, "diagonal loading must be finite and non-negative");
+  if (hipSetDevice(b->device) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "select device");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  // Moments scene by scene (each launch spans the GPU), then every build in
+  // one launch per build form - the builds are one or two workgroups each.
+  for (const int i : b->stats_scenes) {
+    const hsi_scene& sc = b->scenes[static_cast<std::size_t>(i)];
+    hsi_ctx* c = b->ctx[static_cast<std::size_t>(i)];
+    const hsi_status st = hsi_stats_accumulate(c, sc.d_cube, HSI_F32, sc.height, sc.width, sc.bands,
+                                               sc.plane_stride, HSI_PLAIN, nullptr, stream);
+    if (st != HSI_OK) return bfail(b, st, "scene " + std::to_string(i) + ": " + c->error);
+  }
+  for (int k = 0; k < 2; ++k) {
+    const hipError_t e = launch_build_batch(b->d_build[k], b->build_count[k], k == 0, loading, s);
+    if (e != hipSuccess) return bfail(b, HSI_ERR_DEVICE, std::string("batched build: ") + hipGetErrorString(e));
+  }
+  for (const int i : b->stats_scenes) {
+    hsi_ctx* c = b->ctx[static_cast<std::size_t>(i)];
+    mark_built(c, loading);
+    const hsi_status st = ensure_layouts(c, s);
     if (st != HSI_OK) return bfail(b, st, "scene " + std::to_string(i) + ": " + c->error);
   }
   return HSI_OK;
@@ -1525,8 +1600,49 @@ hsi_status hsi_batch_score(hsi_batch* b, hsi_stream stream) {
   return HSI_OK;
 }
 
+hsi_status hsi_batch_run(hsi_batch* b, double loading, int use_graph, hsi_stream stream) {
+  if (!b) return HSI_ERR_INVALID_ARG;
+  if (!use_graph) {
+    const hsi_status st = hsi_batch_statistics(b, loading, stream);
+    return st == HSI_OK ? hsi_batch_score(b, stream) : st;
+  }
+  if (hipSetDevice(b->device) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "select device");
+  if (!b->graph_stream) {
+    if (hipStreamCreateWithFlags(&b->graph_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->graph_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->graph_out, hipEventDisableTiming) != hipSuccess)
+      return bfail(b, HSI_ERR_DEVICE, "create the batch's graph stream");
+  }
+  if (!b->exec || b->graph_loading != loading) {
+    batch_release_graph(b);
+    if (hipStreamBeginCapture(b->graph_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return bfail(b, HSI_ERR_DEVICE, "begin graph capture");
+    hsi_status st = hsi_batch_statistics(b, loading, b->graph_stream);
+    if (st == HSI_OK) st = hsi_batch_score(b, b->graph_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(b->graph_stream, &g);
+    if (st != HSI_OK || end != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return st != HSI_OK ? st : bfail(b, HSI_ERR_DEVICE, std::string("end graph capture: ") + hipGetErrorString(end));
+    }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+      batch_release_graph(b);
+      return bfail(b, HSI_ERR_DEVICE, "instantiate the batch graph");
+    }
+    b->graph_loading = loading;
+  }
+  const hipStream_t caller = static_cast<hipStream_t>(stream);
+  if (hipEventRecord(b->graph_in, caller) != hipSuccess || hipStreamWaitEvent(b->graph_stream, b->graph_in, 0) != hipSuccess ||
+      hipGraphLaunch(b->exec, b->graph_stream) != hipSuccess || hipEventRecord(b->graph_out, b->graph_stream) != hipSuccess ||
+      hipStreamWaitEvent(caller, b->graph_out, 0) != hipSuccess)
+    return bfail(b, HSI_ERR_DEVICE, "launch the batch graph");
+  return HSI_OK;
+}
+
 hsi_status hsi_batch_autotune(hsi_batch* b, hsi_stream stream, char* report, size_t report_len) {
   if (!b) return HSI_ERR_INVALID_ARG;
+  batch_release_graph(b);  // launch shapes change
   if (hipSetDevice(b->device) != hipSuccess) return bfail(b, HSI_ERR_DEVICE, "select device");
   const hipStream_t s = static_cast<hipStream_t>(stream);
   hipEvent_t e0, e1;

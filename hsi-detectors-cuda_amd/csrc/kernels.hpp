@@ -76,6 +76,12 @@ struct BuildArgs {
 };
 std::size_t build_work_doubles(int bands, int num_targets);
 hipError_t launch_build(const BuildArgs& args, hipStream_t stream);
+// Whether a build of `bands` keeps its matrices in LDS (one kernel form) or
+// in its global work area (the other); a batched build launches each form
+// once, for all scenes of that form in a device table, with `loading`
+// replacing each entry's.
+bool build_in_lds(int bands);
+hipError_t launch_build_batch(const BuildArgs* d_table, int count, bool in_lds, double loading, hipStream_t stream);
 
 // Derived layouts the scoring kernels read, refreshed after every build or
 // upload: CEM filters as doubles band-major for the SAM/CEM kernel; the

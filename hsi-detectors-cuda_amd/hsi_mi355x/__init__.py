@@ -209,6 +209,7 @@ def lib():
         "hsi_batch_statistics": (C.c_int, [_vp, C.c_double, _vp]),
         "hsi_batch_check": (C.c_int, [_vp, C.POINTER(C.c_int)]),
         "hsi_batch_score": (C.c_int, [_vp, _vp]),
+        "hsi_batch_run": (C.c_int, [_vp, C.c_double, C.c_int, _vp]),
         "hsi_batch_autotune": (C.c_int, [_vp, _vp, C.c_char_p, C.c_size_t]),
         "hsi_batch_context": (_vp, [_vp, C.c_int]),
         "hsi_synth_default": (None, [C.POINTER(SynthConfig)]),
@@ -633,6 +634,11 @@ class Batch:
 
     def score(self, stream=None):
         self._check(self._lib.hsi_batch_score(self.handle, _stream(stream)))
+
+    def run(self, loading=1e-6, use_graph=True, stream=None):
+        """Statistics and scoring of every scene; with use_graph a captured
+        hipGraph replayed per call."""
+        self._check(self._lib.hsi_batch_run(self.handle, loading, int(bool(use_graph)), _stream(stream)))
 
     def context(self, scene):
         """The scene's context (owned by the batch) as a Context."""

@@ -443,7 +443,9 @@ const char* hsi_batch_last_error(const hsi_batch* batch);
  * The cubes and maps named here must stay valid while the batch is used. */
 hsi_status hsi_batch_set_scenes(hsi_batch* batch, const hsi_scene* scenes, int count);
 
-/* Statistics of every scene that needs them (RX, ACE, CEM). Asynchronous. */
+/* Statistics of every scene that needs them (RX, ACE, CEM): moments scene
+ * by scene, then every scene's FP64 build in one launch per build form.
+ * Asynchronous. */
 hsi_status hsi_batch_statistics(hsi_batch* batch, double diagonal_loading, hsi_stream stream);
 
 /* Synchronous: HSI_OK, or the first scene whose factorisation failed. */
@@ -452,6 +454,14 @@ hsi_status hsi_batch_check(hsi_batch* batch, int* failed_scene);
 /* All requested detectors of all scenes: one persistent launch per shape
  * class. Asynchronous. */
 hsi_status hsi_batch_score(hsi_batch* batch, hsi_stream stream);
+
+/* hsi_batch_statistics then hsi_batch_score. With use_graph, the first call
+ * (and the first after hsi_batch_set_scenes, hsi_batch_autotune or a change
+ * of loading) captures both as one hipGraph on a stream of the batch's own,
+ * and every call replays it there, ordered after the caller's earlier work
+ * on `stream` and before its later work (events). The cubes' contents may
+ * change between calls; their addresses may not. Asynchronous. */
+hsi_status hsi_batch_run(hsi_batch* batch, double diagonal_loading, int use_graph, hsi_stream stream);
 
 /* Synchronous: time the candidate launch shapes of every class on the
  * current scenes (statistics must be built) and keep the fastest; a text

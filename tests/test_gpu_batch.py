@@ -117,3 +117,95 @@ def test_batch_equals_single_scene_path(H):
         for k in keys:
             assert np.array_equal(s[k].cpu().numpy().view(np.uint32), first[i][k].view(np.uint32)), (i, k)
     batch.close()
+
+
+KEYS = ("sam_score", "sam_target", "cem_score", "cem_target", "rx", "ace_score", "ace_target")
+
+
+@pytest.fixture(scope="module")
+def uneven_batch(H):
+    """The 64-scene uneven workload (hsi_mi355x.workloads, the bench's seed):
+    the 16 snapshot mosaics demosaiced on the device, statistics and scores
+    from one eager batch run."""
+    import os
+    import torch
+    from hsi_mi355x.workloads import make_scene, uneven_scene_specs
+    ctx0 = H.Context(0)
+    specs = uneven_scene_specs(20261203 + 4000)
+    threads = min(16, os.cpu_count() or 1)
+    scenes = [make_scene(H, torch, ctx0, sp, threads=threads) for sp in specs]
+    print(f"64 scenes, {sum(s['height'] * s['width'] for s in scenes) / 1e6:.1f} Mpix", flush=True)
+    batch = H.Batch(0)
+    batch.set_scenes(scenes)
+    batch.run(1e-6, use_graph=False)
+    batch.check()
+    torch.cuda.synchronize()
+    eager = [{k: s[k].cpu().numpy().copy() for k in KEYS} for s in scenes]
+    yield dict(batch=batch, scenes=scenes, specs=specs, eager=eager, ctx0=ctx0)
+    batch.close()
+    ctx0.close()
+
+
+def test_uneven_batch_against_the_oracle_on_stripes(H, uneven_batch, oracle_lib):
+    """Every scene's SAM, CEM, RX and ACE on its first 64 rows against the
+    oracle, which scores the same device cube (demosaiced ones included) with
+    the batch's statistics: angles within 1e-6 rad, scores within 1e-5
+    relative (floor 1), targets equal."""
+    import os
+    from gpu_util import ANGLE_TOL, REL, rel_err
+    threads = min(16, os.cpu_count() or 1)
+    batch = uneven_batch["batch"]
+    for i, (sc, out) in enumerate(zip(uneven_batch["scenes"], uneven_batch["eager"])):
+        h, w, b = sc["height"], sc["width"], sc["cube"].shape[0]
+        rows = min(64, h)
+        p = rows * w
+        cube = np.ascontiguousarray(sc["cube"][:, :p].cpu().numpy()).reshape(b, rows, w)
+        lib = sc["library"]
+        t = lib.shape[0]
+        dev = batch.context(i).stats_read(b, t)
+        st = oracle_lib.Statistics(b, t)
+        st.mean[:] = dev["mean"]
+        st.whitener[:] = dev["whitener"]
+        st.target_white[:] = dev["target_white"][:t]
+        st.target_norm2[:] = dev["target_norm2"][:t]
+        st.cem_weight[:] = dev["cem_weight"][:t]
+        st.status = oracle_lib.STATS_OK
+        ws, wt = oracle_lib.sam_best(cube, lib, threads=threads)
+        assert np.abs(out["sam_score"][:p].astype(np.float64) - ws).max() <= ANGLE_TOL, i
+        assert np.array_equal(out["sam_target"][:p], wt), i
+        cs, ct = oracle_lib.cem_best(cube, st, threads=threads)
+        assert rel_err(out["cem_score"][:p], cs, 1.0).max() <= REL, i
+        assert (out["cem_target"][:p] == ct).mean() >= 0.9999, i
+        acs, act, arx = oracle_lib.ace_best(cube, st, threads=threads)
+        assert rel_err(out["rx"][:p], arx, 1.0).max() <= REL, i
+        assert rel_err(out["ace_score"][:p], acs, 1.0).max() <= REL, i
+        assert (out["ace_target"][:p] == act).mean() >= 0.9999, i
+    print("64 scenes: stripes match the oracle")
+
+
+def test_uneven_batch_equals_single_scene_path_at_full_size(H, uneven_batch):
+    import torch
+    for i, (sc, out) in enumerate(zip(uneven_batch["scenes"], uneven_batch["eager"])):
+        want = single(H, torch, sc)
+        for k in KEYS:
+            assert np.array_equal(out[k].view(np.uint32), want[k].view(np.uint32)), (i, k)
+
+
+def test_uneven_batch_graph_replays_equal_the_eager_run(H, uneven_batch):
+    """The captured statistics + scoring graph, replayed three times with the
+    program's own eager work between replays, reproduces the eager run bit
+    for bit each time."""
+    import torch
+    batch, scenes, eager = uneven_batch["batch"], uneven_batch["scenes"], uneven_batch["eager"]
+    for replay in range(3):
+        for s in scenes:
+            for k in KEYS:
+                s[k].fill_(-3)
+        junk = torch.randn(1 << 20, device="cuda").sum()  # unrelated eager work
+        batch.run(1e-6, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        for i, s in enumerate(scenes):
+            for k in KEYS:
+                assert np.array_equal(s[k].cpu().numpy().view(np.uint32), eager[i][k].view(np.uint32)), (replay, i, k)
+    batch.check()
