@@ -459,9 +459,13 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
     judged on `eval_frames` held-out frames: recall of the plain detection
     lists (RX at a chi-squared false-alarm rate of 1e-6, ACE at 0.5, NMS
     radius 2) and the false-skip count. Times are HIP-event averages on one
-    held-out frame with statistics built: plain = full-frame ACE with the RX
-    side output; learned = inference + compaction at the cut + exact
-    rescoring of the candidates."""
+    held-out frame: plain path = FP64 statistics + build + full-frame ACE with
+    the RX side output; learned path (recall judged on it) = one pass that
+    accumulates the mixed-form statistics and screens every pixel, the
+    build, and rescoring of the candidate rows in the precision form (FP16
+    hi/lo matrix cores). Beside it: the same with separate statistics and
+    screening passes (learned_score = screen + rescoring with statistics
+    built), and with the rows rescored by the plain kernel."""
     from hsi_mi355x.prescreen import Prescreen
     h = w = size
     b, t = 113, 8
@@ -471,7 +475,7 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
     ctx = H.Context(0)
     ctx.set_library(lib)
     ctx.reserve(h * w, b)
-    ps = Prescreen(ctx, b, seed=7)
+    ps = Prescreen(ctx, b, seed=7, precision=H.MIXED)
     t0 = time.perf_counter()
     for f in range(train_frames):
         cube = torch.from_numpy(H.synth_frame(cfg, f, threads=16)).cuda()
@@ -485,13 +489,20 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
     cube = None
     for f in range(eval_frames):
         cube = torch.from_numpy(H.synth_frame(cfg, 1000 + f, threads=16)).cuda()
-        evals.append(ps.evaluate(cube, h, w))
+        evals.append(ps.evaluate(cube, h, w, fused=True))
         if eval_frames > 3 and (f + 1) % 10 == 0:
             log(f"  learned: {f + 1} held-out frames, min recall so far "
                 f"rx {min(e['rx']['recall'] for e in evals):.6f} ace {min(e['ace']['recall'] for e in evals):.6f}")
     # timing on the last held-out frame (statistics are built for it)
+    def plain_rescoring():
+        ps.precision = H.PLAIN
+        ps.screen(cube, h, w)
+        ps.precision = H.MIXED
+
     ms = {"plain_score": time_on_stream(torch, lambda: ps.plain(cube, h, w), repeats),
+          "learned_path_fused": time_on_stream(torch, lambda: ps.screen_fused(cube, h, w), repeats),
           "learned_score": time_on_stream(torch, lambda: ps.screen(cube, h, w), repeats),
+          "learned_score_plain_rescoring": time_on_stream(torch, lambda: plain_rescoring(), repeats),
           "learned_score_two_pass": time_on_stream(torch, lambda: ps.screen_two_pass(cube, h, w), repeats),
           "stats_plain": time_on_stream(torch, lambda: (ctx.stats_accumulate(cube, h, w, b), ctx.stats_build(LOADING)), repeats),
           "stats_mixed": time_on_stream(torch, lambda: (ctx.stats_accumulate(cube, h, w, b, precision=H.MIXED), ctx.stats_build(LOADING)), repeats)}
@@ -515,13 +526,19 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
         "pooled_recall": {d: round(1 - sum(e[d]["false_skips"] for e in evals) /
                                    max(1, sum(e[d]["plain"] for e in evals)), 6) for d in ("rx", "ace")},
         "false_skips": {d: int(sum(e[d]["false_skips"] for e in evals)) for d in ("rx", "ace")},
+        "learned_path": "statistics (mixed form) and screen in one pass over the cube "
+                        "(hsi_stats_accumulate_screen, network on pixel - sampled mean), build, "
+                        "precision-form rescoring of the candidate rows",
+        "score_error_vs_plain": {"rx_max_rel": max(e.get("rx_max_rel_err", 0.0) for e in evals),
+                                 "ace_max_abs": max(e.get("ace_max_abs_err", 0.0) for e in evals)},
         "plain_detections": {d: int(sum(e[d]["plain"] for e in evals)) for d in ("rx", "ace")},
         "candidate_fraction": round(float(np.mean([e["candidate_fraction"] for e in evals])), 5),
         "us": {k: round(v * 1e3, 1) for k, v in ms.items()},
         "score_mpix_s": {"plain": round(pix / (ms["plain_score"] / 1e3) / 1e6, 1),
                          "learned": round(pix / (ms["learned_score"] / 1e3) / 1e6, 1)},
         "speedup_score": round(ms["plain_score"] / ms["learned_score"], 3),
-        "speedup_path_with_statistics": round((ms["stats_plain"] + ms["plain_score"]) /
+        "speedup_path_with_statistics": round((ms["stats_plain"] + ms["plain_score"]) / ms["learned_path_fused"], 3),
+        "speedup_path_separate_passes": round((ms["stats_plain"] + ms["plain_score"]) /
                                               (ms["stats_mixed"] + ms["learned_score"]), 3),
         "infer_hbm_gb_s": round(b * 4 * pix / (ms["infer"] / 1e3) / 1e9, 1),
         "sam_cem_2048": {k: {"us": round(ms[k] * 1e3, 1),

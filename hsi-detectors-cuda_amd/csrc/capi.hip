@@ -63,6 +63,8 @@ struct hsi_ctx {
   void* d_mlp_a1 = nullptr;
   void* d_mlp_a2 = nullptr;
   float* d_mlp_tail = nullptr;
+  void* d_mlp_a1h = nullptr;   // FP16 copies for the fused statistics + screening pass
+  void* d_mlp_a2h = nullptr;
   float* d_mlp_act = nullptr;
   std::size_t mlp_act_bytes = 0;
   // Evaluation scratch.
@@ -177,6 +179,7 @@ unsigned* no_alloc_scratch(int device) {
   return g_no_alloc_scratch[device];
 }
 
+hsi_status mlp_ready(hsi_ctx* ctx, int bands);  // a network is installed for `bands`
 }  // namespace
 
 extern "C" {
@@ -226,6 +229,7 @@ void hsi_ctx_destroy(hsi_ctx* ctx) {
                   ctx->d_ace_hl, ctx->d_col_scale, ctx->d_row_inv, ctx->d_target_inv,
                   ctx->d_mom_scratch, ctx->d_det_scratch, ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v,
                   ctx->d_mlp_grad, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, ctx->d_mlp_act,
+                  ctx->d_mlp_a1h, ctx->d_mlp_a2h,
                   ctx->d_positions, ctx->d_screen_rec, ctx->d_screen_counts,
                   ctx->d_eval, ctx->d_q16_offset, ctx->d_q16_scale, ctx->d_q16_extrema};
   for (void* p : ptrs)
@@ -384,6 +388,69 @@ hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtyp
   m.sum = ctx->d_sum;
   m.sum_outer = ctx->d_sum_outer;
   HSI_HIP(ctx, launch_moments(m, ctx->d_mom_scratch, static_cast<hipStream_t>(stream)), "accumulate moments");
+  ctx->moment_bands = bands;
+  return HSI_OK;
+}
+
+hsi_status hsi_stats_accumulate_screen(hsi_ctx* ctx, const float* d_cube, int height, int width, int bands,
+                                       size_t plane_stride, const float* d_shift_or_null, float cut,
+                                       hsi_detection* d_candidates, uint32_t* d_count, int capacity,
+                                       float* d_spectra, size_t spectra_stride, hsi_stream stream) {
+  if (!ctx) return HSI_ERR_INVALID_ARG;
+  hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, HSI_F32);
+  if (st != HSI_OK) return st;
+  if (bands > kFusedScreenMaxBands)
+    return fail(ctx, HSI_ERR_CAPACITY, "statistics with screening support at most " +
+                                           std::to_string(kFusedScreenMaxBands) +
+                                           " bands (use hsi_stats_accumulate and hsi_mlp_screen)");
+  if ((st = mlp_ready(ctx, bands)) != HSI_OK) return st;
+  if (!d_candidates || !d_count || capacity < 0)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "screening needs a candidate list, its count and a capacity >= 0");
+  const std::size_t pb = static_cast<std::size_t>((bands + 31) / 32 * 32);
+  if (d_spectra && spectra_stride < pb)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "candidate spectra need a pitch >= 32 * ceil(bands / 32)");
+  if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (ctx->mlp_dirty) {
+    HSI_HIP(ctx, launch_mlp_layouts(ctx->d_mlp_w, bands, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail,
+                                    ctx->d_mlp_a1h, ctx->d_mlp_a2h, s),
+            "pre-screen layouts");
+    ctx->mlp_dirty = false;
+  }
+  const std::size_t pixels = static_cast<std::size_t>(height) * width;
+  HSI_HIP(ctx, grow(reinterpret_cast<void**>(&ctx->d_mom_scratch), &ctx->mom_scratch_bytes,
+                    moments_scratch_doubles(pixels, bands) * sizeof(double)),
+          "allocate moment scratch");
+  MomentArgs m;
+  m.cube = d_cube;
+  m.dtype = HSI_F32;
+  m.pixels = pixels;
+  m.bands = bands;
+  m.stride = plane_stride;
+  m.precision = HSI_MIXED;
+  m.shift = d_shift_or_null;
+  m.count = ctx->d_count;
+  m.sum = ctx->d_sum;
+  m.sum_outer = ctx->d_sum_outer;
+  FusedScreenArgs fs;
+  fs.cut = cut;
+  fs.width = width;
+  fs.w1h = ctx->d_mlp_a1h;
+  fs.w2h = ctx->d_mlp_a2h;
+  fs.tail = ctx->d_mlp_tail;
+  fs.cand = d_candidates;
+  fs.count = d_count;
+  fs.cap = static_cast<unsigned>(capacity);
+  fs.spectra = d_spectra;
+  fs.spectra_stride = spectra_stride;
+  const std::size_t cap = static_cast<std::size_t>(capacity);
+  HSI_HIP(ctx, grow(&ctx->d_screen_rec, &ctx->screen_rec_bytes, (cap ? cap : 1) * sizeof(hsi_detection)),
+          "screening scratch");
+  HSI_HIP(ctx, grow(&ctx->d_screen_counts, &ctx->screen_counts_bytes, 2 * kScreenMaxWaves * sizeof(unsigned)),
+          "screening scratch");
+  fs.scratch_rec = static_cast<hsi_detection*>(ctx->d_screen_rec);
+  fs.counts = static_cast<unsigned*>(ctx->d_screen_counts);
+  HSI_HIP(ctx, launch_moments_screen(m, fs, ctx->d_mom_scratch, s), "accumulate moments with screening");
   ctx->moment_bands = bands;
   return HSI_OK;
 }
@@ -898,6 +965,8 @@ hsi_status mlp_alloc(hsi_ctx* ctx) {
   HSI_HIP(ctx, hipMalloc(&ctx->d_mlp_a1, mlp_a1_halves(kMaxBands) * 2 + 256), "allocate pre-screen layouts");
   HSI_HIP(ctx, hipMalloc(&ctx->d_mlp_a2, mlp_a2_halves() * 2 + 256), "allocate pre-screen layouts");
   HSI_HIP(ctx, alloc(&ctx->d_mlp_tail, mlp_tail_floats()), "allocate pre-screen layouts");
+  HSI_HIP(ctx, hipMalloc(&ctx->d_mlp_a1h, mlp_a1_halves(kMaxBands) * 2 + 256), "allocate pre-screen layouts");
+  HSI_HIP(ctx, hipMalloc(&ctx->d_mlp_a2h, mlp_a2_halves() * 2 + 256), "allocate pre-screen layouts");
   return HSI_OK;
 }
 
@@ -1035,7 +1104,8 @@ hsi_status hsi_mlp_infer(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtype, int 
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   if (ctx->mlp_dirty) {
-    HSI_HIP(ctx, launch_mlp_layouts(ctx->d_mlp_w, bands, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, s),
+    HSI_HIP(ctx, launch_mlp_layouts(ctx->d_mlp_w, bands, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail,
+                                      ctx->d_mlp_a1h, ctx->d_mlp_a2h, s),
             "pre-screen layouts");
     ctx->mlp_dirty = false;
   }
@@ -1076,13 +1146,24 @@ hsi_status hsi_gather_pixels(hsi_ctx* ctx, const float* d_cube, int height, int 
 }  // extern "C"
 
 namespace {
-hsi_status score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_cube, int height, int width,
-                            int bands, size_t plane_stride, const hsi_detection* d_candidates,
-                            const uint32_t* d_count, int capacity, const float* d_spectra, size_t spectra_stride,
-                            float* d_score, int32_t* d_target, float* d_aux, hsi_stream stream) {
+hsi_status score_candidates(hsi_ctx* ctx, hsi_detector detector, hsi_precision precision, const float* d_cube,
+                            int height, int width, int bands, size_t plane_stride,
+                            const hsi_detection* d_candidates, const uint32_t* d_count, int capacity,
+                            const float* d_spectra, size_t spectra_stride, float* d_score, int32_t* d_target,
+                            float* d_aux, hsi_stream stream) {
   if (!ctx) return HSI_ERR_INVALID_ARG;
   if (detector != HSI_DET_RX && detector != HSI_DET_ACE)
     return fail(ctx, HSI_ERR_INVALID_ARG, "candidate rescoring applies to RX and ACE");
+  if (precision != HSI_PLAIN && precision != HSI_MIXED)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "unknown precision");
+  if (precision == HSI_MIXED && !d_spectra)
+    return fail(ctx, HSI_ERR_INVALID_ARG, "the mixed form rescores candidate spectra rows only");
+  if (precision == HSI_MIXED && (spectra_stride % 4 != 0 || reinterpret_cast<std::uintptr_t>(d_spectra) % 16 != 0))
+    return fail(ctx, HSI_ERR_INVALID_ARG,
+                "mixed-form candidate spectra need 16-byte aligned rows (pitch a multiple of 4)");
+  if (precision == HSI_MIXED && capacity > 0 &&
+      static_cast<std::size_t>(capacity) * spectra_stride * sizeof(float) >= (static_cast<std::size_t>(1) << 31))
+    return fail(ctx, HSI_ERR_CAPACITY, "mixed-form candidate spectra must span less than 2 GiB");
   hsi_status st = check_cube(ctx, d_cube, height, width, bands, plane_stride, HSI_F32);
   if (st != HSI_OK) return st;
   if (!d_candidates || !d_count || capacity < 0 || !d_score)
@@ -1095,7 +1176,7 @@ hsi_status score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_
     return fail(ctx, HSI_ERR_NO_LIBRARY, "ACE needs a target library");
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  if ((st = ensure_layouts(ctx, s)) != HSI_OK) return st;
+  if ((st = precision == HSI_PLAIN ? ensure_layouts(ctx, s) : ensure_mixed_layouts(ctx, s)) != HSI_OK) return st;
   const std::size_t pixels = static_cast<std::size_t>(height) * width;
   HSI_HIP(ctx, launch_fill_scores(d_score, detector == HSI_DET_ACE ? d_target : nullptr,
                                   detector == HSI_DET_ACE ? d_aux : nullptr, pixels, s),
@@ -1112,6 +1193,11 @@ hsi_status score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_
   a.white_blocks = ctx->d_white_blocks;
   a.ace_u = ctx->d_ace_u;
   a.target_norm2 = ctx->d_target_norm2;
+  a.white_hl = ctx->d_white_hl;
+  a.ace_hl = ctx->d_ace_hl;
+  a.col_scale = ctx->d_col_scale;
+  a.row_inv = ctx->d_row_inv;
+  a.target_inv = ctx->d_target_inv;
   if (detector == HSI_DET_RX) {
     a.rx = d_score;
   } else {
@@ -1119,9 +1205,12 @@ hsi_status score_candidates(hsi_ctx* ctx, hsi_detector detector, const float* d_
     a.ace_score = d_score;
     a.ace_target = d_target;
   }
-  HSI_HIP(ctx, launch_rxace_candidates(a, d_candidates, d_count, static_cast<unsigned>(capacity), width,
-                                       d_spectra, spectra_stride, s),
-          "rescore candidates");
+  const unsigned cap = static_cast<unsigned>(capacity);
+  const hipError_t err =
+      precision == HSI_PLAIN
+          ? launch_rxace_candidates(a, d_candidates, d_count, cap, width, d_spectra, spectra_stride, s)
+          : launch_rxace_mixed_candidates(a, d_candidates, d_count, cap, width, d_spectra, spectra_stride, s);
+  HSI_HIP(ctx, err, "rescore candidates");
   return HSI_OK;
 }
 }  // namespace
@@ -1132,20 +1221,21 @@ hsi_status hsi_score_candidates(hsi_ctx* ctx, hsi_detector detector, const float
                                 int bands, size_t plane_stride, const hsi_detection* d_candidates,
                                 const uint32_t* d_count, int capacity, float* d_score, int32_t* d_target,
                                 float* d_aux, hsi_stream stream) {
-  return score_candidates(ctx, detector, d_cube, height, width, bands, plane_stride, d_candidates, d_count,
-                          capacity, nullptr, 0, d_score, d_target, d_aux, stream);
+  return score_candidates(ctx, detector, HSI_PLAIN, d_cube, height, width, bands, plane_stride, d_candidates,
+                          d_count, capacity, nullptr, 0, d_score, d_target, d_aux, stream);
 }
 
-hsi_status hsi_score_candidate_spectra(hsi_ctx* ctx, hsi_detector detector, const float* d_cube, int height,
-                                       int width, int bands, size_t plane_stride,
-                                       const hsi_detection* d_candidates, const uint32_t* d_count, int capacity,
-                                       const float* d_spectra, size_t spectra_stride, float* d_score,
-                                       int32_t* d_target, float* d_aux, hsi_stream stream) {
+hsi_status hsi_score_candidate_spectra(hsi_ctx* ctx, hsi_detector detector, hsi_precision precision,
+                                       const float* d_cube, int height, int width, int bands,
+                                       size_t plane_stride, const hsi_detection* d_candidates,
+                                       const uint32_t* d_count, int capacity, const float* d_spectra,
+                                       size_t spectra_stride, float* d_score, int32_t* d_target, float* d_aux,
+                                       hsi_stream stream) {
   if (!d_spectra || spectra_stride < static_cast<size_t>(bands < 0 ? 0 : bands))
     return ctx ? fail(ctx, HSI_ERR_INVALID_ARG, "candidate spectra need a buffer with pitch >= bands")
                : HSI_ERR_INVALID_ARG;
-  return score_candidates(ctx, detector, d_cube, height, width, bands, plane_stride, d_candidates, d_count,
-                          capacity, d_spectra, spectra_stride, d_score, d_target, d_aux, stream);
+  return score_candidates(ctx, detector, precision, d_cube, height, width, bands, plane_stride, d_candidates,
+                          d_count, capacity, d_spectra, spectra_stride, d_score, d_target, d_aux, stream);
 }
 
 hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int width, int bands,
@@ -1170,7 +1260,8 @@ hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int wid
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   if (ctx->mlp_dirty) {
-    HSI_HIP(ctx, launch_mlp_layouts(ctx->d_mlp_w, bands, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail, s),
+    HSI_HIP(ctx, launch_mlp_layouts(ctx->d_mlp_w, bands, ctx->d_mlp_a1, ctx->d_mlp_a2, ctx->d_mlp_tail,
+                                      ctx->d_mlp_a1h, ctx->d_mlp_a2h, s),
             "pre-screen layouts");
     ctx->mlp_dirty = false;
   }

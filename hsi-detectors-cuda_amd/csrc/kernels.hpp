@@ -58,6 +58,35 @@ struct MomentArgs {
 };
 hipError_t launch_moments(const MomentArgs& args, double* scratch, hipStream_t stream);
 
+// The learned pre-screen fused into the mixed-form moments (F32 cube, bands
+// <= kFusedScreenMaxBands): from the same read of each pixel, the network's
+// logit of (pixel - shift) - the FP16 hi part the moments stage - and, at
+// logit >= cut, a record {x, y, row, logit} with the pixel's spectrum
+// (shift + hi + lo in FP32) in spectra row `row` (nullable; pitch >= 32 *
+// ceil(bands / 32), elements from `bands` to that boundary written 0). Each
+// wave of the pass owns cap / waves rows; the records are compacted into
+// cand[0 .. *count) afterwards (the rows stay where the pass wrote them), and
+// a wave that found more than its share raises *count past `cap`.
+// counts / scratch_rec: 2 * kScreenMaxWaves counters and `cap` records of
+// scratch, as hsi_mlp_screen uses.
+constexpr int kFusedScreenMaxBands = 127;
+struct FusedScreenArgs {
+  float cut = 0.f;
+  int width = 0;
+  const void* w1h = nullptr;      // FP16 layer-1 fragments (launch_mlp_layouts a1h)
+  const void* w2h = nullptr;      // FP16 layer-2 fragments (a2h)
+  const float* tail = nullptr;    // b1[64] | b2[32] | w3[32] | b3
+  hsi_detection* cand = nullptr;
+  unsigned* count = nullptr;
+  unsigned cap = 0;
+  float* spectra = nullptr;
+  std::size_t spectra_stride = 0;
+  unsigned* counts = nullptr;
+  hsi_detection* scratch_rec = nullptr;
+};
+hipError_t launch_moments_screen(const MomentArgs& args, const FusedScreenArgs& fs, double* scratch,
+                                 hipStream_t stream);
+
 struct BuildArgs {
   int bands = 0;
   int num_targets = 0;
@@ -217,6 +246,12 @@ hipError_t launch_sam_cem_batch(int lib_stride, int ppt, const SamSceneDev* scen
 hipError_t launch_rxace_batch(int nb, int tt, const RxSceneDev* scenes, const RxWorkItem* work, int num_items,
                               int workgroups, hipStream_t stream);
 hipError_t launch_rxace_mixed(const RxAceArgs& args, hipStream_t stream);
+// The precision form over candidate spectra rows, with launch_rxace_candidates'
+// list semantics (FP32 rows; spectra_stride >= bands; the rows' bytes below
+// 2 GiB); an overflowed list scores the whole FP32 cube.
+hipError_t launch_rxace_mixed_candidates(const RxAceArgs& args, const hsi_detection* cand, const unsigned* count,
+                                         unsigned capacity, int width, const float* spectra,
+                                         std::size_t spectra_stride, hipStream_t stream);
 
 // ---- learned pre-screen (mlp.hip) --------------------------------------------
 struct MlpInferArgs {
@@ -250,12 +285,20 @@ struct MlpInferArgs {
   unsigned* scratch_counts = nullptr;
 };
 constexpr int kScreenMaxWaves = 4096;
+// Per-wave candidate regions -> one list: counts[0 .. waves) of regions of rc
+// records (rec_in) are scanned into prefix[] and copied to rec_out[0 ..
+// total); *total is raised past cap when a region overflowed.
+hipError_t launch_screen_compact(const unsigned* counts, int waves, unsigned rc, unsigned cap, unsigned* prefix,
+                                 unsigned* total, const hsi_detection* rec_in, hsi_detection* rec_out,
+                                 hipStream_t stream);
 std::size_t mlp_param_count(int bands);
 std::size_t mlp_a1_halves(int bands);
 std::size_t mlp_a2_halves();
 std::size_t mlp_tail_floats();
 std::size_t mlp_act_floats(int n);
-hipError_t launch_mlp_layouts(const float* w, int bands, void* a1, void* a2, float* tail, hipStream_t stream);
+// a1h / a2h (nullable): FP16 copies of the a1 / a2 fragments (same halves).
+hipError_t launch_mlp_layouts(const float* w, int bands, void* a1, void* a2, float* tail, void* a1h, void* a2h,
+                              hipStream_t stream);
 hipError_t launch_mlp_infer(const MlpInferArgs& args, hipStream_t stream);
 hipError_t launch_mlp_gradients(const float* x, std::size_t x_stride, int n, int bands, const float* labels,
                                 float pos_weight, const float* w, float* act, float* grad, float* loss,

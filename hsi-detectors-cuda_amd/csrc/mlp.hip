@@ -58,20 +58,27 @@ __host__ __device__ inline int l2_k(int kb, int g, int i) {
 // a1: [J][I][lane] bf8 = W1[16I + lane%16][32J + 8(lane/16) + i]
 // a2: [kb][I2][lane] bf8 = W2[16 I2 + lane%16][l2_k(kb, lane/16, i)]
 // tail: b1[64] | b2[32] | w3[32] | b3
+// a1h, a2h (nullable): the same fragments in FP16, for the screen fused into
+// the statistics pass (its inputs are the FP16 staging tile).
 __global__ void mlp_layouts_kernel(const float* __restrict__ w, int bands, __bf16* __restrict__ a1,
-                                   __bf16* __restrict__ a2, float* __restrict__ tail) {
+                                   __bf16* __restrict__ a2, float* __restrict__ tail,
+                                   _Float16* __restrict__ a1h, _Float16* __restrict__ a2h) {
   const Off o = offsets(bands);
   const int nkb = div_up(bands, 32);
   const int n1 = nkb * 4 * 64 * 8;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n1; e += gridDim.x * blockDim.x) {
     const int i = e % 8, lane = (e / 8) % 64, I = (e / 512) % 4, J = e / 2048;
     const int row = 16 * I + (lane & 15), col = 32 * J + 8 * (lane >> 4) + i;
-    a1[e] = static_cast<__bf16>(col < bands ? w[o.w1 + static_cast<std::size_t>(row) * bands + col] : 0.f);
+    const float v = col < bands ? w[o.w1 + static_cast<std::size_t>(row) * bands + col] : 0.f;
+    a1[e] = static_cast<__bf16>(v);
+    if (a1h) a1h[e] = static_cast<_Float16>(v);
   }
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 2 * 2 * 64 * 8; e += gridDim.x * blockDim.x) {
     const int i = e % 8, lane = (e / 8) % 64, I2 = (e / 512) % 2, kb = e / 1024;
     const int row = 16 * I2 + (lane & 15);
-    a2[e] = static_cast<__bf16>(w[o.w2 + static_cast<std::size_t>(row) * kH1 + l2_k(kb, lane >> 4, i)]);
+    const float v = w[o.w2 + static_cast<std::size_t>(row) * kH1 + l2_k(kb, lane >> 4, i)];
+    a2[e] = static_cast<__bf16>(v);
+    if (a2h) a2h[e] = static_cast<_Float16>(v);
   }
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < kH1 + 2 * kH2 + 1; e += gridDim.x * blockDim.x) {
     float v;
@@ -452,13 +459,8 @@ hipError_t launch_infer_typed(const MlpInferArgs& a, hipStream_t stream) {
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    unsigned* prefix = a.scratch_counts + kScreenMaxWaves;
-    hipLaunchKernelGGL(screen_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, a.scratch_counts, waves, rc,
-                       a.cand_cap, prefix, a.cand_count);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(screen_compact_kernel, dim3(static_cast<unsigned>(div_up(waves, 4))), dim3(256), 0, stream,
-                       a.scratch_counts, prefix, waves, rc, a.cand_cap, a.scratch_rec, a.cand);
-    return hipGetLastError();
+    return launch_screen_compact(a.scratch_counts, waves, rc, a.cand_cap, a.scratch_counts + kScreenMaxWaves,
+                                 a.cand_count, a.scratch_rec, a.cand, stream);
   }
 }
 
@@ -603,9 +605,24 @@ std::size_t mlp_a2_halves() { return 2 * 2 * 64 * 8; }
 std::size_t mlp_tail_floats() { return kH1 + 2 * kH2 + 1; }
 std::size_t mlp_act_floats(int n) { return static_cast<std::size_t>(kActRows) * n; }
 
-hipError_t launch_mlp_layouts(const float* w, int bands, void* a1, void* a2, float* tail, hipStream_t stream) {
+hipError_t launch_screen_compact(const unsigned* counts, int waves, unsigned rc, unsigned cap, unsigned* prefix,
+                                 unsigned* total, const hsi_detection* rec_in, hsi_detection* rec_out,
+                                 hipStream_t stream) {
+  if (waves <= 0 || waves > kScreenMaxWaves) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(screen_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, counts, waves, rc, cap, prefix,
+                     total);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(screen_compact_kernel, dim3(static_cast<unsigned>(div_up(waves, 4))), dim3(256), 0, stream,
+                     counts, prefix, waves, rc, cap, rec_in, rec_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_layouts(const float* w, int bands, void* a1, void* a2, float* tail, void* a1h, void* a2h,
+                              hipStream_t stream) {
   hipLaunchKernelGGL(mlp_layouts_kernel, dim3(16), dim3(256), 0, stream, w, bands,
-                     static_cast<__bf16*>(a1), static_cast<__bf16*>(a2), tail);
+                     static_cast<__bf16*>(a1), static_cast<__bf16*>(a2), tail, static_cast<_Float16*>(a1h),
+                     static_cast<_Float16*>(a2h));
   return hipGetLastError();
 }
 

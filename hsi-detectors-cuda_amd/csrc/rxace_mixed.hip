@@ -139,13 +139,27 @@ __device__ __forceinline__ void mixed_group_mfma(const h8* __restrict__ w_lds, i
         z[t][I0 + k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[k], yh[t], z[t][I0 + k], 0, 0, 0);
 }
 
+// Candidate form: the elements scored are the first *count records of a
+// candidate list, each read from the spectra row its record names (target
+// field; bands contiguous) and written at its pixel y * width + x. A list
+// longer than its capacity overflowed: the whole frame is scored from the
+// cube instead (decided here, from the count, with no host round trip).
+struct MixedGather {
+  const hsi_detection* cand = nullptr;
+  const unsigned* count = nullptr;
+  unsigned cap = 0;
+  int width = 0;
+  const float* spectra = nullptr;
+  std::size_t spectra_stride = 0;   // floats per row
+};
+
 // A wave scores TP tiles of 16 pixels per step. TP = 1: lane (g, n) holds
 // pixel n of the tile. TP = 2: a step covers 32 consecutive pixels and lane
 // (g, n) loads pixels 2n and 2n + 1 with one load per band - tile 0 holds
 // the even pixels, tile 1 the odd ones - so each whitener fragment read
 // from LDS serves 32 pixels and each load moves twice the bytes.
 // In: float, _Float16 or q16_t (decoded per band before the centring).
-template <typename In, int NB, int TT, int S>
+template <typename In, int NB, int TT, int S, bool GATHER>
 __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
     int num_targets, const float* __restrict__ mean, const _Float16* __restrict__ white_hl,
@@ -153,8 +167,9 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
     const float* __restrict__ row_inv, const float* __restrict__ target_inv,
     const float* __restrict__ target_norm2, const float* __restrict__ q_offset,
     const float* __restrict__ q_scale, float* __restrict__ rx_out,
-    float* __restrict__ ace_score, int* __restrict__ ace_target) {
+    float* __restrict__ ace_score, int* __restrict__ ace_target, const MixedGather mg) {
   constexpr int TP = mixed_tp(S);
+  static_assert(!GATHER || (TP == 1 && sizeof(In) == 4), "the candidate form reads FP32 rows one tile a step");
   constexpr int WAVES = mixed_waves(NB, S);
   constexpr int G = mixed_group(NB, S);
   constexpr int NKB = (NB + 1) / 2;              // 32-band K blocks
@@ -213,7 +228,16 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
   const int g = lane >> 4;
   const int n = lane & 15;
   constexpr int SPAN = 16 * TP;  // pixels per wave step
-  const std::size_t steps = div_up(pixels, SPAN);
+  std::size_t items = pixels;    // elements scored
+  bool rows = false;             // candidate form: elements are spectra rows
+  if constexpr (GATHER) {
+    const unsigned c = *mg.count;
+    if (c <= mg.cap) {
+      items = c;
+      rows = true;
+    }
+  }
+  const std::size_t steps = div_up(items, SPAN);
   // The wave index through readfirstlane, so the step and every address
   // derived from it stay in scalar registers.
   const unsigned wave_in_wg = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)));
@@ -234,7 +258,51 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
     lane_off[i] = static_cast<std::uint32_t>(
         (static_cast<std::size_t>(8 * g + i) * stride + (TP == 1 ? n : 2 * n)) * sizeof(In));
   typedef typename RawT<In, TP>::type Raw;
-  auto load_kblock = [&](std::size_t t, int J, Raw (&xr)[8]) {
+  // Candidate form: element t * 16 + n's spectra row (byte offset), pixel
+  // and whether it is scored (a record outside the frame, or naming a row
+  // outside [0, cap), is not).
+  struct Elem {
+    std::uint32_t off;
+    std::size_t pix;
+    bool ok;
+  };
+  const std::size_t frame_rows = GATHER ? pixels / static_cast<std::size_t>(mg.width > 0 ? mg.width : 1) : 0;
+  auto elem_of = [&](std::size_t t) -> Elem {
+    Elem e{0u, t * 16 + n, false};
+    if (!rows) {
+      e.ok = e.pix < items;
+      return e;
+    }
+    const std::size_t i = t * 16 + n;
+    if (i < items) {
+      const hsi_detection d = mg.cand[i];
+      if (d.x >= 0 && d.x < mg.width && d.y >= 0 && static_cast<std::size_t>(d.y) < frame_rows && d.target >= 0 &&
+          static_cast<unsigned>(d.target) < mg.cap) {
+        e.ok = true;
+        e.pix = static_cast<std::size_t>(d.y) * static_cast<std::size_t>(mg.width) + static_cast<std::size_t>(d.x);
+        e.off = static_cast<std::uint32_t>(static_cast<std::size_t>(d.target) * mg.spectra_stride * sizeof(float));
+      }
+    }
+    return e;
+  };
+  const std::uint32_t spectra_bytes =
+      GATHER ? static_cast<std::uint32_t>(static_cast<std::size_t>(mg.cap) * mg.spectra_stride * sizeof(float)) : 0u;
+  auto load_kblock = [&](std::size_t t, int J, Raw (&xr)[8], std::uint32_t row_off) {
+    if constexpr (GATHER) {
+      if (rows) {
+        // Eight contiguous bands of the element's row, two 16-byte loads;
+        // past the buffer they read 0.
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float*>(mg.spectra), 0, static_cast<int>(spectra_bytes), 0x00020000);
+        const std::uint32_t o = row_off + static_cast<std::uint32_t>((32 * J + 8 * g) * sizeof(float));
+        const float4 a0 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
+        const float4 a1 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 16, 0, 0));
+        xr[0] = a0.x; xr[1] = a0.y; xr[2] = a0.z; xr[3] = a0.w;
+        xr[4] = a1.x; xr[5] = a1.y; xr[6] = a1.z; xr[7] = a1.w;
+        return;
+      }
+    }
+    (void)row_off;
     const std::size_t base = (t * SPAN + static_cast<std::size_t>(32 * J) * stride) * sizeof(In);
     const bool inside = base < cube_bytes;  // else an empty view: every load reads 0
     const std::uint32_t rest = inside ? static_cast<std::uint32_t>(cube_bytes - base) : 0u;
@@ -254,7 +322,12 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const int b = 32 * J + 8 * g + i;
-        v[i] = __builtin_fmaf(ya_lds[b], raw_unpack<In, TP>(raw[i], t), yc_lds[b]);
+        float x = raw_unpack<In, TP>(raw[i], t);
+        // A row's bands past the last may hold anything (the next row, or
+        // padding): drop them before the zero column scale meets them.
+        if constexpr (GATHER)
+          if (J == NKB - 1 && b >= bands) x = 0.f;
+        v[i] = __builtin_fmaf(ya_lds[b], x, yc_lds[b]);
       }
       split8(v, yh[t], yl[t]);
     }
@@ -268,9 +341,12 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
   // Small whiteners at one tile per step leave registers for a whole tile in
   // flight (PD = NKB K blocks); otherwise the loads stream one K block ahead.
   Raw cur[PD][8], nxt[PD][8];
-  if (wave_id < steps)
+  Elem ecur{0u, 0, false}, enxt{0u, 0, false};
+  if (wave_id < steps) {
+    if constexpr (GATHER) ecur = elem_of(wave_id);
 #pragma unroll
-    for (int J = 0; J < PD; ++J) load_kblock(wave_id, J, cur[J]);
+    for (int J = 0; J < PD; ++J) load_kblock(wave_id, J, cur[J], ecur.off);
+  }
 
   for (std::size_t step = wave_id; step < steps; step += waves) {
     f4 z[TP][NB];
@@ -280,9 +356,11 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
       for (int I = 0; I < NB; ++I) z[t][I] = f4{0.f, 0.f, 0.f, 0.f};
 
     if constexpr (PD == NKB) {
-      if (step + waves < steps)
+      if (step + waves < steps) {
+        if constexpr (GATHER) enxt = elem_of(step + waves);
 #pragma unroll
-        for (int J = 0; J < NKB; ++J) load_kblock(step + waves, J, nxt[J]);
+        for (int J = 0; J < NKB; ++J) load_kblock(step + waves, J, nxt[J], enxt.off);
+      }
 #pragma unroll
       for (int J = 0; J < NKB; ++J) {
         kblock(J, cur[J], z);
@@ -295,8 +373,12 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
     } else {
 #pragma unroll
       for (int J = 0; J < NKB; ++J) {
-        if (J + 1 < NKB) load_kblock(step, J + 1, nxt[0]);
-        else if (step + waves < steps) load_kblock(step + waves, 0, nxt[0]);
+        if (J + 1 < NKB) {
+          load_kblock(step, J + 1, nxt[0], ecur.off);
+        } else if (step + waves < steps) {
+          if constexpr (GATHER) enxt = elem_of(step + waves);
+          load_kblock(step + waves, 0, nxt[0], enxt.off);
+        }
         kblock(J, cur[0], z);
 #pragma unroll
         for (int i = 0; i < 8; ++i) cur[0][i] = nxt[0][i];
@@ -384,7 +466,17 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
         best[t] = best[t] < 0.0 ? 0.0 : (1.0 < best[t] ? 1.0 : best[t]);
       }
     }
-    if (g == 0) {
+    if constexpr (GATHER) {
+      if (g == 0 && ecur.ok) {
+        const std::size_t p = ecur.pix;
+        if constexpr (TT > 0) {
+          if (ace_score) ace_score[p] = static_cast<float>(best[0]);
+          if (ace_target) ace_target[p] = best_t[0];
+        }
+        if (rx_out) rx_out[p] = static_cast<float>(zz[0]);
+      }
+      ecur = enxt;
+    } else if (g == 0) {
       const std::size_t p0 = step * SPAN + (TP == 1 ? n : 2 * n);
 #pragma unroll
       for (int t = 0; t < TP; ++t) {
@@ -414,19 +506,22 @@ inline int mixed_shape(int nb, std::size_t elem_bytes, bool ace) {
   return (nb < 10 && elem_bytes == 4 && ace) ? 1 : 2;
 }
 
-template <typename In, int NB, int TT, int S>
-hipError_t launch_mixed_tp(const RxAceArgs& a, hipStream_t stream) {
+template <typename In, int NB, int TT, int S, bool GATHER = false>
+hipError_t launch_mixed_tp(const RxAceArgs& a, hipStream_t stream, const MixedGather& mg = MixedGather{}) {
   if (!fits_buffer(a.bands, a.stride, sizeof(In))) return hipErrorInvalidValue;
   constexpr int WAVES = mixed_waves(NB, S);
-  const std::size_t steps = div_up(a.pixels, 16 * mixed_tp(S));
+  // A candidate list's length is known only on the device: size the grid for
+  // its capacity (the kernel walks its elements, or the frame, grid-stride).
+  const std::size_t steps = div_up(GATHER ? static_cast<std::size_t>(mg.cap) : a.pixels, 16 * mixed_tp(S));
   std::size_t wgs = div_up(steps, WAVES);
   const std::size_t cap = NB > 10 ? 256 : 1024;  // one resident workgroup per CU above 10 blocks
   if (wgs > cap) wgs = cap;
-  hipLaunchKernelGGL((rxace_mixed_kernel<In, NB, TT, S>), dim3(static_cast<unsigned>(wgs)),
+  if (wgs == 0) wgs = 1;
+  hipLaunchKernelGGL((rxace_mixed_kernel<In, NB, TT, S, GATHER>), dim3(static_cast<unsigned>(wgs)),
                      dim3(WAVES * 64), 0, stream, static_cast<const In*>(a.cube), a.pixels,
                      a.bands, a.stride, a.num_targets, a.mean, a.white_hl, a.ace_hl, a.col_scale,
                      a.row_inv, a.target_inv, a.target_norm2, a.q_offset, a.q_scale, a.rx, a.ace_score,
-                     a.ace_target);
+                     a.ace_target, mg);
   return hipGetLastError();
 }
 
@@ -443,6 +538,14 @@ hipError_t launch_mixed_nb(const RxAceArgs& a, hipStream_t stream) {
   return launch_mixed_t<In, NB, 2>(a, stream);
 }
 
+template <int NB>
+hipError_t launch_mixed_gather_nb(const RxAceArgs& a, const MixedGather& mg, hipStream_t stream) {
+  if (a.num_targets == 0 || (!a.ace_score && !a.ace_target))
+    return launch_mixed_tp<float, NB, 0, 1, true>(a, stream, mg);
+  if (a.num_targets <= 16) return launch_mixed_tp<float, NB, 1, 1, true>(a, stream, mg);
+  return launch_mixed_tp<float, NB, 2, 1, true>(a, stream, mg);
+}
+
 template <typename In>
 hipError_t launch_mixed_typed(const RxAceArgs& a, hipStream_t stream) {
   switch (rx_blocks(a.bands)) {
@@ -455,7 +558,10 @@ hipError_t launch_mixed_typed(const RxAceArgs& a, hipStream_t stream) {
 }
 
 // ---------------------------------------------------------------------------
-// Layouts. One workgroup: power-of-two scales, then the split blocks.
+// Layouts: the power-of-two scales in one workgroup (coalesced passes over
+// the whitener, maxima reduced in registers, LDS and lane shuffles, no
+// atomics), then the split blocks by a grid of workgroups, one element per
+// thread.
 
 __device__ __forceinline__ float pow2_for(float m) {
   // 2^(4 - e) where m = f * 2^e, f in [0.5, 1): scales m into [8, 16).
@@ -465,51 +571,81 @@ __device__ __forceinline__ float pow2_for(float m) {
   return ldexpf(1.f, 4 - e);
 }
 
-__global__ __launch_bounds__(1024) void mixed_layouts_kernel(MixedLayoutArgs a) {
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+constexpr int kScaleThreads = 1024;
+
+// Column scales s_b = 1 / pow2_for(max_i |W[i][b]|), row scales r_i =
+// pow2_for(max_b |W[i][b]| / s_b), target scales c_t = pow2_for(max_i
+// |U[t][i]| / r_i); written as col_scale, row_inv = 1 / r, target_inv = 1 / c.
+// Maxima of non-negative floats are exact in any order, so the result does
+// not depend on the reduction's shape.
+__global__ __launch_bounds__(kScaleThreads) void mixed_scales_kernel(MixedLayoutArgs a) {
   const int n = a.bands;
-  const int nb = rx_blocks(n);
   const int T = a.num_targets;
   const int tid = threadIdx.x;
-  // Maxima of |x| kept as the bit patterns of non-negative floats, whose
-  // unsigned order is the float order, so LDS atomicMax finds them in any
-  // order and the result is deterministic.
-  __shared__ unsigned cmax[kMaxBands], rmax[kMaxBands], tmax[kMaxTargets];
-  __shared__ float cs[kMaxBands];   // column scale s_b (y' = y * s_b)
-  __shared__ float rs[kMaxBands];   // row scale r_i
-  __shared__ float ts[kMaxTargets]; // target scale c_t
-  for (int b = tid; b < kMaxBands; b += blockDim.x) cmax[b] = rmax[b] = 0u;
-  for (int t = tid; t < kMaxTargets; t += blockDim.x) tmax[t] = 0u;
-  __syncthreads();
-  for (int e = tid; e < n * n; e += blockDim.x) {
-    const int i = e / n, b = e % n;
-    if (b <= i) atomicMax(&cmax[b], __float_as_uint(fabsf(a.whitener[e])));
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  constexpr int kWaves = kScaleThreads / 64;
+  __shared__ float part[kScaleThreads];
+  __shared__ float cs[kMaxBands];
+  __shared__ float rs[kMaxBands];
+  // Columns: thread t takes column t % cols of rows t / cols, + phases, ...
+  const int cols = n <= 128 ? 128 : 256;
+  const int phases = kScaleThreads / cols;
+  {
+    const int b = tid % cols;
+    float m = 0.f;
+    if (b < n)
+      for (int i = tid / cols; i < n; i += phases)
+        if (b <= i) m = fmaxf(m, fabsf(a.whitener[static_cast<std::size_t>(i) * n + b]));
+    part[tid] = m;
   }
   __syncthreads();
-  // W' = W / s with s = 1 / pow2_for(column max): columns land in [8, 16).
-  for (int b = tid; b < kMaxBands; b += blockDim.x) cs[b] = b < n ? 1.f / pow2_for(__uint_as_float(cmax[b])) : 0.f;
-  __syncthreads();
-  for (int e = tid; e < n * n; e += blockDim.x) {
-    const int i = e / n, b = e % n;
-    if (b <= i) atomicMax(&rmax[i], __float_as_uint(fabsf(a.whitener[e] / cs[b])));
+  for (int b = tid; b < kMaxBands; b += kScaleThreads) {
+    float m = 0.f;
+    if (b < n)
+      for (int ph = 0; ph < phases; ++ph) m = fmaxf(m, part[ph * cols + b]);
+    cs[b] = b < n ? 1.f / pow2_for(m) : 0.f;
   }
   __syncthreads();
-  for (int i = tid; i < kMaxBands; i += blockDim.x) rs[i] = pow2_for(__uint_as_float(rmax[i]));
-  __syncthreads();
-  for (int e = tid; e < T * n; e += blockDim.x) {
-    const int t = e / n, i = e % n;
-    atomicMax(&tmax[t], __float_as_uint(fabsf(a.target_white[e] / rs[i])));
+  // Rows: a wave per row, lanes along it.
+  for (int i = wave; i < kMaxBands; i += kWaves) {
+    float m = 0.f;
+    if (i < n)
+      for (int b = lane; b <= i; b += 64) m = fmaxf(m, fabsf(a.whitener[static_cast<std::size_t>(i) * n + b] / cs[b]));
+    m = wave_max(m);
+    if (lane == 0) rs[i] = pow2_for(m);
   }
   __syncthreads();
-  for (int t = tid; t < kMaxTargets; t += blockDim.x) ts[t] = pow2_for(__uint_as_float(tmax[t]));
-  __syncthreads();
-  for (int b = tid; b < kMaxBands; b += blockDim.x) {
+  for (int b = tid; b < kMaxBands; b += kScaleThreads) {
     a.col_scale[b] = cs[b];
     a.row_inv[b] = 1.f / rs[b];
   }
-  for (int t = tid; t < kMaxTargets; t += blockDim.x) a.target_inv[t] = 1.f / ts[t];
-  // Whitener blocks: [block][hi|lo][lane][8], value W''[16I + l%16][32J + 8(l/16) + i].
-  const int nblk = block_prefix(nb);
-  for (int e = tid; e < nblk * 512; e += blockDim.x) {
+  for (int t = wave; t < kMaxTargets; t += kWaves) {
+    float m = 0.f;
+    if (t < T)
+      for (int i = lane; i < n; i += 64) m = fmaxf(m, fabsf(a.target_white[static_cast<std::size_t>(t) * n + i] / rs[i]));
+    m = wave_max(m);
+    if (lane == 0) a.target_inv[t] = 1.f / pow2_for(m);
+  }
+}
+
+// Whitener blocks: [block][hi|lo][lane][8], value W''[16I + l%16][32J + 8(l/16) + i]
+// with W'' = W / s (columns) * r (rows). Target blocks: [tt][P][hi|lo][lane][8],
+// value c_t * U[t][band] / r_band with band = 32P + 4(l/16) + i for i < 4 and
+// 32P + 16 + 4(l/16) + (i-4) above. One element (its hi and lo halves) per
+// thread over both tables.
+__global__ __launch_bounds__(256) void mixed_blocks_kernel(MixedLayoutArgs a, int nwhite, int ntarget) {
+  const int n = a.bands;
+  const int T = a.num_targets;
+  const int e0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e0 < nwhite) {
+    const int e = e0;
     const int blk = e / 512;
     const int lane = (e % 512) / 8;
     const int i = e % 8;
@@ -519,30 +655,28 @@ __global__ __launch_bounds__(1024) void mixed_layouts_kernel(MixedLayoutArgs a) 
     const int row = 16 * I + (lane & 15);
     const int col = 32 * J + 8 * (lane >> 4) + i;
     float v = 0.f;
-    if (row < n && col <= row) v = a.whitener[static_cast<std::size_t>(row) * n + col] / cs[col] * rs[row];
+    if (row < n && col <= row)
+      v = a.whitener[static_cast<std::size_t>(row) * n + col] / a.col_scale[col] / a.row_inv[row];
     const _Float16 h = static_cast<_Float16>(v);
     a.white_hl[blk * 1024 + lane * 8 + i] = h;
     a.white_hl[blk * 1024 + 512 + lane * 8 + i] = static_cast<_Float16>(v - static_cast<float>(h));
-  }
-  // Target blocks: [tt][P][hi|lo][lane][8], value c_t * U[t][band] / r_band with
-  // band = 32P + 4(l/16) + i for i < 4 and 32P + 16 + 4(l/16) + (i-4) above.
-  if (T > 0 && a.ace_hl) {
+  } else if (e0 < nwhite + ntarget) {
+    const int e = e0 - nwhite;
+    const int nb = rx_blocks(n);
     const int npair = (nb + 1) / 2;
-    const int tiles = T > 16 ? 2 : 1;
-    for (int e = tid; e < tiles * npair * 512; e += blockDim.x) {
-      const int tt = e / (npair * 512);
-      const int P = (e / 512) % npair;
-      const int lane = (e % 512) / 8;
-      const int i = e % 8;
-      const int t = 16 * tt + (lane & 15);
-      const int band = 32 * P + (i < 4 ? 4 * (lane >> 4) + i : 16 + 4 * (lane >> 4) + (i - 4));
-      float v = 0.f;
-      if (t < T && band < n) v = a.target_white[static_cast<std::size_t>(t) * n + band] / rs[band] * ts[t];
-      const _Float16 h = static_cast<_Float16>(v);
-      const int base = (tt * npair + P) * 1024;
-      a.ace_hl[base + lane * 8 + i] = h;
-      a.ace_hl[base + 512 + lane * 8 + i] = static_cast<_Float16>(v - static_cast<float>(h));
-    }
+    const int tt = e / (npair * 512);
+    const int P = (e / 512) % npair;
+    const int lane = (e % 512) / 8;
+    const int i = e % 8;
+    const int t = 16 * tt + (lane & 15);
+    const int band = 32 * P + (i < 4 ? 4 * (lane >> 4) + i : 16 + 4 * (lane >> 4) + (i - 4));
+    float v = 0.f;
+    if (t < T && band < n)
+      v = a.target_white[static_cast<std::size_t>(t) * n + band] * a.row_inv[band] / a.target_inv[t];
+    const _Float16 h = static_cast<_Float16>(v);
+    const int base = (tt * npair + P) * 1024;
+    a.ace_hl[base + lane * 8 + i] = h;
+    a.ace_hl[base + 512 + lane * 8 + i] = static_cast<_Float16>(v - static_cast<float>(h));
   }
 }
 
@@ -561,8 +695,39 @@ hipError_t launch_rxace_mixed(const RxAceArgs& a, hipStream_t stream) {
   return launch_mixed_typed<float>(a, stream);
 }
 
+hipError_t launch_rxace_mixed_candidates(const RxAceArgs& a, const hsi_detection* cand, const unsigned* count,
+                                         unsigned capacity, int width, const float* spectra,
+                                         std::size_t spectra_stride, hipStream_t stream) {
+  if (a.dtype != HSI_F32 || !cand || !count || !spectra || width <= 0) return hipErrorInvalidValue;
+  if (spectra_stride < static_cast<std::size_t>(a.bands) ||
+      static_cast<std::size_t>(capacity) * spectra_stride * sizeof(float) >= (static_cast<std::size_t>(1) << 31))
+    return hipErrorInvalidValue;
+  MixedGather mg;
+  mg.cand = cand;
+  mg.count = count;
+  mg.cap = capacity;
+  mg.width = width;
+  mg.spectra = spectra;
+  mg.spectra_stride = spectra_stride;
+  switch (rx_blocks(a.bands)) {
+#define HSI_NB(N) case N: return launch_mixed_gather_nb<N>(a, mg, stream);
+    HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8)
+    HSI_NB(9) HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14)
+#undef HSI_NB
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_mixed_layouts(const MixedLayoutArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(mixed_layouts_kernel, dim3(1), dim3(1024), 0, stream, a);
+  if (a.bands < 1 || a.bands > kMaxRxBands) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mixed_scales_kernel, dim3(1), dim3(kScaleThreads), 0, stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int nb = rx_blocks(a.bands);
+  const int nwhite = block_prefix(nb) * 512;
+  const int ntarget = (a.num_targets > 0 && a.ace_hl) ? (a.num_targets > 16 ? 2 : 1) * ((nb + 1) / 2) * 512 : 0;
+  hipLaunchKernelGGL(mixed_blocks_kernel, dim3(static_cast<unsigned>(div_up(nwhite + ntarget, 256))), dim3(256), 0,
+                     stream, a, nwhite, ntarget);
   return hipGetLastError();
 }
 

@@ -25,6 +25,8 @@
 //         hierarchy the reference designed for its device accumulator
 //         (linalg.hpp:53-90, stats_cuda.hpp:121-135). The raw moments are then
 //         rebuilt in FP64 as moments_from_shifted does (linalg.cpp:81-103).
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -418,12 +420,148 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const flo
 // pitch of CH + 8 halves (16 lanes reading 16 bytes each at rows m = 0..15
 // land on 16 disjoint bank quads). Each owner position is its own code path,
 // holding only its run's accumulators and operand blocks.
-template <typename In, int NB, int POS>
+// The screen fused into the statistics pass: LDS copies of the network and
+// the outputs. Wave w screens pixels 16w .. 16w + 15 of each 128-pixel chunk.
+// Candidates go to per-wave regions (no atomics): wave v of the grid owns
+// rows [v * rc, v * rc + rc) of the record scratch and of the spectra, keeps
+// its count in a register and leaves it in counts[v]; a scan and a
+// compaction pass make the record list contiguous afterwards.
+struct FusedScreenDev {
+  float cut;
+  int nkb;                  // 32-band K blocks of layer 1: ceil(bands / 32)
+  int width;
+  unsigned rc;              // rows per wave
+  unsigned* counts;         // [waves]
+  hsi_detection* rec;       // region layout
+  float* spectra;
+  std::size_t spectra_stride;
+};
+struct ScreenLds {
+  const h8* w1;             // [J][I][lane]
+  const h8* w2;             // [kb][I2][lane]
+  const float* tail;        // b1[64] | b2[32] | w3[32] | b3
+  FusedScreenDev fs;
+};
+constexpr int kFusedChunk = 128;
+
+// One chunk's screening for wave w (hi, lo: the chunk's FP16 parts of
+// x - shift, [row][pixel] with pitch PITCH; rows past ROWS read 0). Layer 1
+// is v_mfma_f32_16x16x32_f16 with the FP16 weights as A and the hi parts as
+// B (lane (g, n): bands 32J + 8g .. +7 of pixel 16w + n); layers 2 and 3 as
+// in the standalone inference kernel (layer 1's accumulators feed layer 2
+// directly, its K order permuted to match). A candidate's spectrum is
+// rebuilt from the staged parts as shift + (hi + lo) in FP32 - within
+// 2^-21 |x - shift| + 2^-23 |x| + 2^-24 of the pixel (the subtraction, lo's
+// rounding, absolute where lo is subnormal, and the add) - so nothing waits
+// on memory inside the chunk loop (the stores are not waited for).
+// The logit of pixel 16 * wave + (lane % 16) of the chunk, in every lane
+// group. Branch-free (K blocks past ceil(bands / 32) have zero weights in
+// LDS), so the compiler can interleave it with the chunk's moment MFMAs,
+// which read the same tile.
+template <int NB, int PITCH, int ROWS>
+__device__ __forceinline__ float screen_logit(const _Float16* tile, const ScreenLds& sl, int wave) {
+  constexpr int NKBM = (NB + 1) / 2;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int n = lane & 15;
+  const int col = 16 * wave + n;
+  f4 z1[4];
+#pragma unroll
+  for (int I = 0; I < 4; ++I) z1[I] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int J = 0; J < NKBM; ++J) {
+    h8 x;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int row = 32 * J + 8 * g + i;
+      if constexpr (32 * NKBM > ROWS) x[i] = row < ROWS ? tile[row * PITCH + col] : static_cast<_Float16>(0.f);
+      else x[i] = tile[row * PITCH + col];
+    }
+#pragma unroll
+    for (int I = 0; I < 4; ++I)
+      z1[I] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl.w1[(J * 4 + I) * 64 + lane], x, z1[I], 0, 0, 0);
+  }
+  f4 z2[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    h8 hb;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int I = 2 * kb + (i < 4 ? 0 : 1);
+      const int r = i & 3;
+      const float h = z1[I][r] + sl.tail[16 * I + 4 * g + r];
+      hb[i] = static_cast<_Float16>(h > 0.f ? h : 0.f);
+    }
+#pragma unroll
+    for (int I2 = 0; I2 < 2; ++I2)
+      z2[I2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl.w2[(kb * 2 + I2) * 64 + lane], hb, z2[I2], 0, 0, 0);
+  }
+  float out = 0.f;
+#pragma unroll
+  for (int I2 = 0; I2 < 2; ++I2)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = 16 * I2 + 4 * g + r;
+      const float h = z2[I2][r] + sl.tail[64 + k];
+      out += sl.tail[96 + k] * (h > 0.f ? h : 0.f);
+    }
+  out += __shfl_xor(out, 16, 64);
+  out += __shfl_xor(out, 32, 64);
+  return out + sl.tail[128];
+}
+
+// Candidates of the chunk from this wave's logits: a record and a spectra
+// row each, in the wave's region.
+template <int PITCH>
+__device__ __forceinline__ void screen_emit(float logit, const _Float16* hi, const _Float16* lo, const float* sh,
+                                            const ScreenLds& sl, int wave, std::size_t wave_id, std::size_t p0,
+                                            std::size_t p_end, int bands, unsigned& local) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int n = lane & 15;
+  const std::size_t px = p0 + static_cast<std::size_t>(16 * wave + n);
+  const bool flag = g == 0 && px < p_end && logit >= sl.fs.cut;
+  const unsigned long long m = __ballot(flag);
+  if (m == 0ull) return;
+  const int pad = 32 * sl.fs.nkb;
+  const std::size_t region = wave_id * sl.fs.rc;
+  for (unsigned long long mm = m; mm != 0ull; mm &= mm - 1ull) {
+    const int j = __ffsll(static_cast<long long>(mm)) - 1;
+    const unsigned slot = local + static_cast<unsigned>(__popcll(m & ((1ull << j) - 1ull)));
+    const float lg = __shfl(logit, j, 64);
+    if (slot < sl.fs.rc) {
+      const std::size_t row = region + slot;
+      const std::size_t pj = p0 + 16 * static_cast<std::size_t>(wave) + static_cast<std::size_t>(j);
+      if (lane == 0) {
+        const std::size_t w = static_cast<std::size_t>(sl.fs.width);
+        hsi_detection d;
+        d.x = static_cast<int>(pj % w);
+        d.y = static_cast<int>(pj / w);
+        d.target = static_cast<int>(row);
+        d.score = lg;
+        sl.fs.rec[row] = d;
+      }
+      if (sl.fs.spectra) {
+        float* dst = sl.fs.spectra + row * sl.fs.spectra_stride;
+        const int cj = 16 * wave + j;
+        for (int b = lane; b < pad; b += 64) {
+          float v = 0.f;
+          if (b < bands)
+            v = sh[b] + (static_cast<float>(hi[b * PITCH + cj]) + static_cast<float>(lo[b * PITCH + cj]));
+          dst[b] = v;
+        }
+      }
+    }
+  }
+  local += static_cast<unsigned>(__popcll(m));
+}
+
+template <typename In, int NB, int POS, int CH = f16_chunk<NB>(), bool SCREEN = false>
 __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cube, std::size_t p_begin,
                                                     std::size_t p_end, int bands, std::size_t stride,
                                                     const float* sh, const float* qo, const float* qs,
-                                                    _Float16* hi_t, _Float16* lo_t, double* __restrict__ out) {
-  constexpr int CH = f16_chunk<NB>();
+                                                    _Float16* hi_t, _Float16* lo_t, double* __restrict__ out,
+                                                    const ScreenLds* sl = nullptr) {
   constexpr int NP = NB * (NB + 1) / 2;
   constexpr int OWNERS = kWavesPerWg * f16_splits<NB>();
   constexpr int THREADS = kWavesPerWg * 64;
@@ -482,6 +620,13 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
     }
   };
 
+  // Screening: this wave's 16-pixel tile of each chunk, its candidate count.
+  const int screen_wave = static_cast<int>(threadIdx.x >> 6);
+  const std::size_t screen_wave_id = static_cast<std::size_t>(blockIdx.x) * kWavesPerWg + screen_wave;
+  unsigned screen_local = 0u;
+  (void)screen_wave;
+  (void)screen_wave_id;
+
   load(p_begin);
   store(0, p_begin);
   __syncthreads();
@@ -490,6 +635,9 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
     const std::size_t pn = p_begin + static_cast<std::size_t>(c + 1) * CH;
     if (more) load(pn);
     const int buf = c & 1;
+    float screen_lg = 0.f;
+    if constexpr (SCREEN) screen_lg = screen_logit<NB, PITCH, ROWS>(hi_t + buf * TILE, *sl, screen_wave);
+    (void)screen_lg;
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ++ks) {
       // Operands: lane l = 16g + m takes eight pixels 32ks + 8g .. +7 of band
@@ -510,6 +658,7 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
       }
       mfma_run_f16<run_first(POS, NP, OWNERS), NP, NB, OWNERS, POS>(hi, lo, acc);
     }
+
     // FP32 tiles of 128 pixels, then FP64.
     if ((c + 1) % (128 / CH) == 0 || !more) {
 #pragma unroll
@@ -520,8 +669,13 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
       }
     }
     if (more) store(buf ^ 1, pn);
+    if constexpr (SCREEN)
+      screen_emit<PITCH>(screen_lg, hi_t + buf * TILE, lo_t + buf * TILE, sh, *sl, screen_wave, screen_wave_id,
+                         p_begin + static_cast<std::size_t>(c) * CH, p_end, bands, screen_local);
     __syncthreads();
   }
+  if constexpr (SCREEN)
+    if (lane == 0) sl->fs.counts[screen_wave_id] = screen_local;
 
 #pragma unroll
   for (int m = 0; m < PW; ++m) {
@@ -585,6 +739,54 @@ __global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_e
     break;
     HSI_POS(0) HSI_POS(1) HSI_POS(2) HSI_POS(3) HSI_POS(4) HSI_POS(5) HSI_POS(6) HSI_POS(7)
     HSI_POS(8) HSI_POS(9) HSI_POS(10) HSI_POS(11) HSI_POS(12) HSI_POS(13) HSI_POS(14) HSI_POS(15)
+#undef HSI_POS
+  }
+}
+
+// Mixed-form moments with the learned pre-screen on the same staged tiles:
+// 128-pixel chunks, one workgroup per CU (the staging, the network's FP16
+// fragments and the registers of both do not fit two), wave w screening
+// pixels 16w .. 16w + 15 of each chunk. F32 cubes up to 8 moment blocks.
+template <int NB>
+__global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_eu(2))) void moments_screen_kernel(
+    const float* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride, std::size_t px_per_wg,
+    const float* __restrict__ shift, double* __restrict__ partial, const h8* __restrict__ w1h,
+    const h8* __restrict__ w2h, const float* __restrict__ tail, const FusedScreenDev fs) {
+  static_assert(NB <= 8 && f16_splits<NB>() == 1, "the fused screen covers up to 8 moment blocks");
+  constexpr int CH = kFusedChunk;
+  constexpr int THREADS = kWavesPerWg * 64;
+  constexpr int PITCH = CH + 8;
+  constexpr int ROWS = NB * 16;
+  constexpr int NKBM = (NB + 1) / 2;
+  static_assert(kWavesPerWg * 16 == CH, "one 16-pixel screening tile per wave");
+  __shared__ _Float16 hi_t[2 * ROWS * PITCH];
+  __shared__ _Float16 lo_t[2 * ROWS * PITCH];
+  __shared__ float sh[ROWS];
+  __shared__ float qz[1];
+  __shared__ h8 w1l[NKBM * 4 * 64];
+  __shared__ h8 w2l[4 * 64];
+  __shared__ float tl[129];
+  for (int b = threadIdx.x; b < ROWS; b += THREADS) sh[b] = b < bands ? shift[b] : 0.f;
+  // K blocks past ceil(bands / 32) get zero weights: the screen reads every block.
+  for (int e = threadIdx.x; e < NKBM * 4 * 64; e += THREADS) w1l[e] = e < fs.nkb * 4 * 64 ? w1h[e] : h8{};
+  for (int e = threadIdx.x; e < 4 * 64; e += THREADS) w2l[e] = w2h[e];
+  for (int e = threadIdx.x; e < 129; e += THREADS) tl[e] = tail[e];
+  if (threadIdx.x == 0) qz[0] = 0.f;
+  __syncthreads();
+  const std::size_t p_begin = blockIdx.x * px_per_wg;
+  std::size_t p_end = p_begin + px_per_wg;
+  if (p_end > pixels) p_end = pixels;
+  if (p_begin >= p_end) return;
+  constexpr int NP = NB * (NB + 1) / 2;
+  double* out = partial + static_cast<std::size_t>(blockIdx.x) * NP * 256;
+  const ScreenLds sl{w1l, w2l, tl, fs};
+  switch (wave_position(threadIdx.x >> 6)) {
+#define HSI_POS(K)                                                                                      \
+  case K:                                                                                               \
+    moments_mixed_owner<float, NB, K, CH, true>(cube, p_begin, p_end, bands, stride, sh, qz, qz, hi_t, lo_t, \
+                                                out, &sl);                                             \
+    break;
+    HSI_POS(0) HSI_POS(1) HSI_POS(2) HSI_POS(3) HSI_POS(4) HSI_POS(5) HSI_POS(6) HSI_POS(7)
 #undef HSI_POS
   }
 }
@@ -754,12 +956,17 @@ std::size_t mom_wgs(std::size_t pixels, int max_wgs = kMixedMaxWgs) {
   return chunks < static_cast<std::size_t>(max_wgs) ? chunks : static_cast<std::size_t>(max_wgs);
 }
 
+// fs non-null: the fused screening form (F32, up to 8 blocks; one workgroup
+// per CU).
 template <typename In, int NB>
-hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t stream) {
+hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t stream,
+                           const FusedScreenArgs* fs = nullptr) {
+  constexpr bool kCanScreen = std::is_same_v<In, float> && NB <= 8;
+  if (fs && !kCanScreen) return hipErrorInvalidValue;
   const std::size_t pixels = m.pixels;
   const int bands = m.bands;
   const std::size_t chunks = div_up(pixels, kChunk);
-  const std::size_t wgs = mom_wgs(pixels, f16_max_wgs<NB>());
+  const std::size_t wgs = mom_wgs(pixels, fs ? 256 : f16_max_wgs<NB>());
   const std::size_t px_per_wg = div_up(chunks, wgs) * kChunk;
   const std::size_t used = div_up(pixels, px_per_wg);
   const int np = NB * (NB + 1) / 2;
@@ -772,12 +979,29 @@ hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t str
   hipLaunchKernelGGL((shift_kernel<In>), dim3(bands), dim3(256), 0, stream, m.cube, pixels, m.stride, run_stride,
                      m.shift, m.q_offset, m.q_scale, shift, shift_d);
   if ((err = hipGetLastError()) != hipSuccess) return err;
-  // Ranges rounded up to whole groups of 8 so the range/split decode is
-  // exact; ranges past the pixel count return at once and write nothing.
-  const std::size_t ranges = f16_splits<NB>() > 1 ? round_up(used, kXcds) : used;
-  hipLaunchKernelGGL((moments_mixed_kernel<In, NB>), dim3(static_cast<unsigned>(ranges * f16_splits<NB>())),
-                     dim3(kWavesPerWg * 64), 0, stream, m.cube, pixels, bands, m.stride, px_per_wg, shift,
-                     m.q_offset, m.q_scale, partial);
+  if constexpr (kCanScreen) {
+    if (fs) {
+      const int waves = static_cast<int>(used) * kWavesPerWg;
+      const unsigned rc = fs->cap / static_cast<unsigned>(waves);
+      FusedScreenDev d{fs->cut, static_cast<int>(div_up(static_cast<std::size_t>(bands), 32)), fs->width, rc,
+                       fs->counts, fs->scratch_rec, fs->spectra, fs->spectra_stride};
+      hipLaunchKernelGGL(moments_screen_kernel<NB>, dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64), 0,
+                         stream, static_cast<const float*>(m.cube), pixels, bands, m.stride, px_per_wg, shift,
+                         partial, static_cast<const h8*>(fs->w1h), static_cast<const h8*>(fs->w2h), fs->tail, d);
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+      if ((err = launch_screen_compact(fs->counts, waves, rc, fs->cap, fs->counts + kScreenMaxWaves, fs->count,
+                                       fs->scratch_rec, fs->cand, stream)) != hipSuccess)
+        return err;
+    }
+  }
+  if (!fs) {
+    // Ranges rounded up to whole groups of 8 so the range/split decode is
+    // exact; ranges past the pixel count return at once and write nothing.
+    const std::size_t ranges = f16_splits<NB>() > 1 ? round_up(used, kXcds) : used;
+    hipLaunchKernelGGL((moments_mixed_kernel<In, NB>), dim3(static_cast<unsigned>(ranges * f16_splits<NB>())),
+                       dim3(kWavesPerWg * 64), 0, stream, m.cube, pixels, bands, m.stride, px_per_wg, shift,
+                       m.q_offset, m.q_scale, partial);
+  }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   ReduceArgs ra{partial, static_cast<int>(used), static_cast<std::size_t>(np) * 256, NB, bands, 0, 1,
                 0, 0, 0.0, m.count, m.sum, m.sum_outer};
@@ -801,6 +1025,15 @@ hipError_t launch_mixed_typed(const MomentArgs& m, double* scratch, hipStream_t 
     return launch_mixed_nb<In, N>(m, scratch, stream);
     HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8) HSI_NB(9)
     HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14) HSI_NB(15) HSI_NB(16) HSI_NB(17)
+#undef HSI_NB
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fused_screen(const MomentArgs& m, const FusedScreenArgs& fs, double* scratch, hipStream_t stream) {
+  switch (moment_blocks(m.bands)) {
+#define HSI_NB(N) case N: return launch_mixed_nb<float, N>(m, scratch, stream, &fs);
+    HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8)
 #undef HSI_NB
   }
   return hipErrorInvalidValue;
@@ -865,6 +1098,19 @@ hipError_t launch_moments(const MomentArgs& m, double* scratch, hipStream_t stre
     case HSI_Q16: return launch_mixed_typed<q16_t>(m, scratch, stream);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_moments_screen(const MomentArgs& m, const FusedScreenArgs& fs, double* scratch,
+                                 hipStream_t stream) {
+  if (m.precision != HSI_MIXED || m.dtype != HSI_F32 || m.bands < 1 || m.bands > kFusedScreenMaxBands ||
+      !fs.w1h || !fs.w2h || !fs.tail || !fs.count || !fs.cand || !fs.counts || !fs.scratch_rec || fs.width <= 0 ||
+      !fits_buffer(m.bands, m.stride, sizeof(float)))
+    return hipErrorInvalidValue;
+  // one workgroup per CU at most: 256 x 8 waves
+  static_assert(256 * kWavesPerWg <= kScreenMaxWaves, "screening counters");
+  if (fs.spectra && fs.spectra_stride < div_up(static_cast<std::size_t>(m.bands), 32) * 32)
+    return hipErrorInvalidValue;
+  return launch_fused_screen(m, fs, scratch, stream);
 }
 
 }  // namespace hsi_amd

@@ -158,6 +158,8 @@ def lib():
         "hsi_max_targets": (C.c_int, []),
         "hsi_max_bands": (C.c_int, []),
         "hsi_set_library": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int]),
+        "hsi_stats_accumulate_screen": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t, _vp,
+                                                  C.c_float, _vp, _vp, C.c_int, _vp, C.c_size_t, _vp]),
         "hsi_stats_accumulate": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.c_size_t, C.c_int, _vp, _vp]),
         "hsi_encode_q16": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t, _vp, _vp]),
@@ -195,7 +197,7 @@ def lib():
                                            C.c_size_t, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
         "hsi_mlp_screen": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_float,
                                      _vp, _vp, C.c_int, _vp, C.c_size_t, _vp, _vp]),
-        "hsi_score_candidate_spectra": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+        "hsi_score_candidate_spectra": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                                   C.c_size_t, _vp, _vp, C.c_int, _vp, C.c_size_t,
                                                   _vp, _vp, _vp, _vp]),
         "hsi_evaluate": (C.c_int, [_vp, _vp, _vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_float,
@@ -386,6 +388,20 @@ class Context:
                                                    bands, cube.stride(0), precision, _ptr(shift),
                                                    _stream(stream)))
 
+    def stats_accumulate_screen(self, cube, height, width, bands, cut, candidates, count, spectra=None,
+                                shift=None, stream=None):
+        """MIXED moments of a float32 cube and, from the same read, the
+        pre-screen's candidates at `cut` (records in `candidates` [cap][4]
+        int32, rows in `spectra` [cap][pitch] float32 when given); the
+        network sees the pixel minus `shift` (the sampled mean when None)."""
+        cap = candidates.shape[0]
+        if spectra is not None and spectra.shape[0] < cap:
+            raise ValueError(f"spectra hold {spectra.shape[0]} rows; the list's capacity is {cap}")
+        self._check(self._lib.hsi_stats_accumulate_screen(
+            self.handle, _ptr(cube), height, width, bands, cube.stride(0), _ptr(shift), float(cut),
+            _ptr(candidates), _ptr(count), cap, _ptr(spectra), spectra.stride(0) if spectra is not None else 0,
+            _stream(stream)))
+
     def encode_q16(self, src, height, width, bands, dst, stream=None):
         """hsi_encode_q16: device float32 (bands, stride) cube -> Q16 codes in
         `dst` (int16 or uint16 tensor of the same shape); the context keeps
@@ -546,12 +562,12 @@ class Context:
                                              _ptr(logit), _stream(stream)))
 
     def score_candidate_spectra(self, detector, cube, height, width, bands, candidates, count, spectra,
-                                score, target=None, aux=None, stream=None):
+                                score, target=None, aux=None, precision=PLAIN, stream=None):
         cap = candidates.shape[0]
         if spectra.shape[0] < cap:
             raise ValueError(f"spectra hold {spectra.shape[0]} rows; the list's capacity is {cap}")
         self._check(self._lib.hsi_score_candidate_spectra(
-            self.handle, int(detector), _ptr(cube), height, width, bands, cube.stride(0),
+            self.handle, int(detector), int(precision), _ptr(cube), height, width, bands, cube.stride(0),
             _ptr(candidates), _ptr(count), cap, _ptr(spectra), spectra.stride(0), _ptr(score),
             _ptr(target), _ptr(aux), _stream(stream)))
 

@@ -6,9 +6,10 @@ detectors will put over threshold. At run time one pass over the frame
 (hsi_mlp_screen) computes the logits, lists the pixels at or above a
 conservative cut and writes their spectra to rows of a buffer; only those
 candidates are rescored exactly
-(hsi_score_candidate_spectra: the plain kernel on those rows, so
-their scores are bit-identical to the full-frame path), and the usual
-threshold + NMS runs on the result. The plain full-frame detectors are
+(hsi_score_candidate_spectra on those rows: with precision PLAIN their
+scores are bit-identical to the plain full-frame path; with MIXED they are
+the precision form's, FP16 hi/lo matrix cores), and the usual threshold +
+NMS runs on the result. The plain full-frame detectors are
 the judge (training labels, recall) and the fallback (a candidate list that
 overflows its capacity makes the device score the whole frame).
 
@@ -25,13 +26,16 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import Context, DetectionParams, Detector, MlpTrainParams, rx_threshold
+from . import MIXED, PLAIN, Context, DetectionParams, Detector, MlpTrainParams, rx_threshold
 
 
 class Prescreen:
     def __init__(self, ctx: Context, bands: int, rx_false_alarm_rate=1e-6, ace_threshold=0.5,
-                 nms_radius=2, seed=1, capacity=1 << 19, label_slack=0.7):
+                 nms_radius=2, seed=1, capacity=1 << 20, label_slack=0.7, precision=PLAIN):
         import torch
+        if precision not in (PLAIN, MIXED):
+            raise ValueError("precision is PLAIN or MIXED")
+        self.precision = precision
         self.torch = torch
         self.ctx = ctx
         self.bands = bands
@@ -64,8 +68,8 @@ class Prescreen:
                           "dets": t.empty((1 << 16, 4), **i32), "ndet": t.empty(1, **i32)}
         return self._maps
 
-    def statistics(self, cube, h, w, loading=1e-6):
-        self.ctx.stats_accumulate(cube, h, w, self.bands)
+    def statistics(self, cube, h, w, loading=1e-6, precision=PLAIN):
+        self.ctx.stats_accumulate(cube, h, w, self.bands, precision=precision)
         self.ctx.stats_build(loading)
         self.ctx.stats_check()
 
@@ -154,7 +158,22 @@ class Prescreen:
         m = self._buffers(h * w)
         self.ctx.mlp_screen(cube, h, w, self.bands, self.cut, m["cand"], m["count"], m["spectra"])
         self.ctx.score_candidate_spectra(Detector.ACE, cube, h, w, self.bands, m["cand"], m["count"],
-                                         m["spectra"], m["ace"], m["target"], m["rx"])
+                                         m["spectra"], m["ace"], m["target"], m["rx"],
+                                         precision=self.precision)
+        return m
+
+    def screen_fused(self, cube, h, w, shift=None, loading=1e-6):
+        """The learned path from the frame alone: the mixed-form statistics
+        and the screen in one pass over the cube (the network sees the pixel
+        minus `shift`, the sampled mean when None), the build, and the
+        rescoring of the candidate rows; returns the score buffers."""
+        m = self._buffers(h * w)
+        self.ctx.stats_accumulate_screen(cube, h, w, self.bands, self.cut, m["cand"], m["count"], m["spectra"],
+                                         shift=shift)
+        self.ctx.stats_build(loading)
+        self.ctx.score_candidate_spectra(Detector.ACE, cube, h, w, self.bands, m["cand"], m["count"],
+                                         m["spectra"], m["ace"], m["target"], m["rx"],
+                                         precision=self.precision)
         return m
 
     def screen_two_pass(self, cube, h, w):
@@ -181,16 +200,37 @@ class Prescreen:
             out.append(set(map(tuple, m["dets"][:k, :2].cpu().numpy().tolist())))
         return out
 
-    def evaluate(self, cube, h, w):
+    def evaluate(self, cube, h, w, fused=False):
         """Recall of the learned path against the plain path on one frame
-        (statistics built here)."""
+        (statistics built here). With precision MIXED the learned path also
+        builds its statistics in the mixed form (the plain path keeps the
+        plain form), and the result holds the rescored candidates' largest
+        error against the plain scores: RX relative, ACE absolute. fused:
+        the learned path is screen_fused (statistics and screen in one
+        pass)."""
         self.statistics(cube, h, w)
-        plain = self.detections(self.plain(cube, h, w), h, w)
-        m = self.screen(cube, h, w)
+        m = self.plain(cube, h, w)
+        plain = self.detections(m, h, w)
+        ref = None
+        if self.precision == MIXED or fused:
+            ref = (m["rx"].clone(), m["ace"].clone())
+        if fused:
+            m = self.screen_fused(cube, h, w)
+        else:
+            if self.precision == MIXED:
+                self.statistics(cube, h, w, precision=MIXED)
+            m = self.screen(cube, h, w)
         self.torch.cuda.synchronize()
         flagged = int(m["count"].item())
         learned = self.detections(m, h, w)
         res = {"candidates": flagged, "candidate_fraction": flagged / (h * w)}
+        if ref is not None and 0 < flagged <= self.capacity:
+            t = self.torch
+            rec = m["cand"][:flagged].long()
+            px = rec[:, 1] * w + rec[:, 0]
+            rx_p, rx_l = ref[0][px].double(), m["rx"][px].double()
+            res["rx_max_rel_err"] = float((t.abs(rx_l - rx_p) / t.clamp(t.abs(rx_p), min=1e-30)).max())
+            res["ace_max_abs_err"] = float(t.abs(m["ace"][px].double() - ref[1][px].double()).max())
         for name, want, got in (("rx", plain[0], learned[0]), ("ace", plain[1], learned[1])):
             hit = len(want & got)
             res[name] = {"plain": len(want), "learned": len(got), "recall": hit / max(len(want), 1),

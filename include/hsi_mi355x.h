@@ -155,6 +155,32 @@ hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtyp
                                 hsi_precision precision, const float* d_shift_or_null,
                                 hsi_stream stream);
 
+/* hsi_stats_accumulate (F32 cube, HSI_MIXED, same shift rule) with the
+ * learned pre-screen (hsi_mlp_screen's network, installed for `bands`) run
+ * on the same read of every pixel: no counterpart in the reference; it
+ * replaces a statistics pass plus a screening pass over the frame with one.
+ * The network's input is the pixel minus the shift - the caller's
+ * d_shift_or_null (the previous frame's mean, say) or the sampled mean - in
+ * FP16, since the frame's own mean is known only after the pass. Every pixel
+ * whose logit is >= cut is listed in d_candidates[0 .. *d_count) as
+ * {x, y, row, logit}, in a fixed order that is not raster order; with
+ * d_spectra non-null (pitch spectra_stride >= 32 * ceil(bands / 32)) row
+ * `row` receives the pixel's bands as the pass staged them, shift + (hi +
+ * lo) in FP32 (within 2^-21 |x - shift| + 2^-23 |x| + 2^-24 of each value),
+ * and 0
+ * from `bands` to the 32-band boundary; rows are distinct but not in list
+ * order. The pass splits the capacity evenly over its waves (at most 2048);
+ * if any wave flags more than its share, or the total exceeds `capacity`,
+ * *d_count is set above `capacity` (the list overflowed, and
+ * hsi_score_candidate_spectra then scores the whole frame). At most 127
+ * bands. Asynchronous: build the statistics, then rescore the rows. */
+hsi_status hsi_stats_accumulate_screen(hsi_ctx* ctx, const float* d_cube, int height, int width,
+                                       int bands, size_t plane_stride,
+                                       const float* d_shift_or_null, float cut,
+                                       hsi_detection* d_candidates, uint32_t* d_count,
+                                       int capacity, float* d_spectra, size_t spectra_stride,
+                                       hsi_stream stream);
+
 /* Derive DetectorStatistics on the device in FP64 from the accumulated
  * moments and the installed library: covariance and autocorrelation with
  * diagonal loading, their Cholesky factors, the whitener L^-1, whitened
@@ -363,19 +389,23 @@ hsi_status hsi_mlp_screen(hsi_ctx* ctx, const float* d_cube, int height, int wid
                           size_t spectra_stride, float* d_logit, hsi_stream stream);
 
 /* hsi_score_candidates over the rows that hsi_mlp_screen wrote: record i
- * is scored from row d_candidates[i].target of d_spectra. Only a list that
- * hsi_mlp_screen wrote (with these spectra) is valid input: the target field
- * of any other list (hsi_detect's target index) names no spectra row. Rows:
- * (capacity rows,
- * pitch spectra_stride >= bands; a record whose row is outside
- * [0, capacity) is skipped) and written at its (x, y). The whole-frame fallback on overflow reads d_cube.
- * Same results as hsi_score_candidates. Asynchronous. */
-hsi_status hsi_score_candidate_spectra(hsi_ctx* ctx, hsi_detector detector, const float* d_cube,
-                                       int height, int width, int bands, size_t plane_stride,
-                                       const hsi_detection* d_candidates, const uint32_t* d_count,
-                                       int capacity, const float* d_spectra, size_t spectra_stride,
-                                       float* d_score, int32_t* d_target, float* d_aux,
-                                       hsi_stream stream);
+ * is scored from row d_candidates[i].target of d_spectra and written at its
+ * (x, y). Only a list that hsi_mlp_screen wrote (with these spectra) is
+ * valid input: the target field of any other list (hsi_detect's target
+ * index) names no spectra row. Rows: capacity rows of pitch spectra_stride
+ * >= bands; a record whose row is outside [0, capacity) is skipped; a row's
+ * elements past `bands` are ignored. The whole-frame fallback on overflow
+ * reads d_cube. HSI_PLAIN: the same results as hsi_score_candidates.
+ * HSI_MIXED: the precision form of hsi_score (FP16 hi/lo matrix cores) on
+ * the rows, and on the cube for the fallback; the rows must be 16-byte
+ * aligned (spectra_stride a multiple of 4) and span less than 2 GiB
+ * (capacity * spectra_stride * 4 bytes). Asynchronous. */
+hsi_status hsi_score_candidate_spectra(hsi_ctx* ctx, hsi_detector detector, hsi_precision precision,
+                                       const float* d_cube, int height, int width, int bands,
+                                       size_t plane_stride, const hsi_detection* d_candidates,
+                                       const uint32_t* d_count, int capacity, const float* d_spectra,
+                                       size_t spectra_stride, float* d_score, int32_t* d_target,
+                                       float* d_aux, hsi_stream stream);
 
 /* ---- offline evaluation ------------------------------------------------------
  *

@@ -85,6 +85,21 @@ def bf16(a):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def f16(a):
+    """Round to IEEE half precision (nearest, ties to even), as float64."""
+    return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
+
+
+def forward_f16(w, x, bands):
+    """The screen fused into the statistics pass: FP16 weights, inputs and
+    layer-1 activations, FP32-class accumulation (here float64), FP32
+    biases. `x` is the pixel minus the shift, already rounded to FP32."""
+    W1, b1, W2, b2, w3, b3 = unpack(w, bands)
+    h1 = np.maximum(f16(W1) @ f16(x) + b1[:, None], 0.0)
+    h2 = np.maximum(f16(W2) @ f16(h1) + b2[:, None], 0.0)
+    return w3 @ h2 + b3
+
+
 def forward_bf16(w, x, bands):
     """The inference kernel's arithmetic: bf16 weights, inputs and layer-1
     activations, FP32-class accumulation (here float64), FP32 biases."""

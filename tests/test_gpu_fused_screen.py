@@ -1,0 +1,185 @@
+"""Scene statistics with the learned pre-screen on the same read
+(hsi_stats_accumulate_screen) against the oracles: the moments against the
+reference's FP64 moments (golden fixtures) or float64 numpy at the mixed
+form's 1e-6 bar; the candidate list against the FP16 network oracle
+(oracle/mlp_ref.forward_f16) applied to the pixel minus the shift, with
+pixels within NEAR of the cut counted apart; each candidate's spectrum
+against the cube (to the staged hi + lo parts' precision); the overflow
+rule; and the learned path end to end
+(build, precision-form rescoring of the rows) against the golden scores."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from gpu_util import FLOOR, rel_err
+from oracle import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+# The kernel accumulates layer 1 in FP32 and rounds its activations to FP16,
+# so a logit can sit one FP16 step of a hidden unit away from the float64
+# oracle's: 1e-2 of the logit bounds that for these networks.
+NEAR = 1e-2
+MIXED_REL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def H():
+    import torch
+    import hsi_mi355x
+    assert torch.cuda.is_available()
+    return hsi_mi355x
+
+
+def oracle_logits(w, host, shift, bands):
+    d = host.reshape(bands, -1).astype(np.float32) - shift.astype(np.float32)[:, None]
+    return mlp_ref.forward_f16(w, d.astype(np.float64), bands)
+
+
+def run(H, ctx, cube, h, w, b, cut, cap, pitch, shift=None):
+    import torch
+    cand = torch.full((cap, 4), -9, dtype=torch.int32, device="cuda")
+    count = torch.full((1,), 777, dtype=torch.int32, device="cuda")
+    spectra = torch.full((cap, pitch), float("nan"), dtype=torch.float32, device="cuda")
+    ctx.stats_accumulate_screen(cube, h, w, b, cut, cand, count, spectra, shift=shift)
+    torch.cuda.synchronize()
+    return cand, count, spectra
+
+
+def check_list(cand, count, spectra, host, shift, h, w, b, cap):
+    """Records 0 .. n-1 (n = *count, at most cap): distinct in-frame pixels,
+    distinct rows inside the capacity; a row holds the pixel's values within
+    2^-21 |x - shift| + 2^-23 |x| + 2^-24 (its staged FP16 hi + lo parts plus
+    the shift: the FP32 subtraction, lo's rounding - absolute 2^-25 where lo
+    is subnormal - and the final add), zeros up to the 32-band boundary, and
+    the fill beyond; records past n are untouched. Returns (pixels, logits,
+    count)."""
+    total = int(count.item())
+    n = min(total, cap)
+    rec = cand[:n].cpu().numpy()
+    rows = rec[:, 2]
+    assert np.all((rows >= 0) & (rows < cap)) and np.unique(rows).size == n
+    assert np.all((rec[:, 0] >= 0) & (rec[:, 0] < w) & (rec[:, 1] >= 0) & (rec[:, 1] < h))
+    px = rec[:, 1].astype(np.int64) * w + rec[:, 0]
+    assert np.unique(px).size == n
+    pb = (b + 31) // 32 * 32
+    sp = spectra.cpu().numpy()[rows].astype(np.float64)
+    x = host.reshape(b, -1)[:, px].T.astype(np.float64)
+    if shift is not None:
+        d = np.abs(x - shift.astype(np.float64)[None, :])
+        assert np.all(np.abs(sp[:, :b] - x) <= 2.0 ** -21 * d + 2.0 ** -23 * np.abs(x) + 2.0 ** -24)
+    else:
+        assert np.all(np.abs(sp[:, :b] - x) <= 2.0 ** -21 * (np.abs(x) + 1.0))
+    assert np.all(sp[:, b:pb] == 0.0)
+    assert np.all(np.isnan(sp[:, pb:]))
+    if cap > n:
+        assert np.all(cand[n:].cpu().numpy() == -9)
+    return px, rec[:, 3].view(np.float32).astype(np.float64), total
+
+
+@pytest.mark.parametrize("name", ["scene_24x32x16_t3", "scene_32x40x113_t4", "scene_13x37x113_t8"])
+def test_fused_statistics_and_screen_against_oracles(H, name):
+    import torch
+    g = golden(name)
+    b, h, w = g["cube"].shape
+    p = h * w
+    T = g["library"].shape[0]
+    cube = H.to_device_cube(g["cube"], h, w)
+    ctx = H.Context(0)
+    ctx.set_library(g["library"], g["norms"])
+    ctx.mlp_init(b, seed=31)
+    wts = ctx.mlp_get_weights().astype(np.float64)
+    host = g["cube"].reshape(b, p)
+    shift_h = (host.mean(axis=1) + 0.01).astype(np.float32)
+    ref = oracle_logits(wts, host, shift_h, b)
+    cut = float(np.quantile(ref, 0.7))
+    cap = 8 * p  # every wave's share of the capacity covers all of its pixels
+    pitch = (b + 31) // 32 * 32 + 8
+    shift = torch.from_numpy(shift_h).cuda()
+    cand, count, spectra = run(H, ctx, cube, h, w, b, cut, cap, pitch, shift)
+    px, lg, total = check_list(cand, count, spectra, host, shift_h, h, w, b, cap)
+    # the list against the oracle's flags
+    want = set(np.flatnonzero(ref >= cut).tolist())
+    got = set(px.tolist())
+    odd = np.array(sorted(want ^ got), dtype=np.int64)
+    print(f"{name}: {len(got)} flagged, oracle {len(want)}, {odd.size} within NEAR of the cut; "
+          f"logit error {np.abs(lg - ref[px]).max():.3g}", flush=True)
+    assert np.all(np.abs(ref[odd] - cut) <= NEAR)
+    assert np.all(np.abs(lg - ref[px]) <= NEAR)
+    assert np.all(lg >= cut)
+    # the moments from the same pass, then the learned path end to end
+    ctx.stats_build(float(g["loading"]))
+    ctx.stats_check()
+    st = ctx.stats_read(b, T)
+    lower = np.tril_indices(b)
+    assert st["count"][0] == g["m_count"]
+    assert rel_err(st["sum"], g["m_sum"], 1e-30).max() <= 1e-6
+    assert rel_err(st["sum_outer"][lower], g["m_outer"][lower], 1e-30).max() <= 1e-6
+    f32 = dict(dtype=torch.float32, device="cuda")
+    s = torch.full((p,), float("nan"), **f32)
+    a = torch.full((p,), float("nan"), **f32)
+    t = torch.full((p,), -7, dtype=torch.int32, device="cuda")
+    ctx.score_candidate_spectra(H.Detector.ACE, cube, h, w, b, cand, count, spectra, s, t, a, precision=H.MIXED)
+    torch.cuda.synchronize()
+    sv, av = s.cpu().numpy(), a.cpu().numpy()
+    e_ace = rel_err(sv[px], g["ace_score"][px], FLOOR["ace"]).max()
+    e_rx = rel_err(av[px], g["rx_score"][px], FLOOR["rx"]).max()
+    print(f"{name}: learned path at the candidates: ACE {e_ace:.3g}, RX {e_rx:.3g}")
+    assert e_ace <= MIXED_REL and e_rx <= MIXED_REL
+    mask = np.ones(p, bool)
+    mask[px] = False
+    assert np.all(sv[mask] == 0.0) and np.all(av[mask] == 0.0)
+    ctx.close()
+
+
+def test_fused_screen_at_frame_size_and_overflow(H):
+    """A 384 x 500 x 113 frame (many workgroups, a partial last chunk): the
+    list against the oracle's flags, the moments against float64, the
+    sampled shift's moments and list, and a capacity below the count: the
+    count is raised past it and only true candidates are written."""
+    import torch
+    h, w, b = 384, 500, 113
+    p = h * w
+    cfg = H.synth_config(h, w, b, 8, seed=99, anomaly_fraction=0.01)
+    host = H.synth_frame(cfg, 0, stride=p, threads=8)
+    cube = torch.from_numpy(host).cuda()
+    ctx = H.Context(0)
+    ctx.set_library(H.synth_library(cfg))
+    ctx.mlp_init(b, seed=5)
+    wts = ctx.mlp_get_weights().astype(np.float64)
+    x64 = host.astype(np.float64)
+    shift_h = x64.mean(axis=1).astype(np.float32)
+    ref = oracle_logits(wts, host, shift_h, b)
+    cut = float(np.quantile(ref, 0.95))
+    cap = 2 * p
+    pitch = 128
+    shift = torch.from_numpy(shift_h).cuda()
+    cand, count, spectra = run(H, ctx, cube, h, w, b, cut, cap, pitch, shift)
+    px, lg, total = check_list(cand, count, spectra, host, shift_h, h, w, b, cap)
+    assert total <= cap
+    want = set(np.flatnonzero(ref >= cut).tolist())
+    odd = np.array(sorted(want ^ set(px.tolist())), dtype=np.int64)
+    print(f"frame: {total} flagged, oracle {len(want)}, {odd.size} near the cut", flush=True)
+    assert np.all(np.abs(ref[odd] - cut) <= NEAR)
+    assert np.all(np.abs(lg - ref[px]) <= NEAR)
+    m = ctx.moments_read(b)
+    lower = np.tril_indices(b)
+    assert m["count"][0] == p
+    assert rel_err(m["sum"], x64.sum(axis=1), 1e-30).max() <= 1e-6
+    assert rel_err(m["sum_outer"][lower], (x64 @ x64.T)[lower], 1e-30).max() <= 1e-6
+    # the sampled shift: same moments bar, a consistent list
+    cand2, count2, spectra2 = run(H, ctx, cube, h, w, b, cut, cap, pitch)
+    check_list(cand2, count2, spectra2, host, None, h, w, b, cap)
+    m2 = ctx.moments_read(b)
+    assert rel_err(m2["sum_outer"][lower], (x64 @ x64.T)[lower], 1e-30).max() <= 1e-6
+    # overflow: a third of the candidates' number as the capacity - the
+    # count is raised past it, the records written are true candidates
+    small = total // 3
+    cand3, count3, spectra3 = run(H, ctx, cube, h, w, b, cut, small, pitch, shift)
+    assert int(count3.item()) > small
+    rec3 = cand3.cpu().numpy()
+    written = rec3[rec3[:, 0] != -9]
+    assert written.shape[0] <= small
+    px3 = written[:, 1].astype(np.int64) * w + written[:, 0]
+    assert set(px3.tolist()) <= set(px.tolist())
+    ctx.close()

@@ -272,3 +272,79 @@ def test_one_pass_screen_matches_logits_and_full_frame(H, name):
     assert np.array_equal(s.cpu().numpy().view(np.uint32), full_s.cpu().numpy().view(np.uint32))
     assert np.array_equal(t.cpu().numpy(), full_t.cpu().numpy())
     ctx.close()
+
+
+@pytest.mark.parametrize("det", ["rx", "ace"])
+@pytest.mark.parametrize("name", ["scene_32x40x113_t4", "scene_13x37x113_t8", "scene_20x20x224_t16"])
+def test_mixed_rescoring_of_candidate_rows_matches_full_frame_mixed(H, name, det):
+    """The precision form over candidate spectra rows (any row order, a pitch
+    past the band count whose extra elements are NaN, records outside the
+    frame or naming no row skipped) gives each listed pixel the full-frame
+    precision-form score (FP32 rounding of the accumulation apart: 2e-6
+    relative) and the same ACE target; every other pixel keeps the fill; an
+    overflowing list falls back to the whole frame in that form."""
+    import torch
+    from conftest import golden
+    g = golden(name)
+    b, h, w = g["cube"].shape
+    p = h * w
+    D = H.Detector.RX if det == "rx" else H.Detector.ACE
+    cube = H.to_device_cube(g["cube"], h, w)
+    ctx = H.Context(0)
+    ctx.set_library(g["library"])
+    ctx.stats_accumulate(cube, h, w, b)
+    ctx.stats_build(1e-6)
+    f32 = dict(dtype=torch.float32, device="cuda")
+    i32 = dict(dtype=torch.int32, device="cuda")
+    full_s, full_a = torch.empty(p, **f32), torch.empty(p, **f32)
+    full_t = torch.empty(p, **i32)
+    ctx.score(D, cube, h, w, b, full_s, full_t, full_a, precision=H.MIXED)
+    rng = np.random.default_rng(5)
+    pick = np.sort(rng.choice(p, size=max(1, p // 3), replace=False))
+    n = pick.size
+    cap = n + 7
+    pitch = (b + 3) // 4 * 4 + 4
+    rows = rng.permutation(cap)[:n]
+    host = g["cube"].reshape(b, p)
+    spec = np.full((cap, pitch), np.nan, np.float32)
+    spec[rows, :b] = host[:, pick].T
+    rec = np.zeros((cap, 4), np.int32)
+    rec[:n, 0] = pick % w
+    rec[:n, 1] = pick // w
+    rec[:n, 2] = rows
+    # two extra records that name no pixel: outside the frame, and a row past the capacity
+    rec[n] = (w, 0, rows[0], 0)
+    rec[n + 1] = (0, 0, cap, 0)
+    cand = torch.from_numpy(rec).cuda()
+    spectra = torch.from_numpy(spec).cuda()
+    count = torch.tensor([n + 2], **i32)
+    s = torch.full((p,), float("nan"), **f32)
+    a = torch.full((p,), float("nan"), **f32)
+    t = torch.full((p,), -7, **i32)
+    ctx.score_candidate_spectra(D, cube, h, w, b, cand, count, spectra, s, t, a, precision=H.MIXED)
+    torch.cuda.synchronize()
+    mask = np.zeros(p, bool)
+    mask[pick] = True
+    fs = full_s.cpu().numpy()
+    gs = s.cpu().numpy()
+    rel = np.abs(gs[mask] - fs[mask]) / np.maximum(np.abs(fs[mask]), 1e-30)
+    print(f"{name} {det}: {n} rows, max relative difference {rel.max():.3g}", flush=True)
+    assert rel.max() <= 2e-6
+    assert np.all(gs[~mask] == 0.0)
+    if det == "ace":
+        assert np.array_equal(t.cpu().numpy()[mask], full_t.cpu().numpy()[mask])
+        assert np.all(t.cpu().numpy()[~mask] == -1)
+        ga, fa = a.cpu().numpy(), full_a.cpu().numpy()
+        assert np.all(np.abs(ga[mask] - fa[mask]) <= 2e-6 * np.abs(fa[mask]))
+    # a count above the capacity: the whole frame in the precision form
+    count.fill_(cap + 1)
+    ctx.score_candidate_spectra(D, cube, h, w, b, cand, count, spectra, s, t, a, precision=H.MIXED)
+    torch.cuda.synchronize()
+    gs = s.cpu().numpy()
+    assert np.all(np.abs(gs - fs) <= 2e-6 * np.abs(fs))
+    if det == "ace":
+        assert np.array_equal(t.cpu().numpy(), full_t.cpu().numpy())
+    # rows need 16-byte alignment in this form
+    with pytest.raises(H.HsiError):
+        ctx.score_candidate_spectra(D, cube, h, w, b, cand, count, spectra[:, 1:], s, t, a, precision=H.MIXED)
+    ctx.close()

@@ -109,3 +109,46 @@ def test_forward_bf16_hand_worked():
     assert got[0] == 3.305908203125 and got[1] == -0.5
     # the float64 network differs on sample 0: every rounding step matters
     assert mlp_ref.forward(w, x, bands)[0][0] != got[0]
+
+
+def test_forward_f16_equals_forward_on_f16_exact_inputs():
+    """Values that half precision holds exactly (multiples of 1/16 below 8):
+    every rounding step is the identity, so the FP16 pass equals float64."""
+    bands, n = 5, 40
+    rng = np.random.default_rng(9)
+    P = mlp_ref.param_count(bands)
+    w = rng.integers(-4, 5, P) / 4.0
+    W1, b1, W2, b2, w3, b3 = mlp_ref.unpack(w, bands)
+    b1[:] = rng.integers(-8, 9, b1.shape) / 16.0
+    w = mlp_ref.pack(W1, b1, W2, b2, w3, b3)
+    x = rng.integers(-4, 5, (bands, n)) / 4.0
+    z, h1, _ = mlp_ref.forward(w, x, bands)
+    assert np.array_equal(mlp_ref.f16(h1), h1)
+    assert np.abs(h1).max() > 0
+    assert np.array_equal(mlp_ref.forward_f16(w, x, bands), z)
+
+
+def test_forward_f16_hand_worked():
+    """One live unit per layer. Half precision keeps 11 significant bits:
+    f16(1.0001) = 1.0 (the next half above 1 is 1 + 2^-10), f16(0.3) =
+    1229/4096 = 0.300048828125, f16(0.51) = 1044/2048 = 0.509765625.
+    Sample 0, x = (3, 0.3): W1.x = 3 + 2*1229/4096 = 3.60009765625;
+      + b1 = 3.70009765625 -> f16 1894/512 = 3.69921875; W2 h1 + b2 =
+      (1044/2048)(1894/512) + 1/4 = 2.13573455810546875; logit = 2*that - 1
+      = 214399/65536 = 3.2714691162109375.
+    Sample 1, x = (-1, 0): relu 0; logit = 2*0.25 - 1 = -0.5."""
+    bands = 2
+    w = np.zeros(mlp_ref.param_count(bands))
+    W1, b1, W2, b2, w3, b3 = mlp_ref.unpack(w, bands)
+    W1 = W1.copy(); b1 = b1.copy(); W2 = W2.copy(); b2 = b2.copy(); w3 = w3.copy()
+    W1[0] = [1.0001, 2.0]
+    b1[0] = 0.1
+    W2[0, 0] = 0.51
+    b2[0] = 0.25
+    w3[0] = 2.0
+    w = mlp_ref.pack(W1, b1, W2, b2, w3, -1.0)
+    x = np.array([[3.0, -1.0], [0.3, 0.0]])
+    got = mlp_ref.forward_f16(w, x, bands)
+    assert got[1] == -0.5
+    assert got[0] == 214399 / 65536
+    assert mlp_ref.forward(w, x, bands)[0][0] != got[0]
