@@ -600,9 +600,13 @@ __global__ __launch_bounds__(kScaleThreads) void mixed_scales_kernel(MixedLayout
   {
     const int b = tid % cols;
     float m = 0.f;
+    // unconditional loads (b < n, i < n), so they are all in flight at once
     if (b < n)
-      for (int i = tid / cols; i < n; i += phases)
-        if (b <= i) m = fmaxf(m, fabsf(a.whitener[static_cast<std::size_t>(i) * n + b]));
+#pragma unroll 8
+      for (int i = tid / cols; i < n; i += phases) {
+        const float v = fabsf(a.whitener[static_cast<std::size_t>(i) * n + b]);
+        m = b <= i ? fmaxf(m, v) : m;
+      }
     part[tid] = m;
   }
   __syncthreads();
@@ -617,7 +621,10 @@ __global__ __launch_bounds__(kScaleThreads) void mixed_scales_kernel(MixedLayout
   for (int i = wave; i < kMaxBands; i += kWaves) {
     float m = 0.f;
     if (i < n)
-      for (int b = lane; b <= i; b += 64) m = fmaxf(m, fabsf(a.whitener[static_cast<std::size_t>(i) * n + b] / cs[b]));
+      for (int b = lane; b < n; b += 64) {
+        const float v = fabsf(a.whitener[static_cast<std::size_t>(i) * n + b] / cs[b]);
+        m = b <= i ? fmaxf(m, v) : m;
+      }
     m = wave_max(m);
     if (lane == 0) rs[i] = pow2_for(m);
   }

@@ -525,33 +525,55 @@ __device__ __forceinline__ void screen_emit(float logit, const _Float16* hi, con
   if (m == 0ull) return;
   const int pad = 32 * sl.fs.nkb;
   const std::size_t region = wave_id * sl.fs.rc;
-  for (unsigned long long mm = m; mm != 0ull; mm &= mm - 1ull) {
-    const int j = __ffsll(static_cast<long long>(mm)) - 1;
-    const unsigned slot = local + static_cast<unsigned>(__popcll(m & ((1ull << j) - 1ull)));
-    const float lg = __shfl(logit, j, 64);
-    if (slot < sl.fs.rc) {
-      const std::size_t row = region + slot;
-      const std::size_t pj = p0 + 16 * static_cast<std::size_t>(wave) + static_cast<std::size_t>(j);
+  const std::size_t w = static_cast<std::size_t>(sl.fs.width);
+  // Four candidates a round (the mask and the candidates' columns are
+  // wave-uniform, so the bit walk is scalar): all of a round's LDS reads
+  // are issued before any store.
+  constexpr int kRound = 4;
+  unsigned slot = local;
+  for (unsigned long long mm = m; mm != 0ull;) {
+    int js[kRound];
+#pragma unroll
+    for (int u = 0; u < kRound; ++u) {
+      js[u] = mm != 0ull ? __ffsll(static_cast<long long>(mm)) - 1 : -1;
+      if (mm != 0ull) mm &= mm - 1ull;
+    }
+    float v0[kRound], v1[kRound], lg[kRound];
+#pragma unroll
+    for (int u = 0; u < kRound; ++u) {
+      v0[u] = v1[u] = 0.f;
+      lg[u] = __shfl(logit, js[u] < 0 ? 0 : js[u], 64);
+      if (js[u] >= 0 && sl.fs.spectra) {
+        const int cj = 16 * wave + js[u];
+        const int b0 = lane, b1 = lane + 64;
+        if (b0 < bands)
+          v0[u] = sh[b0] + (static_cast<float>(hi[b0 * PITCH + cj]) + static_cast<float>(lo[b0 * PITCH + cj]));
+        if (b1 < bands)
+          v1[u] = sh[b1] + (static_cast<float>(hi[b1 * PITCH + cj]) + static_cast<float>(lo[b1 * PITCH + cj]));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRound; ++u) {
+      if (js[u] < 0) continue;
+      const unsigned su = slot + static_cast<unsigned>(u);
+      if (su >= sl.fs.rc) continue;
+      const std::size_t row = region + su;
       if (lane == 0) {
-        const std::size_t w = static_cast<std::size_t>(sl.fs.width);
+        const std::size_t pj = p0 + 16 * static_cast<std::size_t>(wave) + static_cast<std::size_t>(js[u]);
         hsi_detection d;
         d.x = static_cast<int>(pj % w);
         d.y = static_cast<int>(pj / w);
         d.target = static_cast<int>(row);
-        d.score = lg;
+        d.score = lg[u];
         sl.fs.rec[row] = d;
       }
       if (sl.fs.spectra) {
         float* dst = sl.fs.spectra + row * sl.fs.spectra_stride;
-        const int cj = 16 * wave + j;
-        for (int b = lane; b < pad; b += 64) {
-          float v = 0.f;
-          if (b < bands)
-            v = sh[b] + (static_cast<float>(hi[b * PITCH + cj]) + static_cast<float>(lo[b * PITCH + cj]));
-          dst[b] = v;
-        }
+        if (lane < pad) dst[lane] = v0[u];
+        if (lane + 64 < pad) dst[lane + 64] = v1[u];
       }
     }
+    slot += kRound;
   }
   local += static_cast<unsigned>(__popcll(m));
 }
@@ -902,15 +924,17 @@ __global__ __launch_bounds__(256) void shift_kernel(const void* __restrict__ cub
   const std::size_t runs = div_up(pixels, run_stride);
   const std::size_t samples = runs * 64;
   double s = 0.0, c = 0.0;
+  // Loads from a clamped index with no branch around them, so a thread's
+  // samples are all in flight at once; samples past the frame add 0.
   for (std::size_t i = threadIdx.x; i < samples; i += 256) {
     const std::size_t p = (i / 64) * run_stride + (i % 64);
-    if (p < pixels) {
-      float v;
-      if constexpr (is_q16<In>::value) v = q16_decode(static_cast<float>(plane[p].code), q_scale[b], q_offset[b]);
-      else v = static_cast<float>(plane[p]);
-      s += v;
-      c += 1.0;
-    }
+    const bool in = p < pixels;
+    const std::size_t pc = in ? p : 0;
+    float v;
+    if constexpr (is_q16<In>::value) v = q16_decode(static_cast<float>(plane[pc].code), q_scale[b], q_offset[b]);
+    else v = static_cast<float>(plane[pc]);
+    s += in ? static_cast<double>(v) : 0.0;
+    c += in ? 1.0 : 0.0;
   }
   red[threadIdx.x] = s;
   cnt[threadIdx.x] = c;
@@ -930,10 +954,12 @@ __global__ __launch_bounds__(256) void shift_kernel(const void* __restrict__ cub
 }
 
 // Raw moments from shifted ones (moments_from_shifted, linalg.cpp:81-103):
-// M1 = S1 + N r, M2 = S2 + r S1^T + S1 r^T + N r r^T, lower triangle. M2 is
-// rebuilt first (it reads S1), M1 by a second launch.
+// M1 = S1 + N r, M2 = S2 + r S1^T + S1 r^T + N r r^T, lower triangle, in one
+// launch: the shifted sums S1 come in their own array, so M2 reads them
+// while M1 is written.
 __global__ void unshift_kernel(int bands, const double* __restrict__ r, const double* __restrict__ count,
-                               const double* __restrict__ sum, double* __restrict__ sum_outer) {
+                               const double* __restrict__ shifted_sum, double* __restrict__ sum,
+                               double* __restrict__ sum_outer) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int tri = bands * (bands + 1) / 2;
   if (e >= tri) return;
@@ -942,13 +968,8 @@ __global__ void unshift_kernel(int bands, const double* __restrict__ r, const do
   const int j = e - i * (i + 1) / 2;
   const double n = *count;
   double* m2 = sum_outer + static_cast<std::size_t>(i) * bands + j;
-  *m2 = *m2 + r[i] * sum[j] + sum[i] * r[j] + n * r[i] * r[j];
-}
-
-__global__ void unshift_sum_kernel(int bands, const double* __restrict__ r,
-                                   const double* __restrict__ count, double* __restrict__ sum) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < bands) sum[i] = sum[i] + *count * r[i];
+  *m2 = *m2 + r[i] * shifted_sum[j] + shifted_sum[i] * r[j] + n * r[i] * r[j];
+  if (e < bands) sum[e] = shifted_sum[e] + n * r[e];
 }
 
 std::size_t mom_wgs(std::size_t pixels, int max_wgs = kMixedMaxWgs) {
@@ -972,7 +993,8 @@ hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t str
   const int np = NB * (NB + 1) / 2;
   double* partial = scratch;
   hipError_t err;
-  // Shift vectors live after the partials.
+  // Shift vectors live after the partials (FP64, then FP32), then the
+  // shifted sums.
   double* shift_d = scratch + wgs * np * 256;
   float* shift = reinterpret_cast<float*>(shift_d + kMaxBands);
   const std::size_t run_stride = pixels > 64 * 64 ? div_up(pixels, 64) : 64;
@@ -1003,17 +1025,15 @@ hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t str
                        m.q_offset, m.q_scale, partial);
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
+  double* shifted_sum = shift_d + 2 * kMaxBands;
   ReduceArgs ra{partial, static_cast<int>(used), static_cast<std::size_t>(np) * 256, NB, bands, 0, 1,
-                0, 0, 0.0, m.count, m.sum, m.sum_outer};
+                0, 0, 0.0, m.count, shifted_sum, m.sum_outer};
   hipLaunchKernelGGL(moments_reduce_kernel, dim3(static_cast<unsigned>(np * 256 / kRedElems)),
                      dim3(kRedElems * kRedGroups), 0, stream, ra);
   if ((err = hipGetLastError()) != hipSuccess) return err;
   const int tri = bands * (bands + 1) / 2;
   hipLaunchKernelGGL(unshift_kernel, dim3(static_cast<unsigned>(div_up(tri, 256))), dim3(256), 0,
-                     stream, bands, shift_d, m.count, m.sum, m.sum_outer);
-  if ((err = hipGetLastError()) != hipSuccess) return err;
-  hipLaunchKernelGGL(unshift_sum_kernel, dim3(static_cast<unsigned>(div_up(bands, 256))), dim3(256),
-                     0, stream, bands, shift_d, m.count, m.sum);
+                     stream, bands, shift_d, m.count, shifted_sum, m.sum, m.sum_outer);
   return hipGetLastError();
 }
 
@@ -1078,9 +1098,9 @@ hipError_t launch_moments_f64(const float* cube, std::size_t pixels, int bands, 
 }  // namespace
 
 std::size_t moments_scratch_doubles(std::size_t pixels, int bands) {
-  // mixed form: partials, then a double and a float shift vector; plain
-  // form: partials of block pairs and the vector part
-  const std::size_t mixed = mom_wgs(pixels) * static_cast<std::size_t>(moment_pairs(bands)) * 256 + 2 * kMaxBands;
+  // mixed form: partials, then a double and a float shift vector and the
+  // shifted sums; plain form: partials of block pairs and the vector part
+  const std::size_t mixed = mom_wgs(pixels) * static_cast<std::size_t>(moment_pairs(bands)) * 256 + 3 * kMaxBands;
   const std::size_t plain = mom_wgs(pixels, kMaxMomWgs) * f64_partial_doubles(f64_split(bands).nbm);
   return mixed > plain ? mixed : plain;
 }
