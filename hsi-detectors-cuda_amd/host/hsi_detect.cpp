@@ -59,6 +59,11 @@ Execution
   --memory=copy                 frames are copied to HBM (the only mode on a
                                 discrete GPU; zerocopy is refused)      [copy]
   --precision=plain|mixed       RX/ACE arithmetic form                 [plain]
+  --prescreen=MODEL             RX/ACE through a trained learned pre-screen
+                                (hsi_mi355x.prescreen.Prescreen.save): one
+                                pass builds the statistics (mixed form) and
+                                screens every pixel; only pixels at or above
+                                the model's cut are scored, the rest score 0
   --streams=N                   frames in flight                           [3]
   --frames=N                    frames to process                        [300]
   --warmup=N                    unmeasured frames first                   [30]
@@ -82,7 +87,7 @@ Output
 const std::set<std::string> kFlags = {
     "source", "envi", "resident", "width", "height", "bands", "targets", "noise", "drift", "seed",
     "library", "detector", "threshold", "rx-threshold", "rx-false-alarm", "ace-threshold",
-    "cem-threshold", "nms", "max-detections", "loading", "variant", "memory", "precision", "streams",
+    "cem-threshold", "nms", "max-detections", "loading", "variant", "memory", "precision", "prescreen", "streams",
     "frames", "warmup", "prefill", "no-graph", "device", "csv", "print-detections", "dump-score",
     "dump-detections", "json", "quiet", "help", "h"};
 
@@ -215,6 +220,14 @@ int main(int argc, char** argv) {
       return 2;
     }
     opt.precision = prec == "mixed" ? HSI_MIXED : HSI_PLAIN;
+    if (args.has("prescreen")) {
+      if (args.str("prescreen", "").empty()) {
+        std::fputs("--prescreen needs a model file\n", stderr);
+        return 2;
+      }
+      opt.use_prescreen = true;
+      opt.prescreen = hsi::read_prescreen(args.str("prescreen", ""));
+    }
     opt.variant = variant;
     opt.loading = args.real("loading", 1e-6);
     opt.detection.threshold = static_cast<float>(args.real("threshold", 0.10));

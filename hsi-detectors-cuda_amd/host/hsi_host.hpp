@@ -111,6 +111,18 @@ void write_envi(const std::string& hdr_path, const HsiFrame& frame);
 SpectralLibrary read_spectra_csv(const std::string& path);
 void write_spectra_csv(const std::string& path, const SpectralLibrary& library);
 
+// A trained learned pre-screen for RX/ACE (no counterpart in the reference):
+// the network's packed FP32 weights for `bands` inputs (hsi_mlp_param_count)
+// and the logit cut at or above which pixels are rescored. File layout,
+// little-endian: "HSIMLP01", int32 bands, float32 cut, uint32 count,
+// float32 weights[count] (hsi_mi355x.prescreen.Prescreen.save writes it).
+struct PrescreenModel {
+  int bands = 0;
+  float cut = 0.f;
+  std::vector<float> weights;
+};
+PrescreenModel read_prescreen(const std::string& path);
+
 // ---- frame sources (frame_source.hpp) ------------------------------------------
 
 struct FrameMeta {
@@ -177,6 +189,12 @@ struct PipelineOptions {
   SamVariant variant = SamVariant::Optimized;
   DetectionParams detection;            // SAM threshold, NMS, capacity; polarity is per detector
   DetectorThresholds thresholds;        // RX, ACE, CEM thresholds
+  // RX/ACE through the learned pre-screen: one pass accumulates the
+  // (mixed-form) statistics and screens every pixel with `prescreen`, and
+  // only the pixels at or above its cut are scored, in `precision`; every
+  // other pixel scores 0. At most 127 bands.
+  bool use_prescreen = false;
+  PrescreenModel prescreen;
   double loading = 1e-6;                // diagonal loading for the statistics
   // Frames in flight; each slot owns a stream, pinned staging, a device cube,
   // score maps, a detection list and a context.

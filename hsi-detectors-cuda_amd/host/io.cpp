@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -404,6 +405,36 @@ void write_spectra_csv(const std::string& path, const SpectralLibrary& library) 
     for (float v : s.values) out << "," << v;
     out << "\n";
   }
+}
+
+PrescreenModel read_prescreen(const std::string& path) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) throw std::runtime_error("cannot open pre-screen model: " + path);
+  char magic[8] = {};
+  in.read(magic, sizeof magic);
+  if (!in || std::memcmp(magic, "HSIMLP01", 8) != 0)
+    throw std::runtime_error("not a pre-screen model (bad magic): " + path);
+  std::int32_t bands = 0;
+  float cut = 0.f;
+  std::uint32_t count = 0;
+  in.read(reinterpret_cast<char*>(&bands), sizeof bands);
+  in.read(reinterpret_cast<char*>(&cut), sizeof cut);
+  in.read(reinterpret_cast<char*>(&count), sizeof count);
+  if (!in) throw std::runtime_error("truncated pre-screen model: " + path);
+  const std::size_t want = hsi_mlp_param_count(bands);
+  if (want == 0 || count != want)
+    throw std::runtime_error("pre-screen model " + path + ": " + std::to_string(count) + " weights for " +
+                             std::to_string(bands) + " bands (expected " + std::to_string(want) + ")");
+  if (!std::isfinite(cut)) throw std::runtime_error("pre-screen model " + path + ": the cut is not finite");
+  PrescreenModel m;
+  m.bands = bands;
+  m.cut = cut;
+  m.weights.resize(count);
+  in.read(reinterpret_cast<char*>(m.weights.data()), static_cast<std::streamsize>(count * sizeof(float)));
+  if (!in) throw std::runtime_error("truncated pre-screen model: " + path);
+  for (float v : m.weights)
+    if (!std::isfinite(v)) throw std::runtime_error("pre-screen model " + path + " holds a non-finite weight");
+  return m;
 }
 
 }  // namespace hsi

@@ -66,6 +66,9 @@ struct Slot {
   hsi_detection* d_dets = nullptr;   // [detectors][capacity]
   std::uint32_t* d_counts = nullptr;  // [detectors]
   hsi_detection* h_dets = nullptr;    // pinned mirror
+  hsi_detection* d_cand = nullptr;    // pre-screen candidates [screen_cap_]
+  std::uint32_t* d_cand_count = nullptr;
+  float* d_spectra = nullptr;         // [screen_cap_][screen_pitch_]
   std::uint32_t* h_counts = nullptr;  // pinned: [detectors] counts + 2 status words
   float* h_scores = nullptr;          // pinned [detectors][pixels] if score maps are returned
   hipEvent_t ev[kEvents] = {};
@@ -102,6 +105,18 @@ class HipPipeline final : public Pipeline {
     lib_norms_ = library.norms();
     lib_targets_ = library.size();
     half_ = opt_.variant == SamVariant::Half;
+    if (opt_.use_prescreen) {
+      bool rxace = false;
+      for (Detector d : opt_.detectors) rxace |= d == Detector::Rx || d == Detector::Ace;
+      if (!rxace) throw std::invalid_argument("the learned pre-screen screens RX and ACE: select one of them");
+      if (opt_.prescreen.bands != shape_.bands)
+        throw std::invalid_argument("pre-screen model is for " + std::to_string(opt_.prescreen.bands) +
+                                    " bands but frames have " + std::to_string(shape_.bands));
+      if (shape_.bands > 127) throw std::invalid_argument("the learned pre-screen supports at most 127 bands");
+      // A quarter of the frame as candidates (split evenly over the screening waves).
+      screen_cap_ = std::max<std::size_t>(shape_.pixels() / 4, 4096);
+      screen_pitch_ = static_cast<std::size_t>((shape_.bands + 31) / 32 * 32);
+    }
     for (Detector d : opt_.detectors) thresholds_.push_back(detection_threshold(opt_, d, shape_.bands));
     stride_ = hsi_plane_stride(shape_.height, shape_.width);
     cap_ = std::max(opt_.detection.max_detections, 1);
@@ -137,6 +152,9 @@ class HipPipeline final : public Pipeline {
       if (s.d_half) (void)hipFree(s.d_half);
       (void)hipFree(s.d_dets);
       (void)hipFree(s.d_counts);
+      if (s.d_cand) (void)hipFree(s.d_cand);
+      if (s.d_cand_count) (void)hipFree(s.d_cand_count);
+      if (s.d_spectra) (void)hipFree(s.d_spectra);
       (void)hipHostFree(s.staging);
       (void)hipHostFree(s.h_dets);
       (void)hipHostFree(s.h_counts);
@@ -226,6 +244,11 @@ class HipPipeline final : public Pipeline {
       std::snprintf(buf, sizeof buf, " %s=%.6g", to_string(opt_.detectors[i]), static_cast<double>(thresholds_[i]));
       d += buf;
     }
+    if (opt_.use_prescreen) {
+      char buf[64];
+      std::snprintf(buf, sizeof buf, ", learned pre-screen for RX/ACE (cut %.6g)", static_cast<double>(opt_.prescreen.cut));
+      d += buf;
+    }
     if (opt_.prefill) d += ", prefilled ring";
     d += "; source: " + source_->describe();
     return d;
@@ -254,6 +277,13 @@ class HipPipeline final : public Pipeline {
       hip_check(hipMalloc(reinterpret_cast<void**>(&tg), P * sizeof(std::int32_t)), "target map");
       s.d_score.push_back(sc);
       s.d_target.push_back(tg);
+    }
+    if (opt_.use_prescreen) {
+      hsi_check(s.ctx, hsi_mlp_set_weights(s.ctx, shape_.bands, opt_.prescreen.weights.data()), "install pre-screen");
+      hip_check(hipMalloc(reinterpret_cast<void**>(&s.d_cand), screen_cap_ * sizeof(hsi_detection)), "candidates");
+      hip_check(hipMalloc(reinterpret_cast<void**>(&s.d_cand_count), sizeof(std::uint32_t)), "candidate count");
+      hip_check(hipMalloc(reinterpret_cast<void**>(&s.d_spectra), screen_cap_ * screen_pitch_ * sizeof(float)),
+                "candidate spectra");
     }
     hip_check(hipMalloc(reinterpret_cast<void**>(&s.d_dets), nd * cap_ * sizeof(hsi_detection)), "detections");
     hip_check(hipMalloc(reinterpret_cast<void**>(&s.d_counts), nd * sizeof(std::uint32_t)), "counts");
@@ -304,8 +334,17 @@ class HipPipeline final : public Pipeline {
     const void* cube = half_ ? s.d_half : static_cast<const void*>(s.d_cube);
     const hsi_dtype dtype = half_ ? HSI_F16 : HSI_F32;
     if (events) hip_check(hipEventRecord(s.ev[kUploaded], s.stream), "event");
+    const bool screen = opt_.use_prescreen;
     if (need_stats_) {
-      hsi_check(s.ctx, hsi_stats_accumulate(s.ctx, s.d_cube, HSI_F32, H, W, B, stride_, opt_.precision, nullptr, st), "moments");
+      if (screen)
+        hsi_check(s.ctx,
+                  hsi_stats_accumulate_screen(s.ctx, s.d_cube, H, W, B, stride_, nullptr, opt_.prescreen.cut, s.d_cand,
+                                              s.d_cand_count, static_cast<int>(screen_cap_), s.d_spectra,
+                                              screen_pitch_, st),
+                  "moments and screening");
+      else
+        hsi_check(s.ctx, hsi_stats_accumulate(s.ctx, s.d_cube, HSI_F32, H, W, B, stride_, opt_.precision, nullptr, st),
+                  "moments");
       hsi_check(s.ctx, hsi_stats_build(s.ctx, opt_.loading, st), "statistics");
       hsi_check(s.ctx, hsi_stats_status_copy(s.ctx, s.h_counts + nd, st), "statistics status");
     }
@@ -318,7 +357,16 @@ class HipPipeline final : public Pipeline {
                                   i_sam >= 0 ? s.d_target[i_sam] : nullptr, i_cem >= 0 ? s.d_score[i_cem] : nullptr,
                                   i_cem >= 0 ? s.d_target[i_cem] : nullptr, st),
                 "score SAM/CEM");
-    if (i_ace >= 0)
+    if (screen && (i_ace >= 0 || i_rx >= 0)) {
+      const hsi_detector det = i_ace >= 0 ? HSI_DET_ACE : HSI_DET_RX;
+      float* score = det == HSI_DET_ACE ? s.d_score[i_ace] : s.d_score[i_rx];
+      hsi_check(s.ctx,
+                hsi_score_candidate_spectra(s.ctx, det, opt_.precision, s.d_cube, H, W, B, stride_, s.d_cand,
+                                            s.d_cand_count, static_cast<int>(screen_cap_), s.d_spectra,
+                                            screen_pitch_, score, det == HSI_DET_ACE ? s.d_target[i_ace] : nullptr,
+                                            det == HSI_DET_ACE && i_rx >= 0 ? s.d_score[i_rx] : nullptr, st),
+                "rescore candidates");
+    } else if (i_ace >= 0)
       hsi_check(s.ctx,
                 hsi_score(s.ctx, HSI_DET_ACE, opt_.precision, cube, dtype, H, W, B, stride_, s.d_score[i_ace],
                           s.d_target[i_ace], i_rx >= 0 ? s.d_score[i_rx] : nullptr, st),
@@ -434,6 +482,8 @@ class HipPipeline final : public Pipeline {
   std::size_t cap_ = 1;
   bool need_stats_ = false;
   bool half_ = false;
+  std::size_t screen_cap_ = 0;     // pre-screen candidates per frame
+  std::size_t screen_pitch_ = 0;   // floats per candidate spectrum
   std::vector<float> thresholds_;  // per detector, in PipelineOptions::detectors order
   std::vector<float> lib_values_, lib_norms_;
   int lib_targets_ = 0;

@@ -150,6 +150,37 @@ class Prescreen:
         self.cut = float(np.quantile(lg[Y > 0.5], quantile, method="lower") - margin)
         return self.cut
 
+    # -- model file (read by hsi_detect --prescreen) ---------------------------------
+    MAGIC = b"HSIMLP01"
+
+    def save(self, path):
+        """Write the trained network and its cut: "HSIMLP01", int32 bands,
+        float32 cut, uint32 count, float32 weights[count] (little-endian)."""
+        if self.cut is None:
+            raise ValueError("calibrate the cut before saving")
+        w = np.ascontiguousarray(self.ctx.mlp_get_weights(), dtype="<f4")
+        with open(path, "wb") as f:
+            f.write(self.MAGIC)
+            f.write(np.array([self.bands], "<i4").tobytes())
+            f.write(np.array([self.cut], "<f4").tobytes())
+            f.write(np.array([w.size], "<u4").tobytes())
+            f.write(w.tobytes())
+
+    def load(self, path):
+        """Install a network and cut written by save()."""
+        with open(path, "rb") as f:
+            data = f.read()
+        if data[:8] != self.MAGIC:
+            raise ValueError(f"{path}: not a pre-screen model")
+        bands = int(np.frombuffer(data, "<i4", 1, 8)[0])
+        cut = float(np.frombuffer(data, "<f4", 1, 12)[0])
+        n = int(np.frombuffer(data, "<u4", 1, 16)[0])
+        if bands != self.bands:
+            raise ValueError(f"{path}: a network for {bands} bands, not {self.bands}")
+        w = np.frombuffer(data, "<f4", n, 20).astype(np.float32)
+        self.ctx.mlp_set_weights(bands, w)
+        self.cut = cut
+
     # -- run time --------------------------------------------------------------------
     def screen(self, cube, h, w):
         """Learned path (statistics must be built): one screening pass

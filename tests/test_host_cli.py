@@ -338,3 +338,111 @@ def test_hsi_detect_lists_match_the_oracle_per_detector(tmp_path, oracle_lib, va
     for (k, det), (ng, nw, nd) in sorted(report.items()):
         print(f"frame {k} {det}: {ng} device / {nw} oracle, {nd} differ at near-threshold or near-tie pixels")
         assert nw > 0
+
+
+# ---- learned pre-screen from the CLI ----------------------------------------------
+
+def write_model(path, bands, cut, weights, magic=b"HSIMLP01"):
+    with open(path, "wb") as f:
+        f.write(magic)
+        f.write(np.array([bands], "<i4").tobytes())
+        f.write(np.array([cut], "<f4").tobytes())
+        f.write(np.array([len(weights)], "<u4").tobytes())
+        f.write(np.asarray(weights, "<f4").tobytes())
+
+
+def mlp_count(bands):
+    return 64 * bands + 64 + 32 * 64 + 32 + 32 + 1
+
+
+def test_hsi_detect_checks_the_prescreen_model_before_the_device(tmp_path):
+    """A missing, foreign, truncated or mis-sized model file, or one whose
+    cut or weights are not finite, is reported with exit 1 before any frame
+    is produced; --prescreen with no file is a usage error."""
+    base = ["hsi_detect", "--detector=rx", "--width=16", "--height=16", "--bands=16"]
+    r = run(base + ["--prescreen="])
+    assert r.returncode == 2 and "needs a model file" in r.stderr
+    r = run(base + [f"--prescreen={tmp_path / 'none.bin'}"])
+    assert r.returncode == 1 and "cannot open pre-screen model" in r.stderr
+    foreign = tmp_path / "foreign.bin"
+    write_model(foreign, 16, 0.0, np.zeros(mlp_count(16)), magic=b"NOTAMODL")
+    r = run(base + [f"--prescreen={foreign}"])
+    assert r.returncode == 1 and "bad magic" in r.stderr
+    sized = tmp_path / "sized.bin"
+    write_model(sized, 16, 0.0, np.zeros(mlp_count(16) - 1))
+    r = run(base + [f"--prescreen={sized}"])
+    assert r.returncode == 1 and f"(expected {mlp_count(16)})" in r.stderr
+    short = tmp_path / "short.bin"
+    write_model(short, 16, 0.0, np.zeros(mlp_count(16)))
+    short.write_bytes(short.read_bytes()[:-8])
+    r = run(base + [f"--prescreen={short}"])
+    assert r.returncode == 1 and "truncated" in r.stderr
+    nan_cut = tmp_path / "nan.bin"
+    write_model(nan_cut, 16, np.nan, np.zeros(mlp_count(16)))
+    r = run(base + [f"--prescreen={nan_cut}"])
+    assert r.returncode == 1 and "cut is not finite" in r.stderr
+    w = np.zeros(mlp_count(16))
+    w[3] = np.inf
+    bad_w = tmp_path / "badw.bin"
+    write_model(bad_w, 16, 0.0, w)
+    r = run(base + [f"--prescreen={bad_w}"])
+    assert r.returncode == 1 and "non-finite weight" in r.stderr
+
+
+@pytest.mark.gpu
+def test_hsi_detect_prescreen_lists_equal_the_python_learned_path(tmp_path):
+    """A network trained from Python (hsi_mi355x.prescreen) and saved, run by
+    hsi_detect --prescreen --detector=rx,ace --precision=mixed over four
+    seeded frames (two eager, two from captured graphs): each frame's RX and
+    ACE lists equal the Python learned path's (the same one-pass statistics
+    and screening, the build, precision-form rescoring of the candidate rows,
+    detect) element for element; against the plain full-frame lists of the
+    same frames the recall is printed and must reach 0.99."""
+    import torch
+    import hsi_mi355x as H
+    from hsi_mi355x.prescreen import Prescreen
+    h, w, b, t, seed, rate = 192, 256, 113, 8, 91, 1e-6
+    cfg = H.synth_config(h, w, b, t, seed=seed)
+    lib = H.synth_library(cfg)
+    ctx = H.Context(0)
+    ctx.set_library(lib)
+    ps = Prescreen(ctx, b, rx_false_alarm_rate=rate, seed=4, capacity=max(h * w // 4, 4096), precision=H.MIXED)
+    for k in (40, 41, 42, 43):
+        ps.add_training_frame(torch.from_numpy(H.synth_frame(cfg, k)).cuda(), h, w)
+    ps.train(epochs=40)
+    ps.calibrate()
+    model = str(tmp_path / "net.bin")
+    ps.save(model)
+    common = ["hsi_detect", f"--width={w}", f"--height={h}", f"--bands={b}", f"--targets={t}", f"--seed={seed}",
+              "--frames=4", "--warmup=0", "--streams=2", "--detector=rx,ace", f"--rx-false-alarm={rate}",
+              "--ace-threshold=0.5", "--max-detections=20000", "--quiet"]
+    dump_l, dump_p = str(tmp_path / "learned.csv"), str(tmp_path / "plain.csv")
+    r = run(common + ["--precision=mixed", f"--prescreen={model}", f"--dump-detections={dump_l}"], timeout=180)
+    assert r.returncode == 0, r.stderr
+    r = run(common + [f"--dump-detections={dump_p}"], timeout=180)
+    assert r.returncode == 0, r.stderr
+    got, plain = read_dumped_lists(dump_l), read_dumped_lists(dump_p)
+    # the Python learned path, frame by frame
+    again = Prescreen(ctx, b, rx_false_alarm_rate=rate, capacity=max(h * w // 4, 4096), precision=H.MIXED)
+    again.load(model)
+    thr = {"rx": float(np.float32(H.rx_threshold(b, rate))), "ace": 0.5}
+    dets = torch.empty((20000, 4), dtype=torch.int32, device="cuda")
+    nd = torch.empty(1, dtype=torch.int32, device="cuda")
+    for k in range(4):
+        cube = torch.from_numpy(H.synth_frame(cfg, k)).cuda()
+        m = again.screen_fused(cube, h, w)
+        for det, score, target in (("rx", m["rx"], None), ("ace", m["ace"], m["target"])):
+            ctx.detect(score, target, h, w, H.DetectionParams(thr[det], 2, 20000, True), dets, nd)
+            torch.cuda.synchronize()
+            n = int(nd.item())
+            want = [tuple(int(v) for v in row[:3]) for row in dets[:n].cpu().numpy()]
+            have = [(x, y, tg if det == "ace" else -1) for (x, y, tg, _) in got.get((k, det), [])]
+            if det == "rx":
+                want = [(x, y, -1) for (x, y, _) in want]
+            assert have == want, (k, det, len(have), len(want))
+            ref = {(x, y) for (x, y, _, _) in plain.get((k, det), [])}
+            hit = len(ref & {(x, y) for (x, y, _) in have})
+            print(f"frame {k} {det}: learned {len(have)}, plain {len(ref)}, recall {hit / max(len(ref), 1):.4f}",
+                  flush=True)
+            assert len(ref) == 0 or hit / len(ref) >= 0.99
+    ctx.close()
