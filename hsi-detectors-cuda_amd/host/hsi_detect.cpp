@@ -64,6 +64,14 @@ Execution
                                 pass builds the statistics (mixed form) and
                                 screens every pixel; only pixels at or above
                                 the model's cut are scored, the rest score 0
+  --batch                       score the scenes together instead of as a
+                                stream: each --envi cube (any size; bands
+                                matching the library for SAM/ACE/CEM), or the
+                                first --frames synthetic frames [4], uploaded
+                                once, their statistics and scoring run as one
+                                batch replayed from a captured hipGraph; lists
+                                are dumped with frame = scene index
+  --autotune                    with --batch: time the launch shapes first
   --streams=N                   frames in flight                           [3]
   --frames=N                    frames to process                        [300]
   --warmup=N                    unmeasured frames first                   [30]
@@ -87,7 +95,8 @@ Output
 const std::set<std::string> kFlags = {
     "source", "envi", "resident", "width", "height", "bands", "targets", "noise", "drift", "seed",
     "library", "detector", "threshold", "rx-threshold", "rx-false-alarm", "ace-threshold",
-    "cem-threshold", "nms", "max-detections", "loading", "variant", "memory", "precision", "prescreen", "streams",
+    "cem-threshold", "nms", "max-detections", "loading", "variant", "memory", "precision", "prescreen", "batch",
+    "autotune", "streams",
     "frames", "warmup", "prefill", "no-graph", "device", "csv", "print-detections", "dump-score",
     "dump-detections", "json", "quiet", "help", "h"};
 
@@ -143,6 +152,81 @@ std::vector<std::string> commas(const std::string& s) {
   return out;
 }
 
+// --batch: the scenes scored together (hsi::run_scene_batch).
+int run_batch(const Args& args, const hsi::PipelineOptions& opt, const std::string& kind,
+              const hsi_synth_config& synth, const hsi::SpectralLibrary& library, bool quiet) {
+  for (const char* flag : {"prescreen", "prefill"})
+    if (args.has(flag)) {
+      std::fprintf(stderr, "--%s does not apply to --batch\n", flag);
+      return 2;
+    }
+  if (opt.precision != HSI_PLAIN || opt.variant == hsi::SamVariant::Half) {
+    std::fputs("--batch scores in plain precision on FP32 cubes: drop --precision=mixed and --variant=half\n",
+               stderr);
+    return 2;
+  }
+  std::vector<hsi::HsiFrame> scenes;
+  if (kind == "envi") {
+    for (const std::string& path : commas(args.str("envi", ""))) scenes.push_back(hsi::read_envi(path));
+  } else {
+    const long long n = args.integer("frames", 4);
+    if (n < 1 || n > 256) {
+      std::fputs("--batch takes 1..256 synthetic frames\n", stderr);
+      return 2;
+    }
+    for (long long k = 0; k < n; ++k) {
+      hsi::HsiFrame f;
+      f.shape = hsi::CubeShape{synth.height, synth.width, synth.bands};
+      f.data.resize(f.shape.elements());
+      if (hsi_synth_frame(&synth, static_cast<std::uint64_t>(k), f.data.data(), f.shape.pixels(), nullptr, 0) != 0)
+        throw std::runtime_error("synthetic frame generation failed");
+      scenes.push_back(std::move(f));
+    }
+  }
+  const hsi::BatchRun run = hsi::run_scene_batch(scenes, library, opt, args.has("autotune"));
+  double pixels = 0;
+  unsigned long long total = 0;
+  bool stats_ok = true;
+  for (const hsi::BatchSceneResult& r : run.scenes) {
+    pixels += static_cast<double>(r.shape.pixels());
+    stats_ok &= r.statistics_ok;
+    for (const auto& dr : r.results) total += dr.detections_found;
+  }
+  if (!quiet) {
+    std::printf("batch of %zu scenes, %.0f pixels: %.3f ms (statistics + scoring%s), %.1f Mpix/s\n",
+                run.scenes.size(), pixels, static_cast<double>(run.ms_batch), opt.use_graph ? ", graph replay" : "",
+                pixels / (static_cast<double>(run.ms_batch) * 1e3));
+    if (!run.autotune_report.empty()) std::printf("%s", run.autotune_report.c_str());
+    for (std::size_t s = 0; s < run.scenes.size(); ++s) {
+      const hsi::BatchSceneResult& r = run.scenes[s];
+      std::printf("scene %zu %dx%dx%d%s:", s, r.shape.height, r.shape.width, r.shape.bands,
+                  r.statistics_ok ? "" : " (statistics failed)");
+      for (const auto& dr : r.results) std::printf(" %s %u", hsi::to_string(dr.detector), dr.detections_found);
+      std::printf("\n");
+    }
+  }
+  if (args.has("dump-detections")) {
+    std::ofstream out(args.str("dump-detections", ""));
+    if (!out) throw std::runtime_error("cannot write " + args.str("dump-detections", ""));
+    out << "frame,detector,x,y,target,score\n";
+    char line[160];
+    for (std::size_t s = 0; s < run.scenes.size(); ++s)
+      for (const auto& dr : run.scenes[s].results)
+        for (const hsi::Detection& d : dr.detections) {
+          std::snprintf(line, sizeof line, "%zu,%s,%d,%d,%d,%.9g\n", s, hsi::to_string(dr.detector), d.x, d.y,
+                        d.target, static_cast<double>(d.score));
+          out << line;
+        }
+  }
+  if (args.has("json"))
+    std::printf("{\"scenes\": %zu, \"pixels\": %.0f, \"batch_ms\": %.4f, \"mpix_s\": %.3f, \"graph\": %s, "
+                "\"detections\": %llu, \"statistics_ok\": %s}\n",
+                run.scenes.size(), pixels, static_cast<double>(run.ms_batch),
+                pixels / (static_cast<double>(run.ms_batch) * 1e3), opt.use_graph ? "true" : "false", total,
+                stats_ok ? "true" : "false");
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -177,6 +261,7 @@ int main(int argc, char** argv) {
     // ---- source -----------------------------------------------------------
     std::unique_ptr<hsi::FrameSource> source;
     const std::string kind = args.str("source", "synthetic");
+    hsi_synth_config synth{};
     if (kind == "synthetic") {
       hsi::SyntheticOptions so;
       hsi_synth_default(&so.config);
@@ -188,14 +273,18 @@ int main(int argc, char** argv) {
       so.config.seed = static_cast<std::uint64_t>(args.integer("seed", 20261203));
       so.config.drift = args.real("drift", 0.0);
       so.frames = 0;
+      synth = so.config;
       source = hsi::make_synthetic_source(so);
     } else if (kind == "envi") {
       if (!args.has("envi")) {
         std::fputs("--source=envi needs --envi=<header>[,<header>...]\n", stderr);
         return 2;
       }
-      source = hsi::make_envi_replay_source(commas(args.str("envi", "")), true,
-                                            static_cast<int>(args.integer("resident", 4)));
+      // A batch reads each cube whole (scenes may differ in shape); a stream
+      // replays cubes of one shape.
+      if (!args.has("batch"))
+        source = hsi::make_envi_replay_source(commas(args.str("envi", "")), true,
+                                              static_cast<int>(args.integer("resident", 4)));
     } else if (kind == "hot") {
       std::fputs("--source=hot (folders of snapshot-mosaic PNG frames) is not part of this build; "
                  "decode the frames and demosaic them on the device with hsi_demosaic\n", stderr);
@@ -207,7 +296,7 @@ int main(int argc, char** argv) {
     // ---- library ----------------------------------------------------------
     hsi::SpectralLibrary library;
     if (args.has("library")) library = hsi::read_spectra_csv(args.str("library", ""));
-    else if (source->library()) library = *source->library();
+    else if (source && source->library()) library = *source->library();
     // ---- pipeline ---------------------------------------------------------
     hsi::PipelineOptions opt;
     if (!hsi::parse_detector_list(args.str("detector", "sam"), &opt.detectors)) {
@@ -242,6 +331,12 @@ int main(int argc, char** argv) {
     opt.use_graph = !args.has("no-graph");
     opt.device = static_cast<int>(args.integer("device", 0));
     opt.return_score_map = args.has("dump-score");
+    if (args.has("batch"))
+      return run_batch(args, opt, kind, synth, library, quiet);
+    if (args.has("autotune")) {
+      std::fputs("--autotune applies to --batch\n", stderr);
+      return 2;
+    }
     std::string error;
     auto pipeline = hsi::Pipeline::create(source.get(), library, opt, &error);
     if (!pipeline) {

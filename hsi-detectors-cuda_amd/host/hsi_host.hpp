@@ -248,6 +248,27 @@ struct PipelineStats {
 // (0, 1).
 float detection_threshold(const PipelineOptions& options, Detector detector, int bands);
 
+// ---- scenes of any shape in one batch (hsi_detect --batch) -------------------
+
+struct BatchSceneResult {
+  CubeShape shape;
+  bool statistics_ok = true;            // false: this scene's factorisation failed
+  std::vector<DetectorResult> results;  // one per PipelineOptions::detectors
+};
+struct BatchRun {
+  std::vector<BatchSceneResult> scenes;
+  float ms_batch = 0;                   // statistics + scoring of every scene, device time
+  std::string autotune_report;
+};
+// Uploads every scene, runs their statistics and scoring as one hsi_batch
+// (replayed from a captured graph when options.use_graph; shapes tuned first
+// when autotune), then compacts each scene's maps with each detector's own
+// threshold. Uses options' detectors, thresholds, detection parameters,
+// loading, use_graph and device; plain precision. SAM, ACE and CEM need
+// library.bands() == every scene's band count. Throws on error.
+BatchRun run_scene_batch(const std::vector<HsiFrame>& scenes, const SpectralLibrary& library,
+                         const PipelineOptions& options, bool autotune);
+
 // Frame-rate detector over a FrameSource. Slots run ahead of each other and
 // results come back in frame order.
 class Pipeline {

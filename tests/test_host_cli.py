@@ -118,6 +118,19 @@ def test_score_reports_file_errors_before_touching_the_device(tmp_path):
     assert r.returncode == 1
 
 
+@pytest.mark.parametrize("args,needle", [
+    (["hsi_detect", "--batch", "--precision=mixed"], "plain precision"),
+    (["hsi_detect", "--batch", "--variant=half"], "plain precision"),
+    (["hsi_detect", "--batch", "--prefill"], "--prefill does not apply to --batch"),
+    (["hsi_detect", "--batch", "--frames=1000"], "1..256 synthetic frames"),
+    (["hsi_detect", "--autotune"], "--autotune applies to --batch"),
+])
+def test_batch_mode_refuses_what_it_does_not_do(args, needle):
+    r = run(args)
+    assert r.returncode == 2, (r.returncode, r.stderr)
+    assert needle in r.stderr
+
+
 def test_detect_rejects_bad_options():
     r = run(["hsi_detect", "--detector=sam,foo"])
     assert r.returncode == 2 and "--detector" in r.stderr
@@ -446,3 +459,54 @@ def test_hsi_detect_prescreen_lists_equal_the_python_learned_path(tmp_path):
                   flush=True)
             assert len(ref) == 0 or hit / len(ref) >= 0.99
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_hsi_detect_batch_lists_equal_the_stream_per_scene(tmp_path):
+    """hsi_detect --batch over three ENVI scenes of different sizes (SAM, RX,
+    ACE, CEM; with --autotune, graph replay): each scene's lists equal those
+    hsi_detect gives the same cube as a one-frame stream, element for
+    element; an RX-only batch mixes band counts (16 and 113)."""
+    import hsi_mi355x as H
+    shapes = [(40, 56), (64, 32), (24, 90)]
+    cfg = H.synth_config(64, 96, 113, 6, seed=13)
+    lib = H.synth_library(cfg)
+    libcsv = str(tmp_path / "lib.csv")
+    write_csv(libcsv, lib)
+    hdrs = []
+    for i, (hh, ww) in enumerate(shapes):
+        full = H.synth_frame(cfg, i, stride=64 * 96).reshape(113, 64, 96)
+        cube = np.ascontiguousarray(full[:, :hh, :ww])
+        hdr = str(tmp_path / f"s{i}.hdr")
+        write_envi(hdr, cube)
+        hdrs.append(hdr)
+    common = [f"--library={libcsv}", "--detector=sam,rx,ace,cem", "--threshold=0.1", "--rx-false-alarm=1e-3",
+              "--ace-threshold=0.3", "--max-detections=20000", "--quiet"]
+    dump_b = str(tmp_path / "batch.csv")
+    r = run(["hsi_detect", "--source=envi", f"--envi={','.join(hdrs)}", "--batch", "--autotune", "--json",
+             f"--dump-detections={dump_b}"] + common, timeout=180)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    assert summary["scenes"] == 3 and summary["statistics_ok"] and summary["graph"]
+    got = read_dumped_lists(dump_b)
+    for i, hdr in enumerate(hdrs):
+        dump_s = str(tmp_path / f"stream{i}.csv")
+        r = run(["hsi_detect", "--source=envi", f"--envi={hdr}", "--frames=1", "--warmup=0", "--streams=1",
+                 f"--dump-detections={dump_s}"] + common, timeout=180)
+        assert r.returncode == 0, r.stderr
+        want = read_dumped_lists(dump_s)
+        for det in ("sam", "rx", "ace", "cem"):
+            g, wnt = got.get((i, det), []), want.get((0, det), [])
+            print(f"scene {i} {det}: batch {len(g)}, stream {len(wnt)}", flush=True)
+            assert g == wnt, (i, det)
+            assert len(wnt) > 0 or det == "sam"
+    # RX only: scenes of 16 and 113 bands in one batch, no library
+    c16 = H.synth_frame(H.synth_config(30, 50, 16, 2, seed=5), 0, stride=30 * 50).reshape(16, 30, 50)
+    h16 = str(tmp_path / "b16.hdr")
+    write_envi(h16, np.ascontiguousarray(c16))
+    dump_m = str(tmp_path / "mixed_bands.csv")
+    r = run(["hsi_detect", "--source=envi", f"--envi={h16},{hdrs[0]}", "--batch", "--detector=rx",
+             "--rx-false-alarm=1e-2", "--quiet", f"--dump-detections={dump_m}"], timeout=180)
+    assert r.returncode == 0, r.stderr
+    mixed = read_dumped_lists(dump_m)
+    assert len(mixed.get((0, "rx"), [])) > 0 and len(mixed.get((1, "rx"), [])) > 0
