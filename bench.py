@@ -38,6 +38,7 @@ for _p in (REPO, PKG):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 FP64_PEAK_TFLOPS = 78.6
 FP16_MFMA_PEAK_TFLOPS = 2500.0
 
@@ -507,6 +508,14 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
           "stats_plain": time_on_stream(torch, lambda: (ctx.stats_accumulate(cube, h, w, b), ctx.stats_build(LOADING)), repeats),
           "stats_mixed": time_on_stream(torch, lambda: (ctx.stats_accumulate(cube, h, w, b, precision=H.MIXED), ctx.stats_build(LOADING)), repeats)}
     m = ps._buffers(h * w)
+    # the plain method at single precision: FP64 statistics, full-frame ACE (RX side output) in the FP32 form
+    ms["plain_score_fp32"] = time_on_stream(torch, lambda: ctx.score(H.Detector.ACE, cube, h, w, b, m["ace"], m["target"],
+                                                                     m["rx"], precision=H.FP32), repeats)
+    # the precision form without the pre-screen: mixed statistics, full-frame mixed ACE (RX side output)
+    ms["mixed_path"] = time_on_stream(torch, lambda: (ctx.stats_accumulate(cube, h, w, b, precision=H.MIXED),
+                                                     ctx.stats_build(LOADING),
+                                                     ctx.score(H.Detector.ACE, cube, h, w, b, m["ace"], m["target"],
+                                                               m["rx"], precision=H.MIXED)), repeats)
     logit = m["logit"]
     ms["infer"] = time_on_stream(torch, lambda: ctx.mlp_infer(cube, h, w, b, logit, center=True), repeats)
     # SAM and CEM alone on the same 2048x2048x113 frame (8 targets)
@@ -540,6 +549,8 @@ def learned_form(H, torch, repeats, size=2048, train_frames=3, eval_frames=3):
         "speedup_path_with_statistics": round((ms["stats_plain"] + ms["plain_score"]) / ms["learned_path_fused"], 3),
         "speedup_path_separate_passes": round((ms["stats_plain"] + ms["plain_score"]) /
                                               (ms["stats_mixed"] + ms["learned_score"]), 3),
+        "speedup_vs_mixed_without_prescreen": round(ms["mixed_path"] / ms["learned_path_fused"], 3),
+        "speedup_vs_plain_fp32_form": round((ms["stats_plain"] + ms["plain_score_fp32"]) / ms["learned_path_fused"], 3),
         "infer_hbm_gb_s": round(b * 4 * pix / (ms["infer"] / 1e3) / 1e9, 1),
         "sam_cem_2048": {k: {"us": round(ms[k] * 1e3, 1),
                              "hbm_gb_s": round((b * 4 + 8) * pix / (ms[k] / 1e3) / 1e9, 1),
@@ -764,6 +775,8 @@ def main():
         "ace": time_on_stream(torch, lambda: ctx.score(H.Detector.ACE, c0, H_, W_, B_, cem_s, cem_t, sam_s), r),
         "rx_mixed": time_on_stream(torch, lambda: ctx.score(H.Detector.RX, c0, H_, W_, B_, cem_s, precision=H.MIXED), r),
         "ace_mixed": time_on_stream(torch, lambda: ctx.score(H.Detector.ACE, c0, H_, W_, B_, cem_s, cem_t, sam_s, precision=H.MIXED), r),
+        "rx_fp32": time_on_stream(torch, lambda: ctx.score(H.Detector.RX, c0, H_, W_, B_, cem_s, precision=H.FP32), r),
+        "ace_fp32": time_on_stream(torch, lambda: ctx.score(H.Detector.ACE, c0, H_, W_, B_, cem_s, cem_t, sam_s, precision=H.FP32), r),
     }
     for k, v in ms.items():
         log(f"  {k:14s} {v * 1e3:9.1f} us")
@@ -785,6 +798,12 @@ def main():
         detectors[k] = {"mpix_s": round(pixels / sec / 1e6, 1), "us": round(ms[k] * 1e3, 1),
                         "hbm_gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                         "fp64_tflops": round(tfl, 2), "fp64_frac": round(tfl / FP64_PEAK_TFLOPS, 4)}
+    # the FP32 form of RX/ACE: the same flops on the FP32 matrix cores
+    for k, base in (("rx_fp32", "rx"), ("ace_fp32", "ace")):
+        sec = ms[k] / 1e3
+        tfl = flops_px[base] * pixels / sec / 1e12
+        detectors[k] = {"mpix_s": round(pixels / sec / 1e6, 1), "us": round(ms[k] * 1e3, 1),
+                        "fp32_tflops": round(tfl, 2), "fp32_frac": round(tfl / FP32_MFMA_PEAK_TFLOPS, 4)}
 
     # Dominant kernel of the step: moments (FP64 matrix cores) vs the fused
     # SAM+CEM stream (HBM).

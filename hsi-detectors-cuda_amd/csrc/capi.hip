@@ -750,13 +750,13 @@ hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precisio
                      float* d_score, int32_t* d_target, float* d_aux, hsi_stream stream) {
   if (!ctx) return HSI_ERR_INVALID_ARG;
   if (!d_score) return fail(ctx, HSI_ERR_INVALID_ARG, "null score map");
-  if (precision != HSI_PLAIN && precision != HSI_MIXED)
+  if (precision != HSI_PLAIN && precision != HSI_MIXED && precision != HSI_FP32)
     return fail(ctx, HSI_ERR_INVALID_ARG, "unknown precision");
   switch (detector) {
     case HSI_DET_SAM:
     case HSI_DET_CEM:
       if (precision != HSI_PLAIN)
-        return fail(ctx, HSI_ERR_INVALID_ARG, "the mixed form applies to RX and ACE only");
+        return fail(ctx, HSI_ERR_INVALID_ARG, "the mixed and FP32 forms apply to RX and ACE only");
       return detector == HSI_DET_SAM
                  ? hsi_score_sam_cem(ctx, d_cube, dtype, height, width, bands, plane_stride, d_score,
                                      d_target, nullptr, nullptr, stream)
@@ -784,7 +784,7 @@ hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precisio
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
   {
     const hipStream_t ls = static_cast<hipStream_t>(stream);
-    const hsi_status lst = precision == HSI_PLAIN ? ensure_layouts(ctx, ls) : ensure_mixed_layouts(ctx, ls);
+    const hsi_status lst = precision == HSI_MIXED ? ensure_mixed_layouts(ctx, ls) : ensure_layouts(ctx, ls);
     if (lst != HSI_OK) return lst;
   }
   RxAceArgs a;
@@ -813,7 +813,9 @@ hsi_status hsi_score(hsi_ctx* ctx, hsi_detector detector, hsi_precision precisio
     a.ace_target = d_target;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const hipError_t err = precision == HSI_PLAIN ? launch_rxace_plain(a, s) : launch_rxace_mixed(a, s);
+  const hipError_t err = precision == HSI_PLAIN  ? launch_rxace_plain(a, s)
+                         : precision == HSI_FP32 ? launch_rxace_fp32(a, s)
+                                                 : launch_rxace_mixed(a, s);
   HSI_HIP(ctx, err, "score RX/ACE");
   return HSI_OK;
 }

@@ -246,6 +246,9 @@ hipError_t launch_sam_cem_batch(int lib_stride, int ppt, const SamSceneDev* scen
 hipError_t launch_rxace_batch(int nb, int tt, const RxSceneDev* scenes, const RxWorkItem* work, int num_items,
                               int workgroups, hipStream_t stream);
 hipError_t launch_rxace_mixed(const RxAceArgs& args, hipStream_t stream);
+// The FP32 form of the plain kernel: whitening and ACE products on
+// v_mfma_f32_16x16x4f32 with FP32 accumulation (same layouts as plain).
+hipError_t launch_rxace_fp32(const RxAceArgs& args, hipStream_t stream);
 // The precision form over candidate spectra rows, with launch_rxace_candidates'
 // list semantics (FP32 rows; spectra_stride >= bands; the rows' bytes below
 // 2 GiB); an overflowed list scores the whole FP32 cube.

@@ -54,7 +54,12 @@ struct GatherDev {
   std::size_t spectra_stride = 0;
 };
 
-template <typename In, int NB, int TT, bool GATHER>
+// F32: the FP32 form - whitening and ACE's products on v_mfma_f32_16x16x4f32
+// (FP32 accumulation), whose accumulator holds rows 4g + r in lane group g,
+// register r (the FP64 MFMA interleaves them: rows g + 4r), so ACE's target
+// operand is read permuted and its target index is 16tt + 4g + r; ||z||^2
+// and the ratio stay FP64.
+template <typename In, int NB, int TT, bool GATHER, bool F32 = false>
 __device__ __forceinline__ void rxace_tiles(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride, int num_targets,
     const float* __restrict__ target_norm2, float* __restrict__ rx_out, float* __restrict__ ace_score,
@@ -104,7 +109,7 @@ __device__ __forceinline__ void rxace_tiles(
     for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int t = 16 * tt + g + 4 * r;
+        const int t = F32 ? 16 * tt + 4 * g + r : 16 * tt + g + 4 * r;
         uu[tt][r] = t < num_targets ? static_cast<double>(target_norm2[t]) : 0.0;
       }
   }
@@ -153,26 +158,32 @@ __device__ __forceinline__ void rxace_tiles(
     const std::size_t p = pixel_of(tile, live);
     if (tile + waves < tiles) load_tile(tile + waves, nxt, nxt_ok);
     // Z = W Y accumulated one input band block J at a time.
-    d4 z[NB];
+    typedef std::conditional_t<F32, f4, d4> Acc;
+    typedef std::conditional_t<F32, float, double> Real;
+    Acc z[NB];
 #pragma unroll
-    for (int I = 0; I < NB; ++I) z[I] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int I = 0; I < NB; ++I) z[I] = Acc{0, 0, 0, 0};
 #pragma unroll
     for (int J = 0; J < NB; ++J) {
-      double y[4];
+      Real y[4];
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         const int b = 16 * J + 4 * kk + g;
         // padded bands and pixels read 0; their whitener columns are 0 too
-        const double x = cur_ok ? static_cast<double>(static_cast<float>(cur[J][kk])) : 0.0;
-        y[kk] = (b < bands) ? x - static_cast<double>(mean_lds[b]) : 0.0;
+        const Real x = cur_ok ? static_cast<Real>(static_cast<float>(cur[J][kk])) : Real(0);
+        y[kk] = (b < bands) ? x - static_cast<Real>(mean_lds[b]) : Real(0);
       }
 #pragma unroll
       for (int I = J; I < NB; ++I) {
         const float* blk = w_lds + (I * (I + 1) / 2 + J) * 256;
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
-          z[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(blk[kk * 64 + lane]), y[kk],
-                                                      z[I], 0, 0, 0);
+        for (int kk = 0; kk < 4; ++kk) {
+          if constexpr (F32)
+            z[I] = __builtin_amdgcn_mfma_f32_16x16x4f32(blk[kk * 64 + lane], y[kk], z[I], 0, 0, 0);
+          else
+            z[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(blk[kk * 64 + lane]), y[kk],
+                                                        z[I], 0, 0, 0);
+        }
       }
       // Keep each band block's whitener loads next to their MFMAs.
       __builtin_amdgcn_sched_barrier(0);
@@ -184,20 +195,27 @@ __device__ __forceinline__ void rxace_tiles(
     cur_ok = nxt_ok;
 
     double zz = 0.0;
-    d4 zu[TT > 0 ? TT : 1];
+    Acc zu[TT > 0 ? TT : 1];
 #pragma unroll
-    for (int tt = 0; tt < (TT > 0 ? TT : 1); ++tt) zu[tt] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int tt = 0; tt < (TT > 0 ? TT : 1); ++tt) zu[tt] = Acc{0, 0, 0, 0};
 #pragma unroll
     for (int I = 0; I < NB; ++I) {
-      zz += z[I][0] * z[I][0] + z[I][1] * z[I][1] + z[I][2] * z[I][2] + z[I][3] * z[I][3];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) zz += static_cast<double>(z[I][r]) * static_cast<double>(z[I][r]);
       if constexpr (TT > 0) {
 #pragma unroll
         for (int tt = 0; tt < TT; ++tt) {
           const float* ub = u_lds + (tt * NB + I) * 256;
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
-            zu[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(ub[r * 64 + lane]),
-                                                          z[I][r], zu[tt], 0, 0, 0);
+          for (int r = 0; r < 4; ++r) {
+            // u_lds holds U[16tt + n][16I + g + 4r] at [r][16g + n]; K-step r
+            // of the FP32 form needs row 16I + 4g + r, found at [g][16r + n].
+            if constexpr (F32)
+              zu[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ub[g * 64 + 16 * r + n], z[I][r], zu[tt], 0, 0, 0);
+            else
+              zu[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(ub[r * 64 + lane]),
+                                                            z[I][r], zu[tt], 0, 0, 0);
+          }
         }
       }
     }
@@ -206,17 +224,19 @@ __device__ __forceinline__ void rxace_tiles(
     zz += __shfl_xor(zz, 32, 64);
 
     if constexpr (TT > 0) {
-      // Lane (g, n) holds z.u for targets 16*tt + g + 4*r of pixel n.
+      // Lane (g, n) holds z.u for targets 16*tt + g + 4*r of pixel n (FP32
+      // form: 16*tt + 4*g + r).
       double best = -1.0;
       int best_t = -1;
 #pragma unroll
       for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int t = 16 * tt + g + 4 * r;
+          const int t = F32 ? 16 * tt + 4 * g + r : 16 * tt + g + 4 * r;
           const double u2 = uu[tt][r];
           if (t < num_targets && u2 > 0.0 && zz > 0.0) {
-            const double s = (zu[tt][r] * zu[tt][r]) / (u2 * zz);
+            const double zu_t = static_cast<double>(zu[tt][r]);
+            const double s = (zu_t * zu_t) / (u2 * zz);
             if (s == s) ace_merge(best, best_t, s, t);
           }
         }
@@ -258,7 +278,7 @@ __device__ __forceinline__ void rxace_stage(const float* __restrict__ white_bloc
 // SIMD (the unconstrained allocation took 259 with accumulation registers).
 __host__ __device__ constexpr int plain_min_waves(int NB) { return NB <= 8 ? 2 : 1; }
 
-template <typename In, int NB, int TT, bool GATHER>
+template <typename In, int NB, int TT, bool GATHER, bool F32 = false>
 __global__ __launch_bounds__(kRxWaves * 64) __attribute__((amdgpu_waves_per_eu(plain_min_waves(NB)))) void rxace_plain_kernel(
     const In* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
     int num_targets, const float* __restrict__ mean, const float* __restrict__ white_blocks,
@@ -271,7 +291,7 @@ __global__ __launch_bounds__(kRxWaves * 64) __attribute__((amdgpu_waves_per_eu(p
   rxace_stage<NB, TT>(white_blocks, ace_u, mean, bands, w_lds, u_lds, mean_lds);
   __syncthreads();
   const std::size_t wave_id = blockIdx.x * static_cast<std::size_t>(kRxWaves) + (threadIdx.x >> 6);
-  rxace_tiles<In, NB, TT, GATHER>(cube, pixels, bands, stride, num_targets, target_norm2, rx_out, ace_score,
+  rxace_tiles<In, NB, TT, GATHER, F32>(cube, pixels, bands, stride, num_targets, target_norm2, rx_out, ace_score,
                                   ace_target, w_lds, u_lds, mean_lds, wave_id,
                                   static_cast<std::size_t>(gridDim.x) * kRxWaves, ~static_cast<std::size_t>(0),
                                   gd);
@@ -323,7 +343,7 @@ __global__ __launch_bounds__(kRxWaves * 64) void rxace_batch_kernel(const RxScen
 
 using Gather = GatherDev;
 
-template <typename In, int NB, int TT, bool GATHER>
+template <typename In, int NB, int TT, bool GATHER, bool F32 = false>
 hipError_t launch_plain_t(const RxAceArgs& a, const Gather& gth, hipStream_t stream) {
   // A candidate list's length is known only on the device: size the grid for
   // its capacity and let the kernel stop at the count.
@@ -335,24 +355,25 @@ hipError_t launch_plain_t(const RxAceArgs& a, const Gather& gth, hipStream_t str
   const std::size_t max_wgs = GATHER ? 256 * static_cast<std::size_t>(plain_min_waves(NB)) : 1024;
   if (wgs > max_wgs) wgs = max_wgs;
   if (wgs == 0) wgs = 1;
-  hipLaunchKernelGGL((rxace_plain_kernel<In, NB, TT, GATHER>), dim3(static_cast<unsigned>(wgs)),
+  hipLaunchKernelGGL((rxace_plain_kernel<In, NB, TT, GATHER, F32>), dim3(static_cast<unsigned>(wgs)),
                      dim3(kRxWaves * 64), 0, stream, static_cast<const In*>(a.cube), a.pixels,
                      a.bands, a.stride, a.num_targets, a.mean, a.white_blocks, a.ace_u,
                      a.target_norm2, a.rx, a.ace_score, a.ace_target, gth);
   return hipGetLastError();
 }
 
-template <typename In, int NB, bool GATHER>
+template <typename In, int NB, bool GATHER, bool F32>
 hipError_t launch_plain_nb(const RxAceArgs& a, const Gather& gth, hipStream_t stream) {
-  if (a.num_targets == 0 || (!a.ace_score && !a.ace_target)) return launch_plain_t<In, NB, 0, GATHER>(a, gth, stream);
-  if (a.num_targets <= 16) return launch_plain_t<In, NB, 1, GATHER>(a, gth, stream);
-  return launch_plain_t<In, NB, 2, GATHER>(a, gth, stream);
+  if (a.num_targets == 0 || (!a.ace_score && !a.ace_target))
+    return launch_plain_t<In, NB, 0, GATHER, F32>(a, gth, stream);
+  if (a.num_targets <= 16) return launch_plain_t<In, NB, 1, GATHER, F32>(a, gth, stream);
+  return launch_plain_t<In, NB, 2, GATHER, F32>(a, gth, stream);
 }
 
-template <typename In, bool GATHER>
+template <typename In, bool GATHER, bool F32 = false>
 hipError_t launch_plain_typed(const RxAceArgs& a, const Gather& gth, hipStream_t stream) {
   switch (rx_blocks(a.bands)) {
-#define HSI_NB(N) case N: return launch_plain_nb<In, N, GATHER>(a, gth, stream);
+#define HSI_NB(N) case N: return launch_plain_nb<In, N, GATHER, F32>(a, gth, stream);
     HSI_NB(1) HSI_NB(2) HSI_NB(3) HSI_NB(4) HSI_NB(5) HSI_NB(6) HSI_NB(7) HSI_NB(8)
     HSI_NB(9) HSI_NB(10) HSI_NB(11) HSI_NB(12) HSI_NB(13) HSI_NB(14)
 #undef HSI_NB
@@ -452,6 +473,12 @@ hipError_t launch_rxace_plain(const RxAceArgs& a, hipStream_t stream) {
   const Gather none;
   if (a.dtype == HSI_F16) return launch_plain_typed<_Float16, false>(a, none, stream);
   return launch_plain_typed<float, false>(a, none, stream);
+}
+
+hipError_t launch_rxace_fp32(const RxAceArgs& a, hipStream_t stream) {
+  const Gather none;
+  if (a.dtype == HSI_F16) return launch_plain_typed<_Float16, false, true>(a, none, stream);
+  return launch_plain_typed<float, false, true>(a, none, stream);
 }
 
 hipError_t launch_rxace_candidates(const RxAceArgs& a, const hsi_detection* cand, const unsigned* count,

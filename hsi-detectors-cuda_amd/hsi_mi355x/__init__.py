@@ -40,7 +40,7 @@ DETECT_NO_ALLOC_MAX_PIXELS = 1 << 24
 # Statistics status-word bits (HSI_STATUS_*).
 STATUS_COV_NOT_PD, STATUS_CORR_NOT_PD, STATUS_BELOW_TOLERANCE = 1, 2, 4
 F32, F16, Q16 = 0, 1, 2
-PLAIN, MIXED = 0, 1
+PLAIN, MIXED, FP32 = 0, 1, 2
 
 
 class Detector(enum.IntEnum):

@@ -67,8 +67,11 @@ typedef enum hsi_dtype { HSI_F32 = 0, HSI_F16 = 1, HSI_Q16 = 2 } hsi_dtype;
 
 /* Arithmetic form. PLAIN accumulates in FP64 (the oracle's precision) on an
  * FP32 or FP16 cube. MIXED runs the RX/ACE whitening and the covariance on
- * FP16 matrix cores with FP32 accumulation and FP64 cross-tile reduction. */
-typedef enum hsi_precision { HSI_PLAIN = 0, HSI_MIXED = 1 } hsi_precision;
+ * FP16 matrix cores with FP32 accumulation and FP64 cross-tile reduction.
+ * FP32 (RX/ACE scoring only; the statistics stay PLAIN) runs the RX/ACE
+ * whitening and ACE's products on FP32 matrix cores (v_mfma_f32_16x16x4f32)
+ * with FP32 accumulation: the plain method at single precision. */
+typedef enum hsi_precision { HSI_PLAIN = 0, HSI_MIXED = 1, HSI_FP32 = 2 } hsi_precision;
 
 /* 16-byte detection record, layout-identical to hsi::Detection
  * (include/hsi/types.hpp:101-106). */

@@ -252,6 +252,34 @@ def test_rx_ace_with_reference_statistics(H, torch, ctx, name):
 
 
 @pytest.mark.parametrize("name", SCENES)
+def test_rx_ace_fp32_form_with_reference_statistics(H, torch, ctx, name):
+    """The FP32 form (whitening and ACE products on FP32 matrix cores, FP32
+    accumulation) against the reference's double-precision scores, on the
+    reference's statistics: within the plain bar of 1e-5 relative (floor 1),
+    the ACE target as the plain form's (at most one pixel in 10^4 differs)."""
+    g = golden(name)
+    cube, b, h, w = load_scene(H, torch, g)
+    ctx.set_library(g["library"], g["norms"])
+    upload_golden_stats(ctx, g)
+    p = h * w
+    rx = device_buffers(torch, p)
+    ctx.score(H.Detector.RX, cube, h, w, b, rx["score"], precision=H.FP32)
+    ace = device_buffers(torch, p)
+    ctx.score(H.Detector.ACE, cube, h, w, b, ace["score"], ace["target"], ace["aux"], precision=H.FP32)
+    torch.cuda.synchronize()
+    e_rx = rel_err(rx["score"].cpu().numpy(), g["rx_score"], FLOOR["rx"])
+    e_aux = rel_err(ace["aux"].cpu().numpy(), g["ace_rx"], FLOOR["rx"])
+    e_ace = rel_err(ace["score"].cpu().numpy(), g["ace_score"], FLOOR["ace"])
+    print(f"{name}: FP32 form RX max rel {e_rx.max():.3g}, ACE max {e_ace.max():.3g}")
+    assert e_rx.max() <= REL and e_aux.max() <= REL and e_ace.max() <= REL
+    bad = np.nonzero(ace["target"].cpu().numpy() != g["ace_target"])[0]
+    assert bad.size <= max(1, p // 10000), f"{bad.size} ACE target mismatches"
+    # SAM and CEM have no FP32 form
+    with pytest.raises(H.HsiError):
+        ctx.score(H.Detector.SAM, cube, h, w, b, rx["score"], rx["target"], precision=H.FP32)
+
+
+@pytest.mark.parametrize("name", SCENES)
 def test_moments_and_statistics_built_on_device(H, torch, ctx, oracle_lib, name):
     g = golden(name)
     cube, b, h, w = load_scene(H, torch, g)

@@ -7,7 +7,8 @@ build, FP16 MFMA whitening of an FP16 cube. The bar is 1e-3: RX relative
 (floor 1), ACE absolute. The oracle scores the same FP16 values the device
 reads, with statistics from the exact moments of those values, so the error
 is the arithmetic's alone. The plain path on the same cube must stay within
-1e-5.
+1e-5, and so must the FP32 form (whitening on FP32 matrix cores, FP32
+accumulation) on the plain statistics.
 """
 import numpy as np
 import pytest
@@ -36,7 +37,7 @@ def scene(oracle_lib):
     return dict(H=H, h=h, w=w, b=b, lib=lib, halves=halves, ace=want_ace, target=want_t, rx=want_rx)
 
 
-@pytest.mark.parametrize("form", ["plain", "mixed"])
+@pytest.mark.parametrize("form", ["plain", "mixed", "fp32"])
 def test_rx_ace_path_against_fp64_oracle(scene, form):
     import torch
     H = scene["H"]
@@ -47,11 +48,12 @@ def test_rx_ace_path_against_fp64_oracle(scene, form):
     c16 = c32.half()
     ctx = H.Context(0)
     ctx.set_library(scene["lib"])
-    prec = H.MIXED if form == "mixed" else H.PLAIN
+    prec = {"plain": H.PLAIN, "mixed": H.MIXED, "fp32": H.FP32}[form]
     ace = torch.empty(p, dtype=torch.float32, device="cuda")
     rx = torch.empty(p, dtype=torch.float32, device="cuda")
     tg = torch.empty(p, dtype=torch.int32, device="cuda")
-    ctx.stats_accumulate(c32, h, w, b, precision=prec)
+    # the FP32 form scores on the plain (FP64) statistics
+    ctx.stats_accumulate(c32, h, w, b, precision=H.MIXED if form == "mixed" else H.PLAIN)
     ctx.stats_build(LOADING)
     ctx.stats_check()
     ctx.score(H.Detector.ACE, c16, h, w, b, ace, tg, rx, precision=prec)
