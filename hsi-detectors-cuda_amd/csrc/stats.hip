@@ -420,12 +420,11 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const flo
 // pitch of CH + 8 halves (16 lanes reading 16 bytes each at rows m = 0..15
 // land on 16 disjoint bank quads). Each owner position is its own code path,
 // holding only its run's accumulators and operand blocks.
-// The screen fused into the statistics pass: LDS copies of the network and
-// the outputs. Wave w screens pixels 16w .. 16w + 15 of each 128-pixel chunk.
-// Candidates go to per-wave regions (no atomics): wave v of the grid owns
-// rows [v * rc, v * rc + rc) of the record scratch and of the spectra, keeps
-// its count in a register and leaves it in counts[v]; a scan and a
-// compaction pass make the record list contiguous afterwards.
+// The screen fused into the statistics pass: its outputs. Candidates go to
+// per-wave regions (no atomics): screening wave v of the grid owns rows
+// [v * rc, v * rc + rc) of the record scratch and of the spectra, keeps its
+// count in a register and leaves it in counts[v]; a scan and a compaction
+// pass make the record list contiguous afterwards.
 struct FusedScreenDev {
   float cut;
   int nkb;                  // 32-band K blocks of layer 1: ceil(bands / 32)
@@ -436,38 +435,85 @@ struct FusedScreenDev {
   float* spectra;
   std::size_t spectra_stride;
 };
-struct ScreenLds {
-  const h8* w1;             // [J][I][lane]
-  const h8* w2;             // [kb][I2][lane]
-  const float* tail;        // b1[64] | b2[32] | w3[32] | b3
-  FusedScreenDev fs;
-};
 constexpr int kFusedChunk = 128;
 
-// One chunk's screening for wave w (hi, lo: the chunk's FP16 parts of
-// x - shift, [row][pixel] with pitch PITCH; rows past ROWS read 0). Layer 1
-// is v_mfma_f32_16x16x32_f16 with the FP16 weights as A and the hi parts as
-// B (lane (g, n): bands 32J + 8g .. +7 of pixel 16w + n); layers 2 and 3 as
-// in the standalone inference kernel (layer 1's accumulators feed layer 2
-// directly, its K order permuted to match). A candidate's spectrum is
-// rebuilt from the staged parts as shift + (hi + lo) in FP32 - within
-// 2^-21 |x - shift| + 2^-23 |x| + 2^-24 of the pixel (the subtraction, lo's
-// rounding, absolute where lo is subnormal, and the add) - so nothing waits
-// on memory inside the chunk loop (the stores are not waited for).
-// The logit of pixel 16 * wave + (lane % 16) of the chunk, in every lane
-// group. Branch-free (K blocks past ceil(bands / 32) have zero weights in
-// LDS), so the compiler can interleave it with the chunk's moment MFMAs,
-// which read the same tile.
-template <int NB, int PITCH, int ROWS>
-__device__ __forceinline__ float screen_logit(const _Float16* tile, const ScreenLds& sl, int wave) {
-  constexpr int NKBM = (NB + 1) / 2;
+// FP16 hi/lo split of two FP32 values d0, d1: hi = both rounded to FP16
+// (one packed conversion), lo = d - hi rounded to FP16 by one mixed-precision
+// FMA each (-hi * 1 + d; the difference is exact in FP32, so this equals
+// rounding the FP32 difference). Returns {hi pair, lo pair} as packed words.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint2 split_pair(float d0, float d1) {
+  const h2v h = __builtin_convertvector((f2v{d0, d1}), h2v);
+  const unsigned hp = __builtin_bit_cast(unsigned, h);
+  unsigned lp;
+  asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %0, -%1, 1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=&v"(lp)
+      : "v"(hp), "v"(d0), "v"(d1));
+  return make_uint2(hp, lp);
+}
+
+// The network held in one screening wave's registers: layer 1 and 2 FP16
+// fragments in the matrix-core operand layout (lane l = 16g + m), and the
+// biases and output weights each lane needs (b1 and b2 rows 4g .. 4g + 3 of
+// each 16-row block, w3 likewise).
+template <int NKBM>
+struct ScreenRegs {
+  h8 w1[NKBM * 4];
+  h8 w2[4];
+  float b1[16];
+  float b2[8];
+  float w3[8];
+  float b3;
+};
+
+template <int NKBM>
+__device__ __forceinline__ void screen_regs_load(ScreenRegs<NKBM>& r, const h8* __restrict__ w1h,
+                                                 const h8* __restrict__ w2h, const float* __restrict__ tail,
+                                                 int nkb) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
-  const int n = lane & 15;
-  const int col = 16 * wave + n;
+#pragma unroll
+  for (int J = 0; J < NKBM; ++J)
+#pragma unroll
+    for (int I = 0; I < 4; ++I) r.w1[J * 4 + I] = J < nkb ? w1h[(J * 4 + I) * 64 + lane] : h8{};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) r.w2[e] = w2h[e * 64 + lane];
+#pragma unroll
+  for (int I = 0; I < 4; ++I)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r.b1[4 * I + q] = tail[16 * I + 4 * g + q];
+#pragma unroll
+  for (int I2 = 0; I2 < 2; ++I2)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      r.b2[4 * I2 + q] = tail[64 + 16 * I2 + 4 * g + q];
+      r.w3[4 * I2 + q] = tail[96 + 16 * I2 + 4 * g + q];
+    }
+  r.b3 = tail[128];
+}
+
+// The logits of the 16 pixels at column col0 of a staged chunk (hi: the
+// chunk's FP16 parts of x - shift, [row][pixel] with pitch PITCH), in every
+// lane group for pixel col0 + lane % 16. Layer 1 is v_mfma_f32_16x16x32_f16
+// with the FP16 weights as A and the hi parts as B (lane (g, n): bands
+// 32J + 8g .. +7 of pixel col0 + n); layers 2 and 3 as in the standalone
+// inference kernel (layer 1's accumulators feed layer 2 directly, its K order
+// permuted to match). K blocks past ceil(bands / 32), and weight rows past
+// the band count, are zero.
+template <int NB, int PITCH>
+__device__ __forceinline__ float screen_logit_regs(const _Float16* tile, const ScreenRegs<(NB + 1) / 2>& r,
+                                                   int col0) {
+  constexpr int NKBM = (NB + 1) / 2;
+  constexpr int ROWS = NB * 16;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int col = col0 + (lane & 15);
+  // Accumulators start at the biases.
   f4 z1[4];
 #pragma unroll
-  for (int I = 0; I < 4; ++I) z1[I] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int I = 0; I < 4; ++I) z1[I] = f4{r.b1[4 * I], r.b1[4 * I + 1], r.b1[4 * I + 2], r.b1[4 * I + 3]};
 #pragma unroll
   for (int J = 0; J < NKBM; ++J) {
     h8 x;
@@ -478,54 +524,58 @@ __device__ __forceinline__ float screen_logit(const _Float16* tile, const Screen
       else x[i] = tile[row * PITCH + col];
     }
 #pragma unroll
-    for (int I = 0; I < 4; ++I)
-      z1[I] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl.w1[(J * 4 + I) * 64 + lane], x, z1[I], 0, 0, 0);
+    for (int I = 0; I < 4; ++I) z1[I] = __builtin_amdgcn_mfma_f32_16x16x32_f16(r.w1[J * 4 + I], x, z1[I], 0, 0, 0);
   }
-  f4 z2[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+  f4 z2[2] = {f4{r.b2[0], r.b2[1], r.b2[2], r.b2[3]}, f4{r.b2[4], r.b2[5], r.b2[6], r.b2[7]}};
 #pragma unroll
   for (int kb = 0; kb < 2; ++kb) {
     h8 hb;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int I = 2 * kb + (i < 4 ? 0 : 1);
-      const int r = i & 3;
-      const float h = z1[I][r] + sl.tail[16 * I + 4 * g + r];
+      const int q = i & 3;
+      const float h = z1[I][q];
       hb[i] = static_cast<_Float16>(h > 0.f ? h : 0.f);
     }
 #pragma unroll
     for (int I2 = 0; I2 < 2; ++I2)
-      z2[I2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl.w2[(kb * 2 + I2) * 64 + lane], hb, z2[I2], 0, 0, 0);
+      z2[I2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(r.w2[kb * 2 + I2], hb, z2[I2], 0, 0, 0);
   }
   float out = 0.f;
 #pragma unroll
   for (int I2 = 0; I2 < 2; ++I2)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int k = 16 * I2 + 4 * g + r;
-      const float h = z2[I2][r] + sl.tail[64 + k];
-      out += sl.tail[96 + k] * (h > 0.f ? h : 0.f);
+    for (int q = 0; q < 4; ++q) {
+      const float h = z2[I2][q];
+      out += r.w3[4 * I2 + q] * (h > 0.f ? h : 0.f);
     }
   out += __shfl_xor(out, 16, 64);
   out += __shfl_xor(out, 32, 64);
-  return out + sl.tail[128];
+  return out + r.b3;
 }
 
-// Candidates of the chunk from this wave's logits: a record and a spectra
-// row each, in the wave's region.
+// Candidates among the 16 pixels at column col0 of the chunk starting at
+// pixel p0: a record and a spectra row each, in region `region_id`. A
+// candidate's spectrum is rebuilt from the staged parts as shift + (hi + lo)
+// in FP32 - within 2^-21 |x - shift| + 2^-23 |x| + 2^-24 of the pixel (the
+// subtraction, lo's rounding, absolute where lo is subnormal, and the add) -
+// so nothing waits on memory inside the chunk loop (the stores are not
+// waited for).
 template <int PITCH>
-__device__ __forceinline__ void screen_emit(float logit, const _Float16* hi, const _Float16* lo, const float* sh,
-                                            const ScreenLds& sl, int wave, std::size_t wave_id, std::size_t p0,
-                                            std::size_t p_end, int bands, unsigned& local) {
+__device__ __forceinline__ void screen_emit_tile(float logit, const _Float16* hi, const _Float16* lo,
+                                                 const float* sh, const FusedScreenDev& fs, int col0,
+                                                 std::size_t region_id, std::size_t p0, std::size_t p_end,
+                                                 int bands, unsigned& local) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int n = lane & 15;
-  const std::size_t px = p0 + static_cast<std::size_t>(16 * wave + n);
-  const bool flag = g == 0 && px < p_end && logit >= sl.fs.cut;
+  const std::size_t px = p0 + static_cast<std::size_t>(col0 + n);
+  const bool flag = g == 0 && px < p_end && logit >= fs.cut;
   const unsigned long long m = __ballot(flag);
   if (m == 0ull) return;
-  const int pad = 32 * sl.fs.nkb;
-  const std::size_t region = wave_id * sl.fs.rc;
-  const std::size_t w = static_cast<std::size_t>(sl.fs.width);
+  const int pad = 32 * fs.nkb;
+  const std::size_t region = region_id * fs.rc;
+  const std::size_t w = static_cast<std::size_t>(fs.width);
   // Four candidates a round (the mask and the candidates' columns are
   // wave-uniform, so the bit walk is scalar): all of a round's LDS reads
   // are issued before any store.
@@ -543,8 +593,8 @@ __device__ __forceinline__ void screen_emit(float logit, const _Float16* hi, con
     for (int u = 0; u < kRound; ++u) {
       v0[u] = v1[u] = 0.f;
       lg[u] = __shfl(logit, js[u] < 0 ? 0 : js[u], 64);
-      if (js[u] >= 0 && sl.fs.spectra) {
-        const int cj = 16 * wave + js[u];
+      if (js[u] >= 0 && fs.spectra) {
+        const int cj = col0 + js[u];
         const int b0 = lane, b1 = lane + 64;
         if (b0 < bands)
           v0[u] = sh[b0] + (static_cast<float>(hi[b0 * PITCH + cj]) + static_cast<float>(lo[b0 * PITCH + cj]));
@@ -556,19 +606,19 @@ __device__ __forceinline__ void screen_emit(float logit, const _Float16* hi, con
     for (int u = 0; u < kRound; ++u) {
       if (js[u] < 0) continue;
       const unsigned su = slot + static_cast<unsigned>(u);
-      if (su >= sl.fs.rc) continue;
+      if (su >= fs.rc) continue;
       const std::size_t row = region + su;
       if (lane == 0) {
-        const std::size_t pj = p0 + 16 * static_cast<std::size_t>(wave) + static_cast<std::size_t>(js[u]);
+        const std::size_t pj = p0 + static_cast<std::size_t>(col0 + js[u]);
         hsi_detection d;
         d.x = static_cast<int>(pj % w);
         d.y = static_cast<int>(pj / w);
         d.target = static_cast<int>(row);
         d.score = lg[u];
-        sl.fs.rec[row] = d;
+        fs.rec[row] = d;
       }
-      if (sl.fs.spectra) {
-        float* dst = sl.fs.spectra + row * sl.fs.spectra_stride;
+      if (fs.spectra) {
+        float* dst = fs.spectra + row * fs.spectra_stride;
         if (lane < pad) dst[lane] = v0[u];
         if (lane + 64 < pad) dst[lane + 64] = v1[u];
       }
@@ -578,12 +628,11 @@ __device__ __forceinline__ void screen_emit(float logit, const _Float16* hi, con
   local += static_cast<unsigned>(__popcll(m));
 }
 
-template <typename In, int NB, int POS, int CH = f16_chunk<NB>(), bool SCREEN = false>
+template <typename In, int NB, int POS, int CH = f16_chunk<NB>()>
 __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cube, std::size_t p_begin,
                                                     std::size_t p_end, int bands, std::size_t stride,
                                                     const float* sh, const float* qo, const float* qs,
-                                                    _Float16* hi_t, _Float16* lo_t, double* __restrict__ out,
-                                                    const ScreenLds* sl = nullptr) {
+                                                    _Float16* hi_t, _Float16* lo_t, double* __restrict__ out) {
   constexpr int NP = NB * (NB + 1) / 2;
   constexpr int OWNERS = kWavesPerWg * f16_splits<NB>();
   constexpr int THREADS = kWavesPerWg * 64;
@@ -614,7 +663,21 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
       pre[k] = stage_raw_t<In>(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_begin, p_end);
     }
   };
+  // Each thread stages the same rows of every chunk, so the per-row affine
+  // d = raw * rm + ra lives in registers: x - shift on a real band (an exact
+  // product by 1, so d is the rounded difference), 1 on the ones band, 0 on
+  // the padding rows past it.
+  float rm[PER], ra[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int b = (static_cast<int>(threadIdx.x) + THREADS * k) / (CH / 4);
+    const bool real = b < bands;
+    rm[k] = real ? 1.f : 0.f;
+    ra[k] = real ? -sh[b < ROWS ? b : 0] : (b == bands ? 1.f : 0.f);
+  }
   auto store = [&](int buf, std::size_t p) {
+    // Pixels past the range are zeroed only in a partial chunk.
+    const bool full = p + CH <= p_end;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + THREADS * k;
@@ -622,32 +685,26 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
         const int b = e / (CH / 4);
         const int k4 = (e % (CH / 4)) * 4;
         const std::size_t px = p + k4;
-        // Shift real bands only, and only real pixels (padding stays zero).
-        const float s = b < bands ? sh[b] : 0.f;
         // (b < ROWS here; the Q16 code arrays exist only for Q16 cubes)
         const float bqo = is_q16<In>::value ? qo[b] : 0.f, bqs = is_q16<In>::value ? qs[b] : 0.f;
-        const float4 f = stage_fix(stage_widen<In>(pre[k], bqo, bqs), bands, b, px, p_end, true);
-        float v[4] = {f.x, f.y, f.z, f.w};
-        h4 hv, lv;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float d = (px + i < p_end) ? v[i] - s : 0.f;
-          const _Float16 h = static_cast<_Float16>(d);
-          hv[i] = h;
-          lv[i] = static_cast<_Float16>(d - static_cast<float>(h));
+        const float4 f = stage_widen<In>(pre[k], bqo, bqs);
+        float d0 = __builtin_fmaf(f.x, rm[k], ra[k]);
+        float d1 = __builtin_fmaf(f.y, rm[k], ra[k]);
+        float d2 = __builtin_fmaf(f.z, rm[k], ra[k]);
+        float d3 = __builtin_fmaf(f.w, rm[k], ra[k]);
+        if (!full) {
+          const std::size_t left = p_end > px ? p_end - px : 0;
+          d0 = left > 0 ? d0 : 0.f;
+          d1 = left > 1 ? d1 : 0.f;
+          d2 = left > 2 ? d2 : 0.f;
+          d3 = left > 3 ? d3 : 0.f;
         }
-        *reinterpret_cast<h4*>(hi_t + buf * TILE + b * PITCH + k4) = hv;
-        *reinterpret_cast<h4*>(lo_t + buf * TILE + b * PITCH + k4) = lv;
+        const uint2 s01 = split_pair(d0, d1), s23 = split_pair(d2, d3);
+        *reinterpret_cast<uint2*>(hi_t + buf * TILE + b * PITCH + k4) = make_uint2(s01.x, s23.x);
+        *reinterpret_cast<uint2*>(lo_t + buf * TILE + b * PITCH + k4) = make_uint2(s01.y, s23.y);
       }
     }
   };
-
-  // Screening: this wave's 16-pixel tile of each chunk, its candidate count.
-  const int screen_wave = static_cast<int>(threadIdx.x >> 6);
-  const std::size_t screen_wave_id = static_cast<std::size_t>(blockIdx.x) * kWavesPerWg + screen_wave;
-  unsigned screen_local = 0u;
-  (void)screen_wave;
-  (void)screen_wave_id;
 
   load(p_begin);
   store(0, p_begin);
@@ -657,9 +714,6 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
     const std::size_t pn = p_begin + static_cast<std::size_t>(c + 1) * CH;
     if (more) load(pn);
     const int buf = c & 1;
-    float screen_lg = 0.f;
-    if constexpr (SCREEN) screen_lg = screen_logit<NB, PITCH, ROWS>(hi_t + buf * TILE, *sl, screen_wave);
-    (void)screen_lg;
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ++ks) {
       // Operands: lane l = 16g + m takes eight pixels 32ks + 8g .. +7 of band
@@ -691,13 +745,8 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
       }
     }
     if (more) store(buf ^ 1, pn);
-    if constexpr (SCREEN)
-      screen_emit<PITCH>(screen_lg, hi_t + buf * TILE, lo_t + buf * TILE, sh, *sl, screen_wave, screen_wave_id,
-                         p_begin + static_cast<std::size_t>(c) * CH, p_end, bands, screen_local);
     __syncthreads();
   }
-  if constexpr (SCREEN)
-    if (lane == 0) sl->fs.counts[screen_wave_id] = screen_local;
 
 #pragma unroll
   for (int m = 0; m < PW; ++m) {
@@ -765,52 +814,75 @@ __global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_e
   }
 }
 
-// Mixed-form moments with the learned pre-screen on the same staged tiles:
-// 128-pixel chunks, one workgroup per CU (the staging, the network's FP16
-// fragments and the registers of both do not fit two), wave w screening
-// pixels 16w .. 16w + 15 of each chunk. F32 cubes up to 8 moment blocks.
+// Mixed-form moments with the learned pre-screen on the same staged tiles,
+// F32 cubes up to 8 moment blocks, one workgroup per CU. The roles are split
+// over the waves: waves 0-7 stage 128-pixel chunks and accumulate the
+// moments exactly as the mixed-form kernel does, while kScreenWaves more waves
+// (one per SIMD) screen the staged chunk with the network in their
+// registers - wave 8 + s takes pixels 32s .. 32s + 31 of each chunk as two
+// 16-pixel tiles - and write the candidates to their own regions. Both roles
+// meet at the same workgroup barrier once per chunk, so a screening wave's
+// matrix-core and gather work issues beside the moment waves' staging.
+constexpr int kScreenWaves = 4;
 template <int NB>
-__global__ __launch_bounds__(kWavesPerWg * 64) __attribute__((amdgpu_waves_per_eu(2))) void moments_screen_kernel(
-    const float* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride, std::size_t px_per_wg,
-    const float* __restrict__ shift, double* __restrict__ partial, const h8* __restrict__ w1h,
-    const h8* __restrict__ w2h, const float* __restrict__ tail, const FusedScreenDev fs) {
+__global__ __launch_bounds__((kWavesPerWg + kScreenWaves) * 64) __attribute__((amdgpu_waves_per_eu(3))) void
+moments_screen_kernel(const float* __restrict__ cube, std::size_t pixels, int bands, std::size_t stride,
+                         std::size_t px_per_wg, const float* __restrict__ shift, double* __restrict__ partial,
+                         const h8* __restrict__ w1h, const h8* __restrict__ w2h, const float* __restrict__ tail,
+                         const FusedScreenDev fs) {
   static_assert(NB <= 8 && f16_splits<NB>() == 1, "the fused screen covers up to 8 moment blocks");
   constexpr int CH = kFusedChunk;
-  constexpr int THREADS = kWavesPerWg * 64;
   constexpr int PITCH = CH + 8;
   constexpr int ROWS = NB * 16;
   constexpr int NKBM = (NB + 1) / 2;
-  static_assert(kWavesPerWg * 16 == CH, "one 16-pixel screening tile per wave");
-  __shared__ _Float16 hi_t[2 * ROWS * PITCH];
-  __shared__ _Float16 lo_t[2 * ROWS * PITCH];
+  constexpr int TILE = ROWS * PITCH;
+  static_assert(kScreenWaves * 32 == CH, "two 16-pixel screening tiles per screening wave");
+  __shared__ _Float16 hi_t[2 * TILE];
+  __shared__ _Float16 lo_t[2 * TILE];
   __shared__ float sh[ROWS];
   __shared__ float qz[1];
-  __shared__ h8 w1l[NKBM * 4 * 64];
-  __shared__ h8 w2l[4 * 64];
-  __shared__ float tl[129];
-  for (int b = threadIdx.x; b < ROWS; b += THREADS) sh[b] = b < bands ? shift[b] : 0.f;
-  // K blocks past ceil(bands / 32) get zero weights: the screen reads every block.
-  for (int e = threadIdx.x; e < NKBM * 4 * 64; e += THREADS) w1l[e] = e < fs.nkb * 4 * 64 ? w1h[e] : h8{};
-  for (int e = threadIdx.x; e < 4 * 64; e += THREADS) w2l[e] = w2h[e];
-  for (int e = threadIdx.x; e < 129; e += THREADS) tl[e] = tail[e];
+  for (int b = threadIdx.x; b < ROWS; b += (kWavesPerWg + kScreenWaves) * 64) sh[b] = b < bands ? shift[b] : 0.f;
   if (threadIdx.x == 0) qz[0] = 0.f;
   __syncthreads();
   const std::size_t p_begin = blockIdx.x * px_per_wg;
   std::size_t p_end = p_begin + px_per_wg;
   if (p_end > pixels) p_end = pixels;
   if (p_begin >= p_end) return;
-  constexpr int NP = NB * (NB + 1) / 2;
-  double* out = partial + static_cast<std::size_t>(blockIdx.x) * NP * 256;
-  const ScreenLds sl{w1l, w2l, tl, fs};
-  switch (wave_position(threadIdx.x >> 6)) {
-#define HSI_POS(K)                                                                                      \
-  case K:                                                                                               \
-    moments_mixed_owner<float, NB, K, CH, true>(cube, p_begin, p_end, bands, stride, sh, qz, qz, hi_t, lo_t, \
-                                                out, &sl);                                             \
+  const int wave = static_cast<int>(threadIdx.x >> 6);
+  if (wave < kWavesPerWg) {
+    constexpr int NP = NB * (NB + 1) / 2;
+    double* out = partial + static_cast<std::size_t>(blockIdx.x) * NP * 256;
+    switch (wave_position(wave)) {
+#define HSI_POS(K)                                                                                        \
+  case K:                                                                                                 \
+    moments_mixed_owner<float, NB, K, CH>(cube, p_begin, p_end, bands, stride, sh, qz, qz, hi_t, lo_t, out); \
     break;
-    HSI_POS(0) HSI_POS(1) HSI_POS(2) HSI_POS(3) HSI_POS(4) HSI_POS(5) HSI_POS(6) HSI_POS(7)
+      HSI_POS(0) HSI_POS(1) HSI_POS(2) HSI_POS(3) HSI_POS(4) HSI_POS(5) HSI_POS(6) HSI_POS(7)
 #undef HSI_POS
+    }
+    return;
   }
+  // Screening wave s: the same barrier sequence as the moment waves (one
+  // after the first chunk is staged, then one per chunk).
+  const int s = wave - kWavesPerWg;
+  const std::size_t region_id = static_cast<std::size_t>(blockIdx.x) * kScreenWaves + s;
+  ScreenRegs<NKBM> r;
+  screen_regs_load<NKBM>(r, w1h, w2h, tail, fs.nkb);
+  unsigned local = 0u;
+  const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
+  __syncthreads();
+  for (int c = 0; c < chunks; ++c) {
+    const int buf = c & 1;
+    const std::size_t p0 = p_begin + static_cast<std::size_t>(c) * CH;
+    const _Float16* hi = hi_t + buf * TILE;
+    const _Float16* lo = lo_t + buf * TILE;
+    const float l0 = screen_logit_regs<NB, PITCH>(hi, r, 32 * s);
+    const float l1 = screen_logit_regs<NB, PITCH>(hi, r, 32 * s + 16);
+    screen_emit_tile<PITCH>(l0, hi, lo, sh, fs, 32 * s, region_id, p0, p_end, bands, local);
+    screen_emit_tile<PITCH>(l1, hi, lo, sh, fs, 32 * s + 16, region_id, p0, p_end, bands, local);
+    __syncthreads();
+  }
+  if ((threadIdx.x & 63) == 0) fs.counts[region_id] = local;
 }
 
 // Fixed-order sum of the per-workgroup partials, then scatter of each
@@ -1003,13 +1075,14 @@ hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t str
   if ((err = hipGetLastError()) != hipSuccess) return err;
   if constexpr (kCanScreen) {
     if (fs) {
-      const int waves = static_cast<int>(used) * kWavesPerWg;
+      const int waves = static_cast<int>(used) * kScreenWaves;
       const unsigned rc = fs->cap / static_cast<unsigned>(waves);
       FusedScreenDev d{fs->cut, static_cast<int>(div_up(static_cast<std::size_t>(bands), 32)), fs->width, rc,
                        fs->counts, fs->scratch_rec, fs->spectra, fs->spectra_stride};
-      hipLaunchKernelGGL(moments_screen_kernel<NB>, dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64), 0,
-                         stream, static_cast<const float*>(m.cube), pixels, bands, m.stride, px_per_wg, shift,
-                         partial, static_cast<const h8*>(fs->w1h), static_cast<const h8*>(fs->w2h), fs->tail, d);
+      hipLaunchKernelGGL(moments_screen_kernel<NB>, dim3(static_cast<unsigned>(used)),
+                         dim3((kWavesPerWg + kScreenWaves) * 64), 0, stream, static_cast<const float*>(m.cube),
+                         pixels, bands, m.stride, px_per_wg, shift, partial, static_cast<const h8*>(fs->w1h),
+                         static_cast<const h8*>(fs->w2h), fs->tail, d);
       if ((err = hipGetLastError()) != hipSuccess) return err;
       if ((err = launch_screen_compact(fs->counts, waves, rc, fs->cap, fs->counts + kScreenMaxWaves, fs->count,
                                        fs->scratch_rec, fs->cand, stream)) != hipSuccess)
@@ -1126,8 +1199,8 @@ hipError_t launch_moments_screen(const MomentArgs& m, const FusedScreenArgs& fs,
       !fs.w1h || !fs.w2h || !fs.tail || !fs.count || !fs.cand || !fs.counts || !fs.scratch_rec || fs.width <= 0 ||
       !fits_buffer(m.bands, m.stride, sizeof(float)))
     return hipErrorInvalidValue;
-  // one workgroup per CU at most: 256 x 8 waves
-  static_assert(256 * kWavesPerWg <= kScreenMaxWaves, "screening counters");
+  // one workgroup per CU at most: 256 x kScreenWaves screening regions
+  static_assert(256 * kScreenWaves <= kScreenMaxWaves, "screening counters");
   if (fs.spectra && fs.spectra_stride < div_up(static_cast<std::size_t>(m.bands), 32) * 32)
     return hipErrorInvalidValue;
   return launch_fused_screen(m, fs, scratch, stream);

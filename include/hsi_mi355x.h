@@ -172,8 +172,10 @@ hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtyp
  * lo) in FP32 (within 2^-21 |x - shift| + 2^-23 |x| + 2^-24 of each value),
  * and 0
  * from `bands` to the 32-band boundary; rows are distinct but not in list
- * order. The pass splits the capacity evenly over its waves (at most 2048);
- * if any wave flags more than its share, or the total exceeds `capacity`,
+ * order. The pass splits the capacity evenly over its screening waves (at
+ * most 1024, each screening its own 32 of every 128 pixels of a workgroup's
+ * range); if any of them flags more than its share, or the total exceeds
+ * `capacity`,
  * *d_count is set above `capacity` (the list overflowed, and
  * hsi_score_candidate_spectra then scores the whole frame). At most 127
  * bands. Asynchronous: build the statistics, then rescore the rows. */
