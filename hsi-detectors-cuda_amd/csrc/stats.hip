@@ -509,19 +509,30 @@ __device__ __forceinline__ float screen_logit_regs(const _Float16* tile, const S
   constexpr int ROWS = NB * 16;
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
-  const int col = col0 + (lane & 15);
   // Accumulators start at the biases.
   f4 z1[4];
 #pragma unroll
   for (int I = 0; I < 4; ++I) z1[I] = f4{r.b1[4 * I], r.b1[4 * I + 1], r.b1[4 * I + 2], r.b1[4 * I + 3]};
+  // B operand by transposed LDS reads (ds_read_b64_tr_b16): within each
+  // 16-lane group, lane c supplies the address of the four pixels
+  // col0 + 4 (c % 4) .. +3 of row k0 + c / 4, and lane i receives rows
+  // k0 .. k0 + 3 of pixel col0 + i - the operand layout, two reads per K
+  // block. A row past the tile (odd NB) is read as the tile's last row, a
+  // padding row of zeros or ones whose weights are zero.
+  const int c = lane & 15;
 #pragma unroll
   for (int J = 0; J < NKBM; ++J) {
     h8 x;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int row = 32 * J + 8 * g + i;
-      if constexpr (32 * NKBM > ROWS) x[i] = row < ROWS ? tile[row * PITCH + col] : static_cast<_Float16>(0.f);
-      else x[i] = tile[row * PITCH + col];
+    for (int h = 0; h < 2; ++h) {
+      int row = 32 * J + 8 * g + 4 * h + (c >> 2);
+      if constexpr (32 * NKBM > ROWS) row = row < ROWS ? row : ROWS - 1;
+      typedef short v4s __attribute__((ext_vector_type(4)));
+      const h4 t = __builtin_bit_cast(h4, __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+                                              (__attribute__((address_space(3))) v4s*)(tile + row * PITCH + col0 +
+                                                                                       4 * (c & 3))));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[4 * h + e] = t[e];
     }
 #pragma unroll
     for (int I = 0; I < 4; ++I) z1[I] = __builtin_amdgcn_mfma_f32_16x16x32_f16(r.w1[J * 4 + I], x, z1[I], 0, 0, 0);

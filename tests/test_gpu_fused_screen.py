@@ -183,3 +183,43 @@ def test_fused_screen_at_frame_size_and_overflow(H):
     px3 = written[:, 1].astype(np.int64) * w + written[:, 0]
     assert set(px3.tolist()) <= set(px.tolist())
     ctx.close()
+
+
+@pytest.mark.parametrize("bands", [100, 111])
+def test_fused_screen_with_an_odd_block_count(H, bands):
+    """Band counts whose staged tile holds an odd number of 16-row blocks
+    (seven: 100 bands, and 111, where the ones band is the tile's last row):
+    the network's last 32-band K block reaches past the tile, and its rows
+    there carry zero weights. The list against the oracle's flags and the
+    moments against float64."""
+    import torch
+    h, w, b = 96, 70, bands
+    p = h * w
+    cfg = H.synth_config(h, w, b, 4, seed=7, anomaly_fraction=0.02)
+    host = H.synth_frame(cfg, 0, stride=p, threads=8)
+    cube = torch.from_numpy(host).cuda()
+    ctx = H.Context(0)
+    ctx.set_library(H.synth_library(cfg))
+    ctx.mlp_init(b, seed=11)
+    wts = ctx.mlp_get_weights().astype(np.float64)
+    x64 = host.astype(np.float64)
+    shift_h = x64.mean(axis=1).astype(np.float32)
+    ref = oracle_logits(wts, host, shift_h, b)
+    cut = float(np.quantile(ref, 0.8))
+    cap = 8 * p
+    pitch = 128
+    shift = torch.from_numpy(shift_h).cuda()
+    cand, count, spectra = run(H, ctx, cube, h, w, b, cut, cap, pitch, shift)
+    px, lg, total = check_list(cand, count, spectra, host, shift_h, h, w, b, cap)
+    want = set(np.flatnonzero(ref >= cut).tolist())
+    odd = np.array(sorted(want ^ set(px.tolist())), dtype=np.int64)
+    print(f"{b} bands: {total} flagged, oracle {len(want)}, {odd.size} near the cut", flush=True)
+    assert total > 0
+    assert np.all(np.abs(ref[odd] - cut) <= NEAR)
+    assert np.all(np.abs(lg - ref[px]) <= NEAR)
+    m = ctx.moments_read(b)
+    lower = np.tril_indices(b)
+    assert m["count"][0] == p
+    assert rel_err(m["sum"], x64.sum(axis=1), 1e-30).max() <= 1e-6
+    assert rel_err(m["sum_outer"][lower], (x64 @ x64.T)[lower], 1e-30).max() <= 1e-6
+    ctx.close()
