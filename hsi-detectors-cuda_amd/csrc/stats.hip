@@ -1007,17 +1007,25 @@ __global__ __launch_bounds__(256) void shift_kernel(const void* __restrict__ cub
   const std::size_t runs = div_up(pixels, run_stride);
   const std::size_t samples = runs * 64;
   double s = 0.0, c = 0.0;
-  // Loads from a clamped index with no branch around them, so a thread's
-  // samples are all in flight at once; samples past the frame add 0.
-  for (std::size_t i = threadIdx.x; i < samples; i += 256) {
+  // At most 64 runs of 64 samples: 16 per thread, loaded first from a
+  // clamped index with no branch around them so that all are in flight at
+  // once, then added in order; samples past the frame add 0.
+  constexpr int kPer = 64 * 64 / 256;
+  float v[kPer];
+  bool in[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const std::size_t i = threadIdx.x + 256 * static_cast<std::size_t>(k);
     const std::size_t p = (i / 64) * run_stride + (i % 64);
-    const bool in = p < pixels;
-    const std::size_t pc = in ? p : 0;
-    float v;
-    if constexpr (is_q16<In>::value) v = q16_decode(static_cast<float>(plane[pc].code), q_scale[b], q_offset[b]);
-    else v = static_cast<float>(plane[pc]);
-    s += in ? static_cast<double>(v) : 0.0;
-    c += in ? 1.0 : 0.0;
+    in[k] = i < samples && p < pixels;
+    const std::size_t pc = in[k] ? p : 0;
+    if constexpr (is_q16<In>::value) v[k] = q16_decode(static_cast<float>(plane[pc].code), q_scale[b], q_offset[b]);
+    else v[k] = static_cast<float>(plane[pc]);
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    s += in[k] ? static_cast<double>(v[k]) : 0.0;
+    c += in[k] ? 1.0 : 0.0;
   }
   red[threadIdx.x] = s;
   cnt[threadIdx.x] = c;
