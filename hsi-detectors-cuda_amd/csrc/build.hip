@@ -111,9 +111,14 @@ __device__ __forceinline__ double rsqrt_refined(double p) {
 
 // Factor the 16x16 diagonal tile and invert it, in one wave; both halves are
 // serial chains, so each step is kept short.
-//   Cholesky: lane i < 16 holds row i; pivot j's value and L[k][j] reach the
-//   other lanes through v_readlane; sqrt and the division by the pivot come
-//   from one refined reciprocal square root (d = p / sqrt(p) = p * r).
+//   Cholesky: lane i < 16 holds row i, and the columns go in panels of
+//   four. Within a panel, pivot j's value and L[k][j] reach the other lanes
+//   through v_readlane; sqrt and the division by the pivot come from one
+//   refined reciprocal square root (d = p / sqrt(p) = p * r). A finished
+//   panel reaches every lane through LDS at once (each row's four values
+//   stored, every lane reading all rows as broadcasts), and the columns
+//   right of it take its four updates in column order - the same operations
+//   in the same order as a column-at-a-time sweep, so L is identical to it.
 //   Inverse: lane c < 16 computes column c of L^-1 by forward substitution,
 //   reading L's rows as LDS broadcasts, so no lane waits on another.
 // Returns 0, or the first failing pivot's kind: kPivotNotPositive (not
@@ -125,7 +130,7 @@ __device__ __forceinline__ double rsqrt_refined(double p) {
 constexpr int kPivotNotPositive = 1;
 constexpr int kPivotBelowTolerance = 2;
 template <typename P>
-__device__ __attribute__((noinline)) int diag_factor(P* lt, P* xt, const lds_double* diag0, int lane) {
+__device__ __forceinline__ int diag_factor(P* lt, P* xt, const lds_double* diag0, int lane) {
   double row[kTile];
   double rinv[kTile];
   const int i = lane & 15;
@@ -134,22 +139,50 @@ __device__ __attribute__((noinline)) int diag_factor(P* lt, P* xt, const lds_dou
   bool ok = true;
   int why = 0;
   const double tol = 4.0 * 256 * 2.220446049250313e-16;
+  constexpr int kPanel = 4;
+  __shared__ double pan[kTile * kPanel];
+  typedef __attribute__((address_space(3))) double lds_d;
+  lds_d* pn = (lds_d*)pan;
+  // The pivots' tolerances, read before the serial chain starts.
+  double bound[kTile];
 #pragma unroll
-  for (int j = 0; j < kTile; ++j) {
-    const double pivot = lane_value(row[j], j);
-    const double d0 = diag0[j];
-    if (ok && !(pivot > 0.0)) why = kPivotNotPositive;
-    else if (ok && !(pivot > tol * d0)) why = kPivotBelowTolerance;
-    if (why != 0) ok = false;
-    const double r = ok ? rsqrt_refined(pivot) : 1.0;
-    rinv[j] = r;
-    if (i == j) row[j] = ok ? pivot * r : 1.0;
-    if (i > j) row[j] = row[j] * r;
-    const double lij = row[j];
+  for (int j = 0; j < kTile; ++j) bound[j] = tol * diag0[j];
 #pragma unroll
-    for (int k = j + 1; k < kTile; ++k) {
-      const double lkj = lane_value(lij, k);
-      if (i >= k) row[k] -= lij * lkj;
+  for (int p0 = 0; p0 < kTile; p0 += kPanel) {
+#pragma unroll
+    for (int jj = 0; jj < kPanel; ++jj) {
+      const int j = p0 + jj;
+      const double pivot = lane_value(row[j], j);
+      // branch-free: the first failing pivot's kind is kept
+      const int kind = !(pivot > 0.0) ? kPivotNotPositive : (!(pivot > bound[j]) ? kPivotBelowTolerance : 0);
+      why = ok ? kind : why;
+      ok = ok && kind == 0;
+      const double r = ok ? rsqrt_refined(pivot) : 1.0;
+      rinv[j] = r;
+      if (i == j) row[j] = ok ? pivot * r : 1.0;
+      if (i > j) row[j] = row[j] * r;
+      const double lij = row[j];
+#pragma unroll
+      for (int k = j + 1; k < p0 + kPanel; ++k) {
+        const double lkj = lane_value(lij, k);
+        row[k] -= lij * lkj;  // entries above the diagonal are never read
+      }
+    }
+    if (p0 + kPanel < kTile) {
+      if (lane < 16) {
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q) pn[kPanel * i + q] = row[p0 + q];
+      }
+      wave_sync();
+#pragma unroll
+      for (int k = p0 + kPanel; k < kTile; ++k) {
+        double l[kPanel];
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q) l[q] = pn[kPanel * k + q];
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q) row[k] -= row[p0 + q] * l[q];
+      }
+      wave_sync();
     }
   }
   if (lane < 16) {
@@ -157,15 +190,20 @@ __device__ __attribute__((noinline)) int diag_factor(P* lt, P* xt, const lds_dou
     for (int c = 0; c < kTile; ++c) lt[sw(i, c)] = c <= i ? row[c] : 0.0;
   }
   wave_sync();
-  // Column c of X = L^-1: X[c][c] = 1/L[c][c]; X[r][c] = -(sum_{k<r} L[r][k] X[k][c]) / L[r][r].
+  // Column c of X = L^-1: X[c][c] = 1/L[c][c]; X[r][c] = -(sum_{k<r} L[r][k] X[k][c]) / L[r][r],
+  // with each X[k][c] subtracted from every later row's sum as soon as it
+  // is known (b[r] holds minus the partial sum, so the chain from one
+  // X[k][c] to the next is one FMA and one product; the sums take their
+  // terms in the same order, so X is the same as summing row by row).
   const int c = lane & 15;
-  double x[kTile];
+  double x[kTile], bsum[kTile];
 #pragma unroll
-  for (int r = 0; r < kTile; ++r) {
-    double acc = 0.0;
+  for (int r = 0; r < kTile; ++r) bsum[r] = r == c ? 1.0 : 0.0;
 #pragma unroll
-    for (int k = 0; k < r; ++k) acc = __builtin_fma(static_cast<double>(lt[sw(r, k)]), x[k], acc);
-    x[r] = r < c ? 0.0 : (r == c ? rinv[r] : -acc * rinv[r]);
+  for (int k = 0; k < kTile; ++k) {
+    x[k] = k < c ? 0.0 : bsum[k] * rinv[k];
+#pragma unroll
+    for (int r = k + 1; r < kTile; ++r) bsum[r] = __builtin_fma(-static_cast<double>(lt[sw(r, k)]), x[k], bsum[r]);
   }
   if (lane < 16) {
 #pragma unroll

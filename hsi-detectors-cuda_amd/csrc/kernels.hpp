@@ -10,6 +10,34 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 
+// FP16 hi/lo split of two FP32 values d0, d1: hi = both rounded to FP16 by
+// one packed conversion, lo = d - hi (exact in FP32, hi read back from its
+// packed halves) rounded by another - the same values as splitting each
+// value on its own, with two conversions fewer per pair. Returns {hi pair,
+// lo pair} as packed words.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint2 split_pair(float d0, float d1) {
+  const h2v h = __builtin_convertvector((f2v{d0, d1}), h2v);
+  const f2v hf = __builtin_convertvector(h, f2v);
+  const h2v l = __builtin_convertvector((f2v{d0 - hf.x, d1 - hf.y}), h2v);
+  return make_uint2(__builtin_bit_cast(unsigned, h), __builtin_bit_cast(unsigned, l));
+}
+
+// Eight values split into FP16 hi and lo parts (split_pair four times).
+__device__ __forceinline__ void split8(const float (&v)[8], h8& hi, h8& lo) {
+  unsigned hw[4], lw[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint2 s = split_pair(v[2 * q], v[2 * q + 1]);
+    hw[q] = s.x;
+    lw[q] = s.y;
+  }
+  typedef unsigned u4v __attribute__((ext_vector_type(4)));
+  hi = __builtin_bit_cast(h8, (u4v{hw[0], hw[1], hw[2], hw[3]}));
+  lo = __builtin_bit_cast(h8, (u4v{lw[0], lw[1], lw[2], lw[3]}));
+}
+
 // ---- SAM / CEM ---------------------------------------------------------------
 struct SamCemArgs {
   const void* cube = nullptr;

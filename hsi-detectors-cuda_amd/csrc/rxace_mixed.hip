@@ -84,15 +84,6 @@ __device__ __forceinline__ float raw_unpack(typename RawT<In, TP>::type v, int t
   }
 }
 
-__device__ __forceinline__ void split8(const float (&v)[8], h8& hi, h8& lo) {
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const _Float16 h = static_cast<_Float16>(v[i]);
-    hi[i] = h;
-    lo[i] = static_cast<_Float16>(v[i] - static_cast<float>(h));
-  }
-}
-
 __device__ __forceinline__ void ace_merge(double& s, int& t, double s2, int t2) {
   if (t2 < 0) return;
   if (t < 0 || s2 > s || (s2 == s && t2 < t)) {
@@ -190,7 +181,7 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
   __shared__ float rinv_lds[NB * 16];
   // Per-target constants of the ACE epilogue, read per lane there: from LDS
   // they cost no global load per tile.
-  __shared__ float tn2_lds[(TT > 0 ? TT : 1) * 16];
+  __shared__ double iu2_lds[(TT > 0 ? TT : 1) * 16];   // 1 / ||u_t||^2, 0 when not positive
   __shared__ float tinv_lds[(TT > 0 ? TT : 1) * 16];
 
   {
@@ -218,7 +209,8 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
     for (int e = threadIdx.x; e < NB * 16; e += blockDim.x) rinv_lds[e] = row_inv[e];
     if constexpr (TT > 0)
       for (int e = threadIdx.x; e < TT * 16; e += blockDim.x) {
-        tn2_lds[e] = e < num_targets ? target_norm2[e] : 0.f;
+        const double u2 = e < num_targets ? static_cast<double>(target_norm2[e]) : 0.0;
+        iu2_lds[e] = u2 > 0.0 ? 1.0 / u2 : 0.0;
         tinv_lds[e] = e < num_targets ? target_inv[e] : 0.f;
       }
   }
@@ -441,29 +433,34 @@ __global__ __launch_bounds__(mixed_waves(NB, S) * 64) void rxace_mixed_kernel(
       }
 #pragma unroll
       for (int t = 0; t < TP; ++t) {
+        // The pixel's ||z||^2 is common to its targets, so they are ranked
+        // by (z.u_t)^2 / ||u_t||^2 and only the best is divided by it.
         best[t] = -1.0;
         best_t[t] = -1;
+        if (zz[t] > 0.0) {
 #pragma unroll
-        for (int tt = 0; tt < TT; ++tt)
+          for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int tg = 16 * tt + 4 * g + r;
-            if (tg < num_targets) {
-              const double u2 = static_cast<double>(tn2_lds[tg]);
-              if (u2 > 0.0 && zz[t] > 0.0) {
-                const double zu_t = static_cast<double>(zu[t][tt][r]) * tinv_lds[tg];
-                const double sc = (zu_t * zu_t) / (u2 * zz[t]);
-                if (sc == sc) ace_merge(best[t], best_t[t], sc, tg);
+            for (int r = 0; r < 4; ++r) {
+              const int tg = 16 * tt + 4 * g + r;
+              if (tg < num_targets) {
+                const double iu2 = iu2_lds[tg];
+                if (iu2 > 0.0) {
+                  const double zu_t = static_cast<double>(zu[t][tt][r]) * tinv_lds[tg];
+                  const double q = (zu_t * zu_t) * iu2;
+                  if (q == q) ace_merge(best[t], best_t[t], q, tg);
+                }
               }
             }
-          }
+        }
 #pragma unroll
         for (int off = 16; off <= 32; off <<= 1) {
           const double s2 = __shfl_xor(best[t], off, 64);
           const int t2 = __shfl_xor(best_t[t], off, 64);
           ace_merge(best[t], best_t[t], s2, t2);
         }
-        best[t] = best[t] < 0.0 ? 0.0 : (1.0 < best[t] ? 1.0 : best[t]);
+        const double sc = best_t[t] >= 0 ? best[t] / zz[t] : 0.0;
+        best[t] = sc < 0.0 ? 0.0 : (1.0 < sc ? 1.0 : sc);
       }
     }
     if constexpr (GATHER) {

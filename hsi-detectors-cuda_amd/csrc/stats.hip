@@ -437,23 +437,6 @@ struct FusedScreenDev {
 };
 constexpr int kFusedChunk = 128;
 
-// FP16 hi/lo split of two FP32 values d0, d1: hi = both rounded to FP16
-// (one packed conversion), lo = d - hi rounded to FP16 by one mixed-precision
-// FMA each (-hi * 1 + d; the difference is exact in FP32, so this equals
-// rounding the FP32 difference). Returns {hi pair, lo pair} as packed words.
-typedef float f2v __attribute__((ext_vector_type(2)));
-typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint2 split_pair(float d0, float d1) {
-  const h2v h = __builtin_convertvector((f2v{d0, d1}), h2v);
-  const unsigned hp = __builtin_bit_cast(unsigned, h);
-  unsigned lp;
-  asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixhi_f16 %0, -%1, 1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
-      : "=&v"(lp)
-      : "v"(hp), "v"(d0), "v"(d1));
-  return make_uint2(hp, lp);
-}
-
 // The network held in one screening wave's registers: layer 1 and 2 FP16
 // fragments in the matrix-core operand layout (lane l = 16g + m), and the
 // biases and output weights each lane needs (b1 and b2 rows 4g .. 4g + 3 of
