@@ -555,9 +555,10 @@ hipError_t launch_mixed_typed(const RxAceArgs& a, hipStream_t stream) {
 }
 
 // ---------------------------------------------------------------------------
-// Layouts: the power-of-two scales and the split blocks, in one workgroup
-// (coalesced passes over the whitener, maxima reduced in registers, LDS and
-// lane shuffles, no atomics).
+// Layouts: the power-of-two scales in one workgroup (coalesced passes over
+// the whitener, maxima reduced in registers, LDS and lane shuffles, no
+// atomics), then the split blocks by a grid of workgroups, one element per
+// thread.
 
 __device__ __forceinline__ float pow2_for(float m) {
   // 2^(4 - e) where m = f * 2^e, f in [0.5, 1): scales m into [8, 16).
@@ -573,40 +574,30 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-constexpr int kLayoutThreads = 1024;
-constexpr int kLayoutWaves = kLayoutThreads / 64;
+constexpr int kScaleThreads = 1024;
 
-// The precision form's layouts in one workgroup, every pass with all of its
-// loads issued before its reductions (the inputs are a few hundred KB at
-// most, so the passes are latency-bound):
-//  1. column scales s_b = 1 / pow2_for(max_i |W[i][b]|),
-//  2. row scales r_i = pow2_for(max_b |W[i][b] / s_b|),
-//  3. target scales c_t = pow2_for(max_i |U[t][i] * (1 / r_i)|),
-//     written as col_scale, row_inv = 1 / r and target_inv = 1 / c;
-//  4. the split blocks. Whitener: [block][hi|lo][lane][8], value
-//     W''[16I + l%16][32J + 8(l/16) + i] with W'' = W / s (columns) * r
-//     (rows). Targets: [tt][P][hi|lo][lane][8], value c_t * U[t][band] / r_band
-//     with band = 32P + 4(l/16) + i for i < 4 and 32P + 16 + 4(l/16) + (i-4)
-//     above. One element (its hi and lo halves) per thread and step.
+// Column scales s_b = 1 / pow2_for(max_i |W[i][b]|), row scales r_i =
+// pow2_for(max_b |W[i][b]| / s_b), target scales c_t = pow2_for(max_i
+// |U[t][i]| / r_i); written as col_scale, row_inv = 1 / r, target_inv = 1 / c.
 // Maxima of non-negative floats are exact in any order, so the result does
 // not depend on the reduction's shape.
-__global__ __launch_bounds__(kLayoutThreads) void mixed_layouts_kernel(MixedLayoutArgs a, int nwhite,
-                                                                       int ntarget) {
+__global__ __launch_bounds__(kScaleThreads) void mixed_scales_kernel(MixedLayoutArgs a) {
   const int n = a.bands;
   const int T = a.num_targets;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  __shared__ float part[kLayoutThreads];
+  constexpr int kWaves = kScaleThreads / 64;
+  __shared__ float part[kScaleThreads];
   __shared__ float cs[kMaxBands];
-  __shared__ float ri[kMaxBands];
-  __shared__ float ti[kMaxTargets];
-  // 1. Columns: thread t takes column t % cols of rows t / cols, + phases, ...
+  __shared__ float rs[kMaxBands];
+  // Columns: thread t takes column t % cols of rows t / cols, + phases, ...
   const int cols = n <= 128 ? 128 : 256;
-  const int phases = kLayoutThreads / cols;
+  const int phases = kScaleThreads / cols;
   {
     const int b = tid % cols;
     float m = 0.f;
+    // unconditional loads (b < n, i < n), so they are all in flight at once
     if (b < n)
 #pragma unroll 8
       for (int i = tid / cols; i < n; i += phases) {
@@ -616,105 +607,80 @@ __global__ __launch_bounds__(kLayoutThreads) void mixed_layouts_kernel(MixedLayo
     part[tid] = m;
   }
   __syncthreads();
-  for (int b = tid; b < kMaxBands; b += kLayoutThreads) {
+  for (int b = tid; b < kMaxBands; b += kScaleThreads) {
     float m = 0.f;
     if (b < n)
       for (int ph = 0; ph < phases; ++ph) m = fmaxf(m, part[ph * cols + b]);
     cs[b] = b < n ? 1.f / pow2_for(m) : 0.f;
   }
   __syncthreads();
-  // 2. Rows: wave w takes rows w, w + 16, ..., lanes along each row; all of
-  // its elements are loaded first.
-  constexpr int kRowSteps = (kMaxBands + kLayoutWaves - 1) / kLayoutWaves;
-  constexpr int kColSteps = (kMaxRxBands + 63) / 64;
-  {
-    float v[kRowSteps][kColSteps];
-#pragma unroll
-    for (int q = 0; q < kRowSteps; ++q)
-#pragma unroll
-      for (int k = 0; k < kColSteps; ++k) {
-        const int i = wave + kLayoutWaves * q, b = lane + 64 * k;
-        v[q][k] = (i < n && b <= i) ? a.whitener[static_cast<std::size_t>(i) * n + b] : 0.f;
+  // Rows: a wave per row, lanes along it.
+  for (int i = wave; i < kMaxBands; i += kWaves) {
+    float m = 0.f;
+    if (i < n)
+      for (int b = lane; b < n; b += 64) {
+        const float v = fabsf(a.whitener[static_cast<std::size_t>(i) * n + b] / cs[b]);
+        m = b <= i ? fmaxf(m, v) : m;
       }
-#pragma unroll
-    for (int q = 0; q < kRowSteps; ++q) {
-      const int i = wave + kLayoutWaves * q;
-      float m = 0.f;
-#pragma unroll
-      for (int k = 0; k < kColSteps; ++k) {
-        const int b = lane + 64 * k;
-        if (i < n && b <= i) m = fmaxf(m, fabsf(v[q][k] / cs[b]));
-      }
-      m = wave_max(m);
-      if (lane == 0 && i < kMaxBands) ri[i] = 1.f / pow2_for(m);
-    }
+    m = wave_max(m);
+    if (lane == 0) rs[i] = pow2_for(m);
   }
   __syncthreads();
-  // 3. Targets: wave w takes targets w, w + 16.
-  {
-    constexpr int kTargetSteps = (kMaxTargets + kLayoutWaves - 1) / kLayoutWaves;
-    float v[kTargetSteps][kColSteps];
-#pragma unroll
-    for (int q = 0; q < kTargetSteps; ++q)
-#pragma unroll
-      for (int k = 0; k < kColSteps; ++k) {
-        const int t = wave + kLayoutWaves * q, i = lane + 64 * k;
-        v[q][k] = (t < T && i < n) ? a.target_white[static_cast<std::size_t>(t) * n + i] : 0.f;
-      }
-#pragma unroll
-    for (int q = 0; q < kTargetSteps; ++q) {
-      const int t = wave + kLayoutWaves * q;
-      float m = 0.f;
-#pragma unroll
-      for (int k = 0; k < kColSteps; ++k) {
-        const int i = lane + 64 * k;
-        if (t < T && i < n) m = fmaxf(m, fabsf(v[q][k] * ri[i]));
-      }
-      m = wave_max(m);
-      if (lane == 0 && t < kMaxTargets) ti[t] = 1.f / pow2_for(m);
-    }
-  }
-  __syncthreads();
-  for (int b = tid; b < kMaxBands; b += kLayoutThreads) {
+  for (int b = tid; b < kMaxBands; b += kScaleThreads) {
     a.col_scale[b] = cs[b];
-    a.row_inv[b] = ri[b];
+    a.row_inv[b] = 1.f / rs[b];
   }
-  for (int t = tid; t < kMaxTargets; t += kLayoutThreads) a.target_inv[t] = ti[t];
-  // 4. The split blocks.
-#pragma unroll 4
-  for (int e0 = tid; e0 < nwhite + ntarget; e0 += kLayoutThreads) {
-    if (e0 < nwhite) {
-      const int e = e0;
-      const int blk = e / 512;
-      const int ln = (e % 512) / 8;
-      const int i = e % 8;
-      int I = 0;
-      while (block_prefix(I + 1) <= blk) ++I;
-      const int J = blk - block_prefix(I);
-      const int row = 16 * I + (ln & 15);
-      const int col = 32 * J + 8 * (ln >> 4) + i;
-      float v = 0.f;
-      if (row < n && col <= row) v = a.whitener[static_cast<std::size_t>(row) * n + col] / cs[col] / ri[row];
-      const _Float16 h = static_cast<_Float16>(v);
-      a.white_hl[blk * 1024 + ln * 8 + i] = h;
-      a.white_hl[blk * 1024 + 512 + ln * 8 + i] = static_cast<_Float16>(v - static_cast<float>(h));
-    } else {
-      const int e = e0 - nwhite;
-      const int nb = rx_blocks(n);
-      const int npair = (nb + 1) / 2;
-      const int tt = e / (npair * 512);
-      const int P = (e / 512) % npair;
-      const int ln = (e % 512) / 8;
-      const int i = e % 8;
-      const int t = 16 * tt + (ln & 15);
-      const int band = 32 * P + (i < 4 ? 4 * (ln >> 4) + i : 16 + 4 * (ln >> 4) + (i - 4));
-      float v = 0.f;
-      if (t < T && band < n) v = a.target_white[static_cast<std::size_t>(t) * n + band] * ri[band] / ti[t];
-      const _Float16 h = static_cast<_Float16>(v);
-      const int base = (tt * npair + P) * 1024;
-      a.ace_hl[base + ln * 8 + i] = h;
-      a.ace_hl[base + 512 + ln * 8 + i] = static_cast<_Float16>(v - static_cast<float>(h));
-    }
+  for (int t = wave; t < kMaxTargets; t += kWaves) {
+    float m = 0.f;
+    if (t < T)
+      for (int i = lane; i < n; i += 64) m = fmaxf(m, fabsf(a.target_white[static_cast<std::size_t>(t) * n + i] / rs[i]));
+    m = wave_max(m);
+    if (lane == 0) a.target_inv[t] = 1.f / pow2_for(m);
+  }
+}
+
+// Whitener blocks: [block][hi|lo][lane][8], value W''[16I + l%16][32J + 8(l/16) + i]
+// with W'' = W / s (columns) * r (rows). Target blocks: [tt][P][hi|lo][lane][8],
+// value c_t * U[t][band] / r_band with band = 32P + 4(l/16) + i for i < 4 and
+// 32P + 16 + 4(l/16) + (i-4) above. One element (its hi and lo halves) per
+// thread over both tables.
+__global__ __launch_bounds__(256) void mixed_blocks_kernel(MixedLayoutArgs a, int nwhite, int ntarget) {
+  const int n = a.bands;
+  const int T = a.num_targets;
+  const int e0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e0 < nwhite) {
+    const int e = e0;
+    const int blk = e / 512;
+    const int lane = (e % 512) / 8;
+    const int i = e % 8;
+    int I = 0;
+    while (block_prefix(I + 1) <= blk) ++I;
+    const int J = blk - block_prefix(I);
+    const int row = 16 * I + (lane & 15);
+    const int col = 32 * J + 8 * (lane >> 4) + i;
+    float v = 0.f;
+    if (row < n && col <= row)
+      v = a.whitener[static_cast<std::size_t>(row) * n + col] / a.col_scale[col] / a.row_inv[row];
+    const _Float16 h = static_cast<_Float16>(v);
+    a.white_hl[blk * 1024 + lane * 8 + i] = h;
+    a.white_hl[blk * 1024 + 512 + lane * 8 + i] = static_cast<_Float16>(v - static_cast<float>(h));
+  } else if (e0 < nwhite + ntarget) {
+    const int e = e0 - nwhite;
+    const int nb = rx_blocks(n);
+    const int npair = (nb + 1) / 2;
+    const int tt = e / (npair * 512);
+    const int P = (e / 512) % npair;
+    const int lane = (e % 512) / 8;
+    const int i = e % 8;
+    const int t = 16 * tt + (lane & 15);
+    const int band = 32 * P + (i < 4 ? 4 * (lane >> 4) + i : 16 + 4 * (lane >> 4) + (i - 4));
+    float v = 0.f;
+    if (t < T && band < n)
+      v = a.target_white[static_cast<std::size_t>(t) * n + band] * a.row_inv[band] / a.target_inv[t];
+    const _Float16 h = static_cast<_Float16>(v);
+    const int base = (tt * npair + P) * 1024;
+    a.ace_hl[base + lane * 8 + i] = h;
+    a.ace_hl[base + 512 + lane * 8 + i] = static_cast<_Float16>(v - static_cast<float>(h));
   }
 }
 
@@ -761,7 +727,11 @@ hipError_t launch_mixed_layouts(const MixedLayoutArgs& a, hipStream_t stream) {
   const int nb = rx_blocks(a.bands);
   const int nwhite = block_prefix(nb) * 512;
   const int ntarget = (a.num_targets > 0 && a.ace_hl) ? (a.num_targets > 16 ? 2 : 1) * ((nb + 1) / 2) * 512 : 0;
-  hipLaunchKernelGGL(mixed_layouts_kernel, dim3(1), dim3(kLayoutThreads), 0, stream, a, nwhite, ntarget);
+  hipLaunchKernelGGL(mixed_scales_kernel, dim3(1), dim3(kScaleThreads), 0, stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(mixed_blocks_kernel, dim3(static_cast<unsigned>(div_up(nwhite + ntarget, 256))), dim3(256), 0,
+                     stream, a, nwhite, ntarget);
   return hipGetLastError();
 }
 
