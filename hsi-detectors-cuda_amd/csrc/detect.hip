@@ -87,29 +87,110 @@ __device__ __forceinline__ bool passes(const float* __restrict__ score, std::siz
 
 // Suppression test against a workgroup's neighbourhood staged in LDS: row
 // dy + R of `nb` holds the linear run p0 + dy * width - R .. p0 + dy *
-// width + kDetBlock - 1 + R, so pixel p0 + t finds neighbour (dy, dx) at
-// nb[dy + R][t + R + dx]. A run may wrap across image rows; the coordinate
-// test rejects every neighbour that is not really inside the image.
-template <int R>
-__device__ __forceinline__ bool local_best_lds(const float (*nb)[kDetBlock + 2 * R], int t, std::size_t p, int width,
-                                               int height, float self, bool higher) {
-  const int x = static_cast<int>(p % static_cast<std::size_t>(width));
-  const int y = static_cast<int>(p / static_cast<std::size_t>(width));
-  bool ok = true;
+// width + SPAN - 1 + R, so the pixel at span position t finds neighbour
+// (dy, dx) at nb[dy + R][t + R + dx]. A run may wrap across image rows; the
+// coordinate test rejects every neighbour that is not really inside the
+// image. A neighbour suppresses when it is strictly better, or equal and
+// earlier in raster order (before: dy < 0, or dy == 0 and dx < 0), so each
+// neighbour is one comparison: >= / <= before p, > / < after it (a NaN
+// neighbour compares false either way and never suppresses, as in
+// detect_cpu).
+template <int R, bool HI, int RUN>
+__device__ __forceinline__ bool local_best_lds(const float (*nb)[RUN], int t, int x, int y, int width, int height,
+                                               float self) {
+  bool row_in[2 * R + 1], col_in[2 * R + 1];
+#pragma unroll
+  for (int d = -R; d <= R; ++d) {
+    row_in[d + R] = static_cast<unsigned>(y + d) < static_cast<unsigned>(height);
+    col_in[d + R] = static_cast<unsigned>(x + d) < static_cast<unsigned>(width);
+  }
+  bool suppressed = false;
 #pragma unroll
   for (int dy = -R; dy <= R; ++dy)
 #pragma unroll
     for (int dx = -R; dx <= R; ++dx) {
       if (dx == 0 && dy == 0) continue;
-      const int ny = y + dy, nx = x + dx;
-      const bool inside = ny >= 0 && ny < height && nx >= 0 && nx < width;
-      const float other = nb[dy + R][t + R + dx];
+      const float o = nb[dy + R][t + R + dx];
       const bool before = dy < 0 || (dy == 0 && dx < 0);
-      if (inside && (better(other, self, higher) || (other == self && before))) ok = false;
+      const bool beats = before ? (HI ? o >= self : o <= self) : (HI ? o > self : o < self);
+      suppressed = suppressed || (row_in[dy + R] && col_in[dx + R] && beats);
     }
-  return ok;
+  return !suppressed;
 }
 
+// Radii 1-3 (compile-time R) on frames whose linear indices, with a
+// neighbourhood's reach, fit in 32 bits. One workgroup takes kFlagGroup
+// consecutive detect blocks (1024 pixels), so the grid is one round of
+// resident workgroups, and one trip to memory: the (2R + 1) runs of its
+// neighbourhood are loaded coalesced with every load in flight at once
+// (each pixel's own score is in the middle run), then the lanes over the
+// threshold take the suppression test. Masks and counts are written per
+// detect block, exactly as one workgroup per block would write them.
+constexpr int kFlagGroup = 4;
+
+template <int R, bool HI>
+__global__ __launch_bounds__(kDetBlock) void detect_flag_fast_kernel(
+    const float* __restrict__ score, int width, int height, float threshold, int blocks,
+    unsigned long long* __restrict__ masks, unsigned* __restrict__ block_counts) {
+  constexpr int SPAN = kDetBlock * kFlagGroup;
+  constexpr int RUN = SPAN + 2 * R;
+  constexpr int STAGED = (2 * R + 1) * RUN;
+  constexpr int PER = (STAGED + kDetBlock - 1) / kDetBlock;
+  constexpr int WAVES = kDetBlock / kWave;
+  __shared__ float nb[2 * R + 1][RUN];
+  __shared__ unsigned wave_counts[kFlagGroup][WAVES];
+  const int pixels = width * height;
+  const int p0 = static_cast<int>(blockIdx.x) * SPAN;
+  const int t = threadIdx.x;
+  {
+    float v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = t + k * kDetBlock;
+      const int row = e / RUN;
+      const int q = p0 + (row - R) * width + (e - row * RUN - R);
+      const bool ok = e < STAGED && q >= 0 && q < pixels;
+      v[k] = score[ok ? q : 0];
+      if (!ok) v[k] = 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = t + k * kDetBlock;
+      if (e < STAGED) nb[e / RUN][e - (e / RUN) * RUN] = v[k];
+    }
+  }
+  __syncthreads();
+  const int lane = t & 63;
+  const int wave = t >> 6;
+#pragma unroll
+  for (int s = 0; s < kFlagGroup; ++s) {
+    const int tt = s * kDetBlock + t;
+    const int p = p0 + tt;
+    const float self = nb[R][tt + R];
+    bool pass = false;
+    if (p < pixels && (HI ? self >= threshold : self <= threshold)) {
+      const int y = p / width;
+      pass = local_best_lds<R, HI, RUN>(nb, tt, p - y * width, y, width, height, self);
+    }
+    const unsigned long long ballot = __ballot(pass);
+    if (lane == 0) {
+      const int blk = static_cast<int>(blockIdx.x) * kFlagGroup + s;
+      if (blk < blocks) masks[static_cast<std::size_t>(blk) * WAVES + wave] = ballot;
+      wave_counts[s][wave] = static_cast<unsigned>(__popcll(ballot));
+    }
+  }
+  __syncthreads();
+  if (t < kFlagGroup) {
+    const int blk = static_cast<int>(blockIdx.x) * kFlagGroup + t;
+    unsigned total = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) total += wave_counts[t][w];
+    if (blk < blocks) block_counts[blk] = total;
+  }
+}
+
+// Any radius, any frame size: 64-bit indices, neighbours read from global
+// memory (every load of a compile-time radius issued before any compare).
 template <int R>
 __global__ __launch_bounds__(kDetBlock) void detect_flag_kernel(
     const float* __restrict__ score, int width, int height, float threshold, int radius,
@@ -117,26 +198,7 @@ __global__ __launch_bounds__(kDetBlock) void detect_flag_kernel(
   __shared__ unsigned wave_counts[kDetBlock / kWave];
   const std::size_t pixels = static_cast<std::size_t>(width) * height;
   const std::size_t p = blockIdx.x * static_cast<std::size_t>(kDetBlock) + threadIdx.x;
-  bool pass;
-  if constexpr (R > 0) {
-    // The (2R + 1) runs of the workgroup's neighbourhood, loaded coalesced
-    // (about 2R + 1 loads per thread instead of (2R + 1)^2 per candidate).
-    constexpr int RUN = kDetBlock + 2 * R;
-    __shared__ float nb[2 * R + 1][RUN];
-    const long long p0 = static_cast<long long>(blockIdx.x) * kDetBlock;
-    for (int e = threadIdx.x; e < (2 * R + 1) * RUN; e += kDetBlock) {
-      const int row = e / RUN, col = e % RUN;
-      const long long q = p0 + static_cast<long long>(row - R) * width + (col - R);
-      nb[row][col] = (q >= 0 && q < static_cast<long long>(pixels)) ? score[q] : 0.f;
-    }
-    __syncthreads();
-    const float self = nb[R][threadIdx.x + R];
-    const bool hi = higher != 0;
-    pass = p < pixels && (hi ? self >= threshold : self <= threshold) &&
-           local_best_lds<R>(nb, threadIdx.x, p, width, height, self, hi);
-  } else {
-    pass = p < pixels && passes<R>(score, p, width, height, threshold, radius, higher != 0);
-  }
+  const bool pass = p < pixels && passes<R>(score, p, width, height, threshold, radius, higher != 0);
   const unsigned long long ballot = __ballot(pass);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -159,10 +221,27 @@ __global__ __launch_bounds__(kScanThreads) void detect_scan_kernel(
   __shared__ unsigned partial[kScanThreads];
   const unsigned per = (blocks + kScanThreads - 1) / kScanThreads;
   const unsigned begin = threadIdx.x * per;
+  // Up to kScanHeld counts per thread (8192 blocks: frames of up to 2M
+  // pixels) stay in registers, all loaded at once; beyond that the thread
+  // walks its run twice.
+  constexpr unsigned kScanHeld = 8;
+  unsigned held[kScanHeld];
   unsigned local = 0;
-  for (unsigned i = 0; i < per; ++i) {
-    const unsigned b = begin + i;
-    if (b < blocks) local += block_counts[b];
+  if (per <= kScanHeld) {
+#pragma unroll
+    for (unsigned i = 0; i < kScanHeld; ++i) {
+      const unsigned b = begin + i;
+      const bool ok = i < per && b < blocks;
+      held[i] = block_counts[ok ? b : 0];
+      if (!ok) held[i] = 0;
+    }
+#pragma unroll
+    for (unsigned i = 0; i < kScanHeld; ++i) local += held[i];
+  } else {
+    for (unsigned i = 0; i < per; ++i) {
+      const unsigned b = begin + i;
+      if (b < blocks) local += block_counts[b];
+    }
   }
   // Inclusive scan of the thread sums: within each wave by shuffles, then
   // over the wave totals by the first wave (two barriers in all).
@@ -187,11 +266,20 @@ __global__ __launch_bounds__(kScanThreads) void detect_scan_kernel(
   __syncthreads();
   const unsigned inclusive = v + (wave > 0 ? partial[wave - 1] : 0u);
   unsigned run = inclusive - local;
-  for (unsigned i = 0; i < per; ++i) {
-    const unsigned b = begin + i;
-    if (b < blocks) {
-      block_offsets[b] = run;
-      run += block_counts[b];
+  if (per <= kScanHeld) {
+#pragma unroll
+    for (unsigned i = 0; i < kScanHeld; ++i) {
+      const unsigned b = begin + i;
+      if (i < per && b < blocks) block_offsets[b] = run;
+      run += held[i];
+    }
+  } else {
+    for (unsigned i = 0; i < per; ++i) {
+      const unsigned b = begin + i;
+      if (b < blocks) {
+        block_offsets[b] = run;
+        run += block_counts[b];
+      }
     }
   }
   if (threadIdx.x == kScanThreads - 1) *total_out = inclusive;
@@ -255,22 +343,45 @@ hipError_t launch_detect(const float* score, const int* target, int height, int 
   unsigned* offsets = counts + blocks;
   const int r = params.nms_radius;
   const dim3 grid(static_cast<unsigned>(blocks)), block(kDetBlock);
-  switch (r) {
-    case 1:
-      hipLaunchKernelGGL(detect_flag_kernel<1>, grid, block, 0, stream, score, width, height,
-                         params.threshold, r, params.higher_is_better, masks, counts);
-      break;
-    case 2:
-      hipLaunchKernelGGL(detect_flag_kernel<2>, grid, block, 0, stream, score, width, height,
-                         params.threshold, r, params.higher_is_better, masks, counts);
-      break;
-    case 3:
-      hipLaunchKernelGGL(detect_flag_kernel<3>, grid, block, 0, stream, score, width, height,
-                         params.threshold, r, params.higher_is_better, masks, counts);
-      break;
-    default:
-      hipLaunchKernelGGL(detect_flag_kernel<0>, grid, block, 0, stream, score, width, height,
-                         params.threshold, r, params.higher_is_better, masks, counts);
+  // The 32-bit form holds every index it forms (the last workgroup's reach
+  // included) below 2^31.
+  const std::size_t fast_groups = div_up(blocks, kFlagGroup);
+  const bool fast = r >= 1 && r <= 3 &&
+                    fast_groups * kFlagGroup * kDetBlock +
+                            static_cast<std::size_t>(r + 1) * static_cast<std::size_t>(width) + r <
+                        (std::size_t{1} << 31);
+  const bool hi = params.higher_is_better != 0;
+  if (fast) {
+#define HSI_FLAG_FAST(R, HI)                                                                           \
+  hipLaunchKernelGGL((detect_flag_fast_kernel<R, HI>), dim3(static_cast<unsigned>(fast_groups)), block, 0, \
+                     stream, score, width, height, params.threshold, static_cast<int>(blocks), masks, counts)
+    switch (r * 2 + (hi ? 1 : 0)) {
+      case 2: HSI_FLAG_FAST(1, false); break;
+      case 3: HSI_FLAG_FAST(1, true); break;
+      case 4: HSI_FLAG_FAST(2, false); break;
+      case 5: HSI_FLAG_FAST(2, true); break;
+      case 6: HSI_FLAG_FAST(3, false); break;
+      default: HSI_FLAG_FAST(3, true); break;
+    }
+#undef HSI_FLAG_FAST
+  } else {
+    switch (r) {
+      case 1:
+        hipLaunchKernelGGL(detect_flag_kernel<1>, grid, block, 0, stream, score, width, height,
+                           params.threshold, r, params.higher_is_better, masks, counts);
+        break;
+      case 2:
+        hipLaunchKernelGGL(detect_flag_kernel<2>, grid, block, 0, stream, score, width, height,
+                           params.threshold, r, params.higher_is_better, masks, counts);
+        break;
+      case 3:
+        hipLaunchKernelGGL(detect_flag_kernel<3>, grid, block, 0, stream, score, width, height,
+                           params.threshold, r, params.higher_is_better, masks, counts);
+        break;
+      default:
+        hipLaunchKernelGGL(detect_flag_kernel<0>, grid, block, 0, stream, score, width, height,
+                           params.threshold, r, params.higher_is_better, masks, counts);
+    }
   }
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
