@@ -45,15 +45,17 @@ __host__ __device__ constexpr int block_prefix(int I) {
 // MFMAs are issued for 2 row blocks at a time; 8 waves (256 registers each)
 // rather than 16 (128 each) keeps the 224-band kernels free of spills (2.2x
 // faster). S = 2: two tiles per step, each whitener fragment read from LDS
-// serving both, S = 1's waves, loads streamed one K block ahead, one row
-// block per fragment group. At 1024 x 1024 px S = 2 is the faster form
+// serving both, S = 1's waves, loads streamed one K block ahead, two row
+// blocks per fragment group (four accumulators between dependent MFMAs:
+// 224-band Q16 ACE 254 -> 249 us against one row block, the same results
+// bit for bit). At 1024 x 1024 px S = 2 is the faster form
 // except for ACE on FP32 cubes below 10 row blocks (224 bands, Q16: ACE 258
 // against 296 us, RX 205 against 248 us; 113 bands, FP32 ACE: 180 against
 // 166 us). A four-wave two-tile form with the whole next step's loads in
 // flight was slower at both band counts.
 __host__ __device__ constexpr int mixed_tp(int S) { return S == 1 ? 1 : 2; }
 __host__ __device__ constexpr int mixed_waves(int NB, int S) { return NB > 8 ? 8 : 4; }
-__host__ __device__ constexpr int mixed_group(int NB, int S) { return S == 2 ? 1 : (NB > 10 ? 2 : 4); }
+__host__ __device__ constexpr int mixed_group(int NB, int S) { return S == 2 ? 2 : (NB > 10 ? 2 : 4); }
 __host__ __device__ constexpr int mixed_prefetch(int NB, int S) { return S == 2 || NB > 8 ? 1 : (NB + 1) / 2; }
 
 // Prefetched raw inputs of one band, kept packed until use: one element
