@@ -73,7 +73,8 @@ def parse():
     ap.add_argument("--separate-sam", action="store_true",
                     help="score SAM on its own stream (before the statistics) instead of fused with CEM")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-rows", type=int, default=256, help="rows of a frame in the frame-step CPU sample")
+    ap.add_argument("--cpu-rows", type=int, default=H_,
+                    help="rows of a frame in the frame-step CPU sample (default: the whole frame)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-scene batched leg")
     ap.add_argument("--learned-frames", default="3,3", metavar="TRAIN,HELD_OUT",

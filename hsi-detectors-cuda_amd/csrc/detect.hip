@@ -12,6 +12,11 @@
 //   3. write: each survivor's slot is its workgroup offset plus the
 //             popcount of the survivors before it, so detections land in
 //             raster order; slots past capacity are counted, not written.
+// For radii 1-3 the flag pass takes four blocks per workgroup and counts per
+// group of four, and on frames of up to 2M pixels the write pass finds each
+// group's offset itself (the sum of the counts before it), so a detection is
+// two launches: on this hardware a launch costs ~4 us even when it does
+// nothing, which is most of a detection's time.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -124,14 +129,18 @@ __device__ __forceinline__ bool local_best_lds(const float (*nb)[RUN], int t, in
 // resident workgroups, and one trip to memory: the (2R + 1) runs of its
 // neighbourhood are loaded coalesced with every load in flight at once
 // (each pixel's own score is in the middle run), then the lanes over the
-// threshold take the suppression test. Masks and counts are written per
-// detect block, exactly as one workgroup per block would write them.
+// threshold take the suppression test. The masks are written per detect
+// block, as one workgroup per block would write them, and the count once per
+// workgroup (group_counts[group]), for detect_write_group_kernel.
 constexpr int kFlagGroup = 4;
+// Frames of up to this many flag groups (2M pixels) rank without a separate
+// scan pass: each write workgroup sums the counts of the groups before it.
+constexpr int kMaxRankGroups = 2048;
 
 template <int R, bool HI>
 __global__ __launch_bounds__(kDetBlock) void detect_flag_fast_kernel(
     const float* __restrict__ score, int width, int height, float threshold, int blocks,
-    unsigned long long* __restrict__ masks, unsigned* __restrict__ block_counts) {
+    unsigned long long* __restrict__ masks, unsigned* __restrict__ group_counts) {
   constexpr int SPAN = kDetBlock * kFlagGroup;
   constexpr int RUN = SPAN + 2 * R;
   constexpr int STAGED = (2 * R + 1) * RUN;
@@ -180,12 +189,91 @@ __global__ __launch_bounds__(kDetBlock) void detect_flag_fast_kernel(
     }
   }
   __syncthreads();
-  if (t < kFlagGroup) {
-    const int blk = static_cast<int>(blockIdx.x) * kFlagGroup + t;
+  if (t == 0) {
     unsigned total = 0;
 #pragma unroll
-    for (int w = 0; w < WAVES; ++w) total += wave_counts[t][w];
-    if (blk < blocks) block_counts[blk] = total;
+    for (int s = 0; s < kFlagGroup; ++s)
+#pragma unroll
+      for (int w = 0; w < WAVES; ++w) total += wave_counts[s][w];
+    group_counts[blockIdx.x] = total;
+  }
+}
+
+// Raster-order write for the flag groups above. Without a scan pass
+// (group_offsets null, at most kMaxRankGroups groups) the workgroup of group
+// g finds its first slot as the sum of the counts of groups 0 .. g-1, all
+// loaded at once, and the last group writes the total found; otherwise the
+// scan pass has written each group's first slot and the total. Each
+// survivor's slot adds the popcounts of the group's mask words before it. A
+// group with no survivor stops after reading its masks.
+__global__ __launch_bounds__(kDetBlock) void detect_write_group_kernel(
+    const float* __restrict__ score, const int* __restrict__ target, int width, int height, int blocks,
+    const unsigned long long* __restrict__ masks, const unsigned* __restrict__ group_counts,
+    const unsigned* __restrict__ group_offsets, int capacity, hsi_detection* __restrict__ out,
+    unsigned* __restrict__ total_out) {
+  constexpr int WAVES = kDetBlock / kWave;
+  constexpr int WORDS = kFlagGroup * WAVES;             // mask words of a group
+  constexpr int PER = kMaxRankGroups / kDetBlock;       // group counts per thread
+  __shared__ unsigned long long words[WORDS];
+  __shared__ unsigned wave_sum[WAVES];
+  const int groups = static_cast<int>(gridDim.x);
+  const int g = static_cast<int>(blockIdx.x);
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  if (t < WORDS) {
+    const int blk = g * kFlagGroup + t / WAVES;
+    words[t] = blk < blocks ? masks[static_cast<std::size_t>(blk) * WAVES + t % WAVES] : 0ull;
+  }
+  __syncthreads();
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < WORDS; ++w) any = any || words[w] != 0ull;
+  const bool last = g == groups - 1 && !group_offsets;
+  if (!any && !last) return;
+  unsigned first = 0;
+  if (group_offsets) {
+    first = group_offsets[g];
+  } else {
+    // first slot of the group: sum of the counts before it
+    unsigned c[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = t + k * kDetBlock;
+      const bool ok = i < g;
+      c[k] = group_counts[ok ? i : 0];
+      if (!ok) c[k] = 0u;
+    }
+    unsigned local = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) local += c[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off, kWave);
+    if (lane == 0) wave_sum[wave] = local;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) first += wave_sum[w];
+    if (last && t == 0) *total_out = first + group_counts[g];
+  }
+  if (!any) return;
+  const std::size_t pixels = static_cast<std::size_t>(width) * height;
+  unsigned before = first;  // survivors of the words before word w
+#pragma unroll
+  for (int w = 0; w < WORDS; ++w) {
+    const unsigned long long m = words[w];
+    if (w % WAVES == wave && ((m >> lane) & 1ull)) {
+      const unsigned rank = before + static_cast<unsigned>(__popcll(m & ((1ull << lane) - 1ull)));
+      const std::size_t p = static_cast<std::size_t>(g) * (kDetBlock * kFlagGroup) + static_cast<std::size_t>(w) * kWave + lane;
+      if (rank < static_cast<unsigned>(capacity) && p < pixels) {
+        hsi_detection d;
+        d.x = static_cast<int>(p % static_cast<std::size_t>(width));
+        d.y = static_cast<int>(p / static_cast<std::size_t>(width));
+        d.target = target ? target[p] : 0;
+        d.score = score[p];
+        out[rank] = d;
+      }
+    }
+    before += static_cast<unsigned>(__popcll(m));
   }
 }
 
@@ -352,6 +440,8 @@ hipError_t launch_detect(const float* score, const int* target, int height, int 
                         (std::size_t{1} << 31);
   const bool hi = params.higher_is_better != 0;
   if (fast) {
+    // flag groups of four blocks, then (past kMaxRankGroups groups) a scan of
+    // the group counts, then the write
 #define HSI_FLAG_FAST(R, HI)                                                                           \
   hipLaunchKernelGGL((detect_flag_fast_kernel<R, HI>), dim3(static_cast<unsigned>(fast_groups)), block, 0, \
                      stream, score, width, height, params.threshold, static_cast<int>(blocks), masks, counts)
@@ -364,24 +454,36 @@ hipError_t launch_detect(const float* score, const int* target, int height, int 
       default: HSI_FLAG_FAST(3, true); break;
     }
 #undef HSI_FLAG_FAST
-  } else {
-    switch (r) {
-      case 1:
-        hipLaunchKernelGGL(detect_flag_kernel<1>, grid, block, 0, stream, score, width, height,
-                           params.threshold, r, params.higher_is_better, masks, counts);
-        break;
-      case 2:
-        hipLaunchKernelGGL(detect_flag_kernel<2>, grid, block, 0, stream, score, width, height,
-                           params.threshold, r, params.higher_is_better, masks, counts);
-        break;
-      case 3:
-        hipLaunchKernelGGL(detect_flag_kernel<3>, grid, block, 0, stream, score, width, height,
-                           params.threshold, r, params.higher_is_better, masks, counts);
-        break;
-      default:
-        hipLaunchKernelGGL(detect_flag_kernel<0>, grid, block, 0, stream, score, width, height,
-                           params.threshold, r, params.higher_is_better, masks, counts);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    const bool scan = fast_groups > static_cast<std::size_t>(kMaxRankGroups);
+    if (scan) {
+      hipLaunchKernelGGL(detect_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, counts, offsets,
+                         static_cast<unsigned>(fast_groups), count);
+      err = hipGetLastError();
+      if (err != hipSuccess) return err;
     }
+    hipLaunchKernelGGL(detect_write_group_kernel, dim3(static_cast<unsigned>(fast_groups)), block, 0, stream, score,
+                       target, width, height, static_cast<int>(blocks), masks, counts, scan ? offsets : nullptr,
+                       params.max_detections, out, count);
+    return hipGetLastError();
+  }
+  switch (r) {
+    case 1:
+      hipLaunchKernelGGL(detect_flag_kernel<1>, grid, block, 0, stream, score, width, height,
+                         params.threshold, r, params.higher_is_better, masks, counts);
+      break;
+    case 2:
+      hipLaunchKernelGGL(detect_flag_kernel<2>, grid, block, 0, stream, score, width, height,
+                         params.threshold, r, params.higher_is_better, masks, counts);
+      break;
+    case 3:
+      hipLaunchKernelGGL(detect_flag_kernel<3>, grid, block, 0, stream, score, width, height,
+                         params.threshold, r, params.higher_is_better, masks, counts);
+      break;
+    default:
+      hipLaunchKernelGGL(detect_flag_kernel<0>, grid, block, 0, stream, score, width, height,
+                         params.threshold, r, params.higher_is_better, masks, counts);
   }
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
