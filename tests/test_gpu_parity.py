@@ -473,6 +473,32 @@ def test_detect_large_ragged_map(H, torch, ctx, oracle_lib, nms, higher):
     assert same_bits(dets["score"], want["score"])
 
 
+@pytest.mark.parametrize("h,w,nms,higher", [(2048, 1024, 1, True), (2049, 1024, 2, False), (2111, 997, 3, True)])
+def test_detect_group_rank_boundaries(H, torch, ctx, oracle_lib, h, w, nms, higher):
+    """For radii 1-3 the flag pass counts per 1024-pixel group and, up to
+    2048 groups, each write workgroup finds its first slot from the counts
+    of the groups before it; past that a scan over the group counts runs
+    first. 2048 x 1024 is exactly 2048 groups, 2049 x 1024 is 2049 (the
+    scan), and 2111 x 997 is 2056 with a ragged last group of 347 pixels and
+    neighbourhood runs that wrap across rows. Lists equal detect_cpu's
+    element by element, also when the capacity cuts them short."""
+    rng = np.random.default_rng(101 + nms)
+    score = (np.round(rng.random((h, w)) * 64) / 64).astype(np.float32)
+    target = rng.integers(0, 8, size=h * w).astype(np.int32)
+    thr = 0.97 if higher else 0.03
+    want, found_cpu = oracle_lib.detect(score, target, h, w, thr, nms, higher)
+    dets, found = run_detect(H, torch, ctx, score, target, h, w, thr, nms, higher)
+    assert found == found_cpu > 10000
+    for k in ("x", "y", "target"):
+        assert np.array_equal(dets[k], want[k])
+    assert same_bits(dets["score"], want["score"])
+    cap = found_cpu // 3
+    dets, found = run_detect(H, torch, ctx, score, target, h, w, thr, nms, higher, cap=cap)
+    assert found == found_cpu and len(dets) == cap
+    for k in ("x", "y", "target"):
+        assert np.array_equal(dets[k], want[k][:cap])
+
+
 def test_detect_saturates_without_overrun(H, torch, ctx, oracle_lib):
     rng = np.random.default_rng(3)
     h, w = 300, 500
