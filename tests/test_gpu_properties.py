@@ -1,0 +1,207 @@
+"""Property tests of the device detectors: the reference's own detector tests
+restated on the HIP path, through the C ABI. They hold for any correct
+implementation, independently of the oracle and of its fixtures:
+  - RX ranks planted anomalies above the whole background, and averages to the
+    band count over the background - the chi-squared expectation, which
+    catches a whitener off by a scale factor that a ranking cannot see
+    (tests/test_detectors.cpp:82-133);
+  - RX equals an explicit (x - mu)^T S^-1 (x - mu), and ACE its definition,
+    from S^-1 formed in float64 by numpy (tests/test_detector.cpp:89-225);
+  - ACE lies in [0, 1], peaks on its target, and its RX side output equals a
+    standalone RX (tests/test_detectors.cpp:135-185);
+  - CEM has unit gain on its target (w . d = 1) and suppresses the background
+    (tests/test_detectors.cpp:187-212, tests/test_detector.cpp:227-249);
+  - end to end, a generated frame's SAM detections land on planted pixels of
+    the right target, find every target, and leave the background well clear
+    of the threshold (tests/test_sources.cpp:83-144).
+RX and ACE run in the plain FP64 form, the FP32 form and the mixed (FP16
+hi/lo) form, each against its own bar: 1e-4 relative for the first two, as
+the reference's tests, and 1e-3 for the mixed form.
+
+Scenes: a correlated background (a ramp, three smooth spectral factors with
+random amplitudes, white noise) with a few planted pixels of a spectrum
+unlike it (a falling ramp with a narrow bump)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ["plain", "fp32", "mixed"]
+BAR = {"plain": 1e-4, "fp32": 1e-4, "mixed": 1e-3}
+
+
+@pytest.fixture(scope="module")
+def H():
+    import torch
+    import hsi_mi355x
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return hsi_mi355x
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    return t
+
+
+@pytest.fixture()
+def ctx(H):
+    c = H.Context(0)
+    yield c
+    c.close()
+
+
+def correlated_scene(h, w, bands, seed, planted=3):
+    rng = np.random.default_rng(seed)
+    p = h * w
+    t = np.linspace(0.0, 1.0, bands)
+    mean = 0.25 + 0.2 * t
+    factors = np.stack([np.cos(np.pi * k * t) for k in (1, 2, 3)])
+    amp = rng.normal(0.0, 1.0, (3, p)) * np.array([0.04, 0.025, 0.015])[:, None]
+    cube = mean[:, None] + factors.T @ amp + rng.normal(0.0, 0.008, (bands, p))
+    target = 0.7 - 0.5 * t + 0.05 * np.exp(-((t - 0.5) / 0.08) ** 2)
+    where = rng.choice(p, planted, replace=False)
+    cube[:, where] = target[:, None] + rng.normal(0.0, 0.002, (bands, planted))
+    background = np.ones(p, bool)
+    background[where] = False
+    return cube.astype(np.float32), target.astype(np.float32), mean.astype(np.float32), where, background
+
+
+def device_scene(H, ctx, cube, h, w, library, loading, form):
+    b = cube.shape[0]
+    dev = H.to_device_cube(cube, h, w)
+    ctx.set_library(np.asarray(library, np.float32))
+    ctx.stats_accumulate(dev, h, w, b, precision=H.MIXED if form == "mixed" else H.PLAIN)
+    ctx.stats_build(loading)
+    ctx.stats_check()
+    return dev
+
+
+def precision_of(H, form):
+    return {"plain": H.PLAIN, "fp32": H.FP32, "mixed": H.MIXED}[form]
+
+
+def run_rx(H, torch, ctx, dev, h, w, b, form):
+    rx = torch.empty(h * w, dtype=torch.float32, device="cuda")
+    ctx.score(H.Detector.RX, dev, h, w, b, rx, precision=precision_of(H, form))
+    torch.cuda.synchronize()
+    return rx.cpu().numpy().astype(np.float64)
+
+
+def explicit_inverse(cube, loading):
+    x = cube.astype(np.float64)
+    mu = x.mean(axis=1)
+    s = (x @ x.T) / x.shape[1] - np.outer(mu, mu) + loading * np.eye(x.shape[0])
+    return x - mu[:, None], np.linalg.inv(s), mu
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_rx_ranks_planted_anomalies_above_the_background(H, torch, ctx, form):
+    h, w, b = 40, 48, 16
+    cube, target, _, where, background = correlated_scene(h, w, b, seed=1)
+    dev = device_scene(H, ctx, cube, h, w, target[None, :], 1e-6, form)
+    rx = run_rx(H, torch, ctx, dev, h, w, b, form)
+    assert rx[where].min() > rx[background].max()
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_rx_background_mean_is_the_band_count(H, torch, ctx, form):
+    h, w, b = 60, 60, 12
+    cube, target, _, _, background = correlated_scene(h, w, b, seed=2)
+    dev = device_scene(H, ctx, cube, h, w, target[None, :], 1e-9, form)
+    mean = run_rx(H, torch, ctx, dev, h, w, b, form)[background].mean()
+    # three planted pixels inflate the covariance a little: a loose band
+    assert 0.80 * b < mean < 1.05 * b, mean
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_rx_and_ace_match_an_explicit_inverse(H, torch, ctx, form):
+    h, w, b = 40, 48, 16
+    cube, target, mean, _, _ = correlated_scene(h, w, b, seed=5)
+    library = np.stack([target, mean + 0.02])
+    loading = 1e-6
+    dev = device_scene(H, ctx, cube, h, w, library, loading, form)
+    d, s_inv, mu = explicit_inverse(cube, loading)
+    want_rx = np.einsum("ip,ij,jp->p", d, s_inv, d)
+    rx = run_rx(H, torch, ctx, dev, h, w, b, form)
+    assert np.max(np.abs(rx - want_rx) / np.maximum(np.abs(want_rx), 1.0)) <= BAR[form]
+    # ACE of target t: (s^T S^-1 x)^2 / ((s^T S^-1 s)(x^T S^-1 x)), s = d_t - mu, best over t
+    s = library.astype(np.float64).T - mu[:, None]
+    num = (s.T @ s_inv @ d) ** 2
+    den = np.einsum("it,ij,jt->t", s, s_inv, s)[:, None] * want_rx[None, :]
+    want_ace = (num / den).max(axis=0)
+    ace = torch.empty(h * w, dtype=torch.float32, device="cuda")
+    which = torch.empty(h * w, dtype=torch.int32, device="cuda")
+    side = torch.empty(h * w, dtype=torch.float32, device="cuda")
+    ctx.score(H.Detector.ACE, dev, h, w, b, ace, which, side, precision=precision_of(H, form))
+    torch.cuda.synchronize()
+    assert np.max(np.abs(ace.cpu().numpy().astype(np.float64) - want_ace)) <= BAR[form]
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_ace_is_bounded_peaks_on_its_target_and_shares_rx(H, torch, ctx, form):
+    h, w, b = 40, 48, 14
+    cube, target, mean, where, _ = correlated_scene(h, w, b, seed=3)
+    # a decoy that looks like the background, for the best-match reduction to reject
+    dev = device_scene(H, ctx, cube, h, w, np.stack([target, mean]), 1e-6, form)
+    p = h * w
+    ace = torch.empty(p, dtype=torch.float32, device="cuda")
+    which = torch.empty(p, dtype=torch.int32, device="cuda")
+    side = torch.empty(p, dtype=torch.float32, device="cuda")
+    ctx.score(H.Detector.ACE, dev, h, w, b, ace, which, side, precision=precision_of(H, form))
+    torch.cuda.synchronize()
+    a, t = ace.cpu().numpy(), which.cpu().numpy()
+    assert (a >= 0.0).all() and (a <= 1.0).all()
+    assert (a[where] >= 0.9).all() and (t[where] == 0).all()
+    rx_alone = run_rx(H, torch, ctx, dev, h, w, b, form)
+    assert np.max(np.abs(side.cpu().numpy().astype(np.float64) - rx_alone)) < 1e-3
+
+
+def test_cem_has_unit_gain_on_its_target(H, torch, ctx):
+    h, w, b = 40, 48, 14
+    cube, target, _, where, background = correlated_scene(h, w, b, seed=4)
+    dev = device_scene(H, ctx, cube, h, w, target[None, :], 1e-6, "plain")
+    p = h * w
+    cem = torch.empty(p, dtype=torch.float32, device="cuda")
+    which = torch.empty(p, dtype=torch.int32, device="cuda")
+    ctx.score(H.Detector.CEM, dev, h, w, b, cem, which)
+    torch.cuda.synchronize()
+    c = cem.cpu().numpy()
+    assert ((c[where] > 0.8) & (c[where] < 1.2)).all()
+    assert c[background].max() < 0.8
+    # the filter itself: w . d = 1
+    st = ctx.stats_read(b, 1)
+    gain = float(st["cem_weight"][0].astype(np.float64) @ target.astype(np.float64))
+    assert abs(gain - 1.0) < 1e-4
+
+
+def test_sam_detections_land_on_the_planted_targets(H, torch, ctx):
+    """A generated 256 x 256 x 113 frame with 4 targets planted at 0.5% each
+    (abundance 0.6-1 over the background mix): SAM at 0.05 rad with
+    suppression radius 2 detects only planted pixels, each with its own
+    target, and every target at least once; no background pixel comes within
+    twice the threshold."""
+    h = w = 256
+    b = 113
+    cfg = H.synth_config(h, w, b, 4)
+    frame, truth = H.synth_frame(cfg, 0, truth=True, threads=8)
+    library = H.synth_library(cfg)
+    p = h * w
+    cube = torch.from_numpy(frame).cuda()
+    ctx.set_library(library)
+    angle = torch.empty(p, dtype=torch.float32, device="cuda")
+    which = torch.empty(p, dtype=torch.int32, device="cuda")
+    ctx.score(H.Detector.SAM, cube, h, w, b, angle, which)
+    cap = 4096
+    out = torch.zeros((cap, 4), dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.detect(angle, which, h, w, H.DetectionParams(0.05, 2, cap, False), out, count)
+    torch.cuda.synchronize()
+    found = int(count.item())
+    assert 0 < found <= cap
+    dets = H.detections_to_numpy(out, found)
+    pix = dets["y"].astype(np.int64) * w + dets["x"]
+    assert (truth[pix] != 0).all()
+    assert (truth[pix] == dets["target"] + 1).all()
+    assert set(dets["target"].tolist()) == {0, 1, 2, 3}
+    assert angle.cpu().numpy()[truth[:p] == 0].min() > 1.5 * 0.05
