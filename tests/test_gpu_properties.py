@@ -13,7 +13,15 @@ implementation, independently of the oracle and of its fixtures:
     (tests/test_detectors.cpp:187-212, tests/test_detector.cpp:227-249);
   - end to end, a generated frame's SAM detections land on planted pixels of
     the right target, find every target, and leave the background well clear
-    of the threshold (tests/test_sources.cpp:83-144).
+    of the threshold (tests/test_sources.cpp:83-144);
+  - SAM: a target's own spectrum, at any gain, scores an angle of rounding
+    size on it (the same bits at any power-of-two gain); a pixel's angle does
+    not change with its gain; disjoint spectra are pi/2 apart
+    (tests/test_sam.cpp:23-98);
+  - the device whitener W = L^-1 whitens the covariance, W S W^T = I
+    (tests/test_linalg.cpp:44-127, the Cholesky reconstruction);
+  - a library whose band count differs from the cube's is refused
+    (tests/test_detector.cpp, band mismatch).
 RX and ACE run in the plain FP64 form, the FP32 form and the mixed (FP16
 hi/lo) form, each against its own bar: 1e-4 relative for the first two, as
 the reference's tests, and 1e-3 for the mixed form.
@@ -205,3 +213,72 @@ def test_sam_detections_land_on_the_planted_targets(H, torch, ctx):
     assert (truth[pix] == dets["target"] + 1).all()
     assert set(dets["target"].tolist()) == {0, 1, 2, 3}
     assert angle.cpu().numpy()[truth[:p] == 0].min() > 1.5 * 0.05
+
+
+def test_sam_closed_forms(H, torch, ctx):
+    rng = np.random.default_rng(9)
+    b = 16
+    library = rng.uniform(0.05, 1.0, (3, b)).astype(np.float32)
+    library[2, 8:] = 0.0  # target 2 lives in bands 0-7 only
+    base = rng.uniform(0.05, 1.0, b).astype(np.float32)
+    disjoint = np.zeros(b, np.float32)
+    disjoint[8:] = rng.uniform(0.1, 1.0, 8)
+    pixels = [library[0], library[1], library[2], library[1] * np.float32(0.25), library[2] * np.float32(8.0),
+              library[1] * np.float32(3.7)]
+    gains = [1.0, 0.01, 0.5, 3.0, 250.0, 0.125, 64.0]
+    pixels += [base * np.float32(g) for g in gains]
+    pixels += [disjoint]
+    cube = np.stack(pixels, axis=1)  # [bands, pixels]
+    h, w = 1, cube.shape[1]
+    dev = H.to_device_cube(cube, h, w)
+    ctx.set_library(library)
+    angle = torch.empty(w, dtype=torch.float32, device="cuda")
+    which = torch.empty(w, dtype=torch.int32, device="cuda")
+    ctx.score(H.Detector.SAM, dev, h, w, b, angle, which)
+    torch.cuda.synchronize()
+    a, t = angle.cpu().numpy(), which.cpu().numpy()
+    # a target's own spectrum: the angle is at most the acos of a cosine one
+    # float step from 1 (the target norms are floats, as the reference's)
+    assert (a[:3] <= 5e-4).all() and list(t[:3]) == [0, 1, 2]
+    assert a[3] == a[1] and a[4] == a[2] and t[3] == 1 and t[4] == 2
+    assert a[5] <= 5e-4 and t[5] == 1
+    scaled = a[6:6 + len(gains)]
+    assert np.abs(scaled.astype(np.float64) - scaled[0]).max() <= 1e-6
+    exact = [i for i, g in enumerate(gains) if np.log2(g) == np.round(np.log2(g))]
+    assert (scaled[exact] == scaled[0]).all()  # a power-of-two gain changes no bit
+    # the disjoint pixel is orthogonal to target 2 and not to the others
+    ctx.set_library(library[2:3])
+    ctx.score(H.Detector.SAM, dev, h, w, b, angle, which)
+    torch.cuda.synchronize()
+    assert angle.cpu().numpy()[-1] == np.float32(np.pi / 2)
+
+
+@pytest.mark.parametrize("bands", [16, 113])
+def test_whitener_whitens_the_covariance(H, torch, ctx, bands):
+    h, w = 40, 48
+    cube, target, _, _, _ = correlated_scene(h, w, bands, seed=6)
+    loading = 1e-6
+    device_scene(H, ctx, cube, h, w, target[None, :], loading, "plain")
+    whitener = ctx.stats_read(bands, 1)["whitener"].astype(np.float64)
+    x = cube.astype(np.float64)
+    mu = x.mean(axis=1)
+    s = (x @ x.T) / x.shape[1] - np.outer(mu, mu) + loading * np.eye(bands)
+    assert np.allclose(np.triu(whitener, 1), 0.0)  # lower triangular
+    assert np.abs(whitener @ s @ whitener.T - np.eye(bands)).max() <= 1e-5
+
+
+def test_library_band_mismatch_is_refused(H, torch, ctx):
+    h, w, b = 8, 8, 12
+    cube, target, _, _, _ = correlated_scene(h, w, b, seed=7)
+    dev = H.to_device_cube(cube, h, w)
+    ctx.set_library(np.ones((2, b + 1), np.float32))
+    out = torch.empty(h * w, dtype=torch.float32, device="cuda")
+    idx = torch.empty(h * w, dtype=torch.int32, device="cuda")
+    with pytest.raises(Exception) as info:
+        ctx.score(H.Detector.SAM, dev, h, w, b, out, idx)
+    assert "bands" in str(info.value)
+    ctx.stats_accumulate(dev, h, w, b)
+    with pytest.raises(Exception) as info:
+        ctx.stats_build(1e-6)
+        ctx.stats_check()
+    assert "bands" in str(info.value)
