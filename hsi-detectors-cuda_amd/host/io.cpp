@@ -140,7 +140,12 @@ std::string data_path_for(const std::string& hdr) {
   std::vector<std::string> candidates;
   const std::size_t dot = hdr.rfind('.');
   const std::string stem = dot == std::string::npos ? hdr : hdr.substr(0, dot);
-  for (const char* ext : {"", ".img", ".raw", ".dat", ".bin", ".IMG", ".RAW"}) candidates.push_back(stem + ext);
+  // The extensions the reference tries, in its order (envi.cpp:184-191), so
+  // both pick the same file when several lie beside one header; then a few
+  // upper-case spellings.
+  for (const char* ext : {"", ".float", ".dat", ".img", ".bin", ".raw", ".bsq", ".bil", ".bip", ".cube", ".IMG",
+                          ".RAW"})
+    candidates.push_back(stem + ext);
   candidates.push_back(hdr + ".img");
   for (const std::string& c : candidates) {
     if (c == hdr) continue;

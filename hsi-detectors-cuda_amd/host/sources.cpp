@@ -76,14 +76,14 @@ class EnviReplaySource final : public FrameSource {
   std::uint64_t length() const override { return loop_ ? 0 : frames_.size(); }
   bool read_into(float* dst, std::size_t plane_stride, FrameMeta* meta) override {
     if (!loop_ && next_ >= frames_.size()) return false;
+    // Only the band planes are written; the elements between them are the
+    // caller's and stay as they were (frame_source.cpp's replay does the same,
+    // test_sources.cpp:196-214).
     const HsiFrame& f = frames_[next_ % frames_.size()];
     const std::size_t P = f.shape.pixels();
-    for (int b = 0; b < f.shape.bands; ++b) {
+    for (int b = 0; b < f.shape.bands; ++b)
       std::memcpy(dst + static_cast<std::size_t>(b) * plane_stride, f.data.data() + static_cast<std::size_t>(b) * P,
                   P * sizeof(float));
-      std::fill(dst + static_cast<std::size_t>(b) * plane_stride + P,
-                dst + static_cast<std::size_t>(b) * plane_stride + plane_stride, 0.0f);
-    }
     if (meta) {
       meta->index = next_;
       meta->timestamp_s = 0.0;
