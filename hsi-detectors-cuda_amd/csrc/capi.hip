@@ -104,6 +104,11 @@ hsi_status device_fail(hsi_ctx* ctx, const char* what, hipError_t err) {
   return fail(ctx, HSI_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(err));
 }
 
+// The mixed-form moments read the cube through one buffer view.
+const char* const kMixedViewLimit =
+    "the mixed-form statistics read the cube through one buffer view: bands x plane_stride elements must stay "
+    "under 4 GiB";
+
 #define HSI_HIP(ctx, expr, what)                          \
   do {                                                    \
     const hipError_t hsi_err_ = (expr);                   \
@@ -369,6 +374,8 @@ hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtyp
     return fail(ctx, HSI_ERR_INVALID_ARG,
                 "no Q16 code for " + std::to_string(bands) + " bands: encode with hsi_encode_q16 or install one "
                 "with hsi_set_q16_code");
+  if (precision == HSI_MIXED && !mixed_moments_fit(bands, plane_stride, dtype == HSI_F32 ? 4 : 2, false))
+    return fail(ctx, HSI_ERR_INVALID_ARG, kMixedViewLimit);
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
   const std::size_t pixels = static_cast<std::size_t>(height) * width;
   HSI_HIP(ctx, grow(reinterpret_cast<void**>(&ctx->d_mom_scratch), &ctx->mom_scratch_bytes,
@@ -409,6 +416,7 @@ hsi_status hsi_stats_accumulate_screen(hsi_ctx* ctx, const float* d_cube, int he
   const std::size_t pb = static_cast<std::size_t>((bands + 31) / 32 * 32);
   if (d_spectra && spectra_stride < pb)
     return fail(ctx, HSI_ERR_INVALID_ARG, "candidate spectra need a pitch >= 32 * ceil(bands / 32)");
+  if (!mixed_moments_fit(bands, plane_stride, 4, true)) return fail(ctx, HSI_ERR_INVALID_ARG, kMixedViewLimit);
   if (set_device(ctx) != HSI_OK) return HSI_ERR_DEVICE;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   if (ctx->mlp_dirty) {

@@ -85,6 +85,10 @@ struct MomentArgs {
   double* sum_outer = nullptr;
 };
 hipError_t launch_moments(const MomentArgs& args, double* scratch, hipStream_t stream);
+// Whether the mixed-form moments (screen: with the fused screening pass) can
+// read a cube of `bands` planes of `stride` elements of `elem_bytes` bytes
+// through one buffer view (under 4 GiB, with every staging offset below 2^32).
+bool mixed_moments_fit(int bands, std::size_t stride, std::size_t elem_bytes, bool screen);
 
 // The learned pre-screen fused into the mixed-form moments (F32 cube, bands
 // <= kFusedScreenMaxBands): from the same read of each pixel, the network's

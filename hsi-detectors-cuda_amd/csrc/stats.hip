@@ -127,23 +127,20 @@ __device__ __forceinline__ float4 stage_fix(float4 v, int bands, int b, std::siz
 template <typename In> struct Stage4 { typedef uint2 type; };
 template <> struct Stage4<float> { typedef float4 type; };
 
+// Four consecutive elements of the cube at byte offset `off` of its buffer
+// view (zeros at or past the view's end).
 template <typename In>
-__device__ __forceinline__ typename Stage4<In>::type stage_raw_t(const void* __restrict__ cube, std::size_t stride,
-                                                                 int bands, int b, std::size_t px,
-                                                                 std::size_t p_begin, std::size_t p_end) {
-  if constexpr (sizeof(In) == 4) {
-    return stage_raw(static_cast<const float*>(cube), stride, bands, b, px, p_begin, p_end);
-  } else {
-    const int bc = b < bands ? b : bands - 1;
-    const std::size_t pc = px < p_end ? px : p_begin;
-    return *reinterpret_cast<const uint2*>(static_cast<const std::uint16_t*>(cube) +
-                                           static_cast<std::size_t>(bc) * stride + pc);
-  }
+__device__ __forceinline__ typename Stage4<In>::type stage_buffer(__amdgpu_buffer_rsrc_t r, std::uint32_t off) {
+  if constexpr (sizeof(In) == 4)
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
-// Raw staged elements as floats; qo/qs: the Q16 code of this band row.
+// Raw staged elements as floats: F32 as read, F16 widened, Q16 as their
+// codes (the caller folds each band's decode into its per-row affine).
 template <typename In>
-__device__ __forceinline__ float4 stage_widen(typename Stage4<In>::type r, float qo, float qs) {
+__device__ __forceinline__ float4 stage_widen(typename Stage4<In>::type r) {
   if constexpr (sizeof(In) == 4) {
     return r;
   } else {
@@ -152,8 +149,7 @@ __device__ __forceinline__ float4 stage_widen(typename Stage4<In>::type r, float
     float v[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      v[i] = is_q16<In>::value ? q16_decode(static_cast<float>(e[i]), qs, qo)
-                               : static_cast<float>(__builtin_bit_cast(_Float16, e[i]));
+      v[i] = is_q16<In>::value ? static_cast<float>(e[i]) : static_cast<float>(__builtin_bit_cast(_Float16, e[i]));
     return make_float4(v[0], v[1], v[2], v[3]);
   }
 }
@@ -622,6 +618,17 @@ __device__ __forceinline__ void screen_emit_tile(float logit, const _Float16* hi
   local += static_cast<unsigned>(__popcll(m));
 }
 
+// Whether every staging read offset of the mixed-form owners (ch-pixel
+// chunks, nb blocks) stays below 2^32 bytes for planes of `stride` elements
+// of `eb` bytes: then reads of rows past the band count land at or past the
+// cube's buffer view and return zeros.
+inline bool stage_offsets_fit(int nb, int ch, std::size_t stride, std::size_t eb) {
+  const std::size_t threads = kWavesPerWg * 64, per_row = static_cast<std::size_t>(ch) / 4;
+  const std::size_t per = div_up(static_cast<std::size_t>(nb) * 16 * per_row, threads);
+  const std::size_t rows = per * (threads / per_row);  // rows a workgroup forms offsets for
+  return rows * stride * eb + 16 < (static_cast<std::size_t>(1) << 32);
+}
+
 template <typename In, int NB, int POS, int CH = f16_chunk<NB>()>
 __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cube, std::size_t p_begin,
                                                     std::size_t p_end, int bands, std::size_t stride,
@@ -648,45 +655,56 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
   }
   const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
 
+  // Staging reads go through a buffer view of the whole cube (the launchers
+  // check that it fits one). Thread t takes band rows t / (CH / 4) + k * RSTEP
+  // at pixel column 4 * (t % (CH / 4)) of every chunk, so a read's byte
+  // offset is one register plus a per-k constant, with no clamp: rows past
+  // the band count and pixels past the cube's end read as zeros, and neither
+  // they nor pixels past p_end reach the tiles (see the store).
+  static_assert(THREADS % (CH / 4) == 0, "a thread stages one pixel column of every row it takes");
+  constexpr int RSTEP = THREADS / (CH / 4);
+  constexpr std::size_t EB = sizeof(In);
+  const __amdgpu_buffer_rsrc_t view =
+      cube_rsrc(cube, static_cast<std::uint32_t>(static_cast<std::size_t>(bands) * stride * EB));
+  const std::uint32_t row0 = static_cast<std::uint32_t>(
+      (static_cast<std::size_t>(threadIdx.x / (CH / 4)) * stride + (threadIdx.x % (CH / 4)) * 4) * EB);
+  const std::uint32_t kstep = static_cast<std::uint32_t>(static_cast<std::size_t>(RSTEP) * stride * EB);
   typename Stage4<In>::type pre[PER];
   auto load = [&](std::size_t p) {
+    const std::uint32_t at = row0 + static_cast<std::uint32_t>(p * EB);
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = threadIdx.x + THREADS * k;
-      const int b = e / (CH / 4);
-      pre[k] = stage_raw_t<In>(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_begin, p_end);
-    }
+    for (int k = 0; k < PER; ++k) pre[k] = stage_buffer<In>(view, at + static_cast<std::uint32_t>(k) * kstep);
   };
   // Each thread stages the same rows of every chunk, so the per-row affine
-  // d = raw * rm + ra lives in registers: x - shift on a real band (an exact
-  // product by 1, so d is the rounded difference), 1 on the ones band, 0 on
-  // the padding rows past it.
+  // d = raw * rm + ra lives in registers: x - shift on a real band (for F32
+  // and F16 an exact product by 1, so d is the rounded difference; for Q16
+  // code * scale + (offset - shift), the decode and the shift in one
+  // rounding), 1 on the ones band, 0 on the padding rows past it.
   float rm[PER], ra[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int b = (static_cast<int>(threadIdx.x) + THREADS * k) / (CH / 4);
+    const int bc = b < ROWS ? b : 0;
     const bool real = b < bands;
-    rm[k] = real ? 1.f : 0.f;
-    ra[k] = real ? -sh[b < ROWS ? b : 0] : (b == bands ? 1.f : 0.f);
+    rm[k] = real ? (is_q16<In>::value ? qs[bc] : 1.f) : 0.f;
+    ra[k] = real ? (is_q16<In>::value ? qo[bc] - sh[bc] : -sh[bc]) : (b == bands ? 1.f : 0.f);
   }
-  auto store = [&](int buf, std::size_t p) {
-    // Pixels past the range are zeroed only in a partial chunk.
-    const bool full = p + CH <= p_end;
+  // A chunk is partial only at the end of a range; its pixels past p_end are
+  // zeroed. The two forms are separate code, so full chunks carry no masks.
+  auto store_rows = [&](int buf, std::size_t p, auto partial) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + THREADS * k;
       if (e < LOADS) {
         const int b = e / (CH / 4);
         const int k4 = (e % (CH / 4)) * 4;
-        const std::size_t px = p + k4;
-        // (b < ROWS here; the Q16 code arrays exist only for Q16 cubes)
-        const float bqo = is_q16<In>::value ? qo[b] : 0.f, bqs = is_q16<In>::value ? qs[b] : 0.f;
-        const float4 f = stage_widen<In>(pre[k], bqo, bqs);
+        const float4 f = stage_widen<In>(pre[k]);
         float d0 = __builtin_fmaf(f.x, rm[k], ra[k]);
         float d1 = __builtin_fmaf(f.y, rm[k], ra[k]);
         float d2 = __builtin_fmaf(f.z, rm[k], ra[k]);
         float d3 = __builtin_fmaf(f.w, rm[k], ra[k]);
-        if (!full) {
+        if constexpr (decltype(partial)::value) {
+          const std::size_t px = p + k4;
           const std::size_t left = p_end > px ? p_end - px : 0;
           d0 = left > 0 ? d0 : 0.f;
           d1 = left > 1 ? d1 : 0.f;
@@ -698,6 +716,12 @@ __device__ __forceinline__ void moments_mixed_owner(const void* __restrict__ cub
         *reinterpret_cast<uint2*>(lo_t + buf * TILE + b * PITCH + k4) = make_uint2(s01.y, s23.y);
       }
     }
+  };
+  auto store = [&](int buf, std::size_t p) {
+    if (p + CH <= p_end)
+      store_rows(buf, p, std::false_type{});
+    else
+      store_rows(buf, p, std::true_type{});
   };
 
   load(p_begin);
@@ -1058,6 +1082,7 @@ hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t str
                            const FusedScreenArgs* fs = nullptr) {
   constexpr bool kCanScreen = std::is_same_v<In, float> && NB <= 8;
   if (fs && !kCanScreen) return hipErrorInvalidValue;
+  if (!stage_offsets_fit(NB, fs ? kFusedChunk : f16_chunk<NB>(), m.stride, sizeof(In))) return hipErrorInvalidValue;
   const std::size_t pixels = m.pixels;
   const int bands = m.bands;
   const std::size_t chunks = div_up(pixels, kChunk);
@@ -1114,6 +1139,7 @@ hipError_t launch_mixed_nb(const MomentArgs& m, double* scratch, hipStream_t str
 
 template <typename In>
 hipError_t launch_mixed_typed(const MomentArgs& m, double* scratch, hipStream_t stream) {
+  if (!fits_buffer(m.bands, m.stride, sizeof(In))) return hipErrorInvalidValue;
   switch (moment_blocks(m.bands)) {
 #define HSI_NB(N) \
   case N:         \
@@ -1178,6 +1204,11 @@ std::size_t moments_scratch_doubles(std::size_t pixels, int bands) {
   const std::size_t mixed = mom_wgs(pixels) * static_cast<std::size_t>(moment_pairs(bands)) * 256 + 3 * kMaxBands;
   const std::size_t plain = mom_wgs(pixels, kMaxMomWgs) * f64_partial_doubles(f64_split(bands).nbm);
   return mixed > plain ? mixed : plain;
+}
+
+bool mixed_moments_fit(int bands, std::size_t stride, std::size_t elem_bytes, bool screen) {
+  return fits_buffer(bands, stride, elem_bytes) &&
+         stage_offsets_fit(moment_blocks(bands), screen ? kFusedChunk : f16_chunk<1>(), stride, elem_bytes);
 }
 
 hipError_t launch_moments(const MomentArgs& m, double* scratch, hipStream_t stream) {

@@ -148,7 +148,9 @@ hsi_status hsi_set_library(hsi_ctx* ctx, const float* values, const float* norms
  * mean of a sparse sample of the frame; products are formed on FP16 matrix
  * cores from a hi/lo split of d with FP32 tile accumulators, tiles are
  * reduced in FP64, and the raw moments are rebuilt from the shifted ones in
- * FP64 as moments_from_shifted does (linalg.cpp:81-103). PLAIN ignores the
+ * FP64 as moments_from_shifted does (linalg.cpp:81-103); like the mixed
+ * scoring it reads the cube through one buffer view, so bands x plane_stride
+ * elements must stay under 4 GiB (HSI_ERR_INVALID_ARG otherwise). PLAIN ignores the
  * shift (its products are exact). Replaces launch_reset_moments +
  * launch_accumulate_mean + launch_accumulate_moments (stats_cuda.hpp:45-72,
  * stats.cu:174-214; their `reference` argument is d_shift_or_null) and
@@ -178,7 +180,7 @@ hsi_status hsi_stats_accumulate(hsi_ctx* ctx, const void* d_cube, hsi_dtype dtyp
  * `capacity`,
  * *d_count is set above `capacity` (the list overflowed, and
  * hsi_score_candidate_spectra then scores the whole frame). At most 127
- * bands. Asynchronous: build the statistics, then rescore the rows. */
+ * bands, in a cube under 4 GiB as for the mixed statistics. Asynchronous: build the statistics, then rescore the rows. */
 hsi_status hsi_stats_accumulate_screen(hsi_ctx* ctx, const float* d_cube, int height, int width,
                                        int bands, size_t plane_stride,
                                        const float* d_shift_or_null, float cut,

@@ -21,7 +21,9 @@ implementation, independently of the oracle and of its fixtures:
   - the device whitener W = L^-1 whitens the covariance, W S W^T = I
     (tests/test_linalg.cpp:44-127, the Cholesky reconstruction);
   - a library whose band count differs from the cube's is refused
-    (tests/test_detector.cpp, band mismatch).
+    (tests/test_detector.cpp, band mismatch);
+  - the mixed-form statistics refuse a cube past one 4 GiB buffer view
+    (no counterpart: the limit of this build's mixed form).
 RX and ACE run in the plain FP64 form, the FP32 form and the mixed (FP16
 hi/lo) form, each against its own bar: 1e-4 relative for the first two, as
 the reference's tests, and 1e-3 for the mixed form.
@@ -282,3 +284,21 @@ def test_library_band_mismatch_is_refused(H, torch, ctx):
         ctx.stats_build(1e-6)
         ctx.stats_check()
     assert "bands" in str(info.value)
+
+
+@pytest.mark.gpu
+def test_mixed_statistics_refuse_a_cube_past_one_buffer_view(H, torch, ctx):
+    """The mixed-form statistics read the cube through one buffer view: a
+    plane stride that puts the cube past 4 GiB, or only the staging offsets of
+    its padding rows past 2^32 (209 FP16 bands staged as 224 rows), is refused
+    with an argument error before anything is launched (the cube itself is a
+    few bytes here and is never read)."""
+    import ctypes as C
+    lib = H.lib()
+    small = torch.zeros(64, dtype=torch.float32, device="cuda")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for bands, stride, dtype in [(4, 1 << 30, H.F16), (209, 10_000_000, H.F16), (4, 1 << 28, H.F32)]:
+        st = lib.hsi_stats_accumulate(ctx.handle, C.c_void_p(small.data_ptr()), dtype, 2, 2, bands,
+                                      C.c_size_t(stride), H.MIXED, None, stream)
+        assert st == H.ERR_INVALID_ARG, (bands, stride, H.status_name(st))
+        assert "4 GiB" in lib.hsi_ctx_last_error(ctx.handle).decode()
