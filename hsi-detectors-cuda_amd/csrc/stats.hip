@@ -253,7 +253,7 @@ __host__ __device__ constexpr bool vec_blocks_complete() {
   return true;
 }
 
-template <int NBM, int W, int POS>
+template <int NBM, int W, int POS, bool BUF>
 __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube, std::size_t p_begin,
                                                  std::size_t p_end, int bands, std::size_t stride,
                                                  int vec_rows, double* tiles, double* __restrict__ out) {
@@ -289,29 +289,65 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
   }
   const int chunks = static_cast<int>(div_up(p_end - p_begin, CH));
 
+  // BUF: the reads go through a buffer view of the cube, as in the mixed
+  // form (thread t takes band rows t / (CH / 4) + k * RSTEP at one pixel
+  // column of every chunk: one register plus a per-row constant, no clamp;
+  // rows past the band count read zeros from past the view's end, which the
+  // launcher guarantees), and only a range's last chunk masks pixels past
+  // p_end. Otherwise every read is clamped into the cube and fixed up.
+  constexpr int RSTEP = THREADS / (CH / 4);
+  static_assert(THREADS % (CH / 4) == 0, "a thread stages one pixel column of every row it takes");
+  const __amdgpu_buffer_rsrc_t view =
+      cube_rsrc(cube, BUF ? static_cast<std::uint32_t>(static_cast<std::size_t>(bands) * stride * 4) : 0u);
+  const std::uint32_t row0 = static_cast<std::uint32_t>(
+      (static_cast<std::size_t>(threadIdx.x / (CH / 4)) * stride + (threadIdx.x % (CH / 4)) * 4) * 4);
+  const std::uint32_t kstep = static_cast<std::uint32_t>(static_cast<std::size_t>(RSTEP) * stride * 4);
   float4 pre[PER];
   auto load = [&](std::size_t p) {
+    if constexpr (BUF) {
+      const std::uint32_t at = row0 + static_cast<std::uint32_t>(p * 4);
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = threadIdx.x + THREADS * k;
-      const int b = e / (CH / 4);
-      pre[k] = stage_raw(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_begin, p_end);
+      for (int k = 0; k < PER; ++k)
+        pre[k] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                view, at + static_cast<std::uint32_t>(k) * kstep, 0, 0));
+    } else {
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + THREADS * k;
+        const int b = e / (CH / 4);
+        pre[k] = stage_raw(cube, stride, bands, b, p + (e % (CH / 4)) * 4, p_begin, p_end);
+      }
     }
   };
-  auto store = [&](double* t, std::size_t p) {
+  auto store_rows = [&](double* t, std::size_t p, auto partial) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + THREADS * k;
       if (e < LOADS) {
         const int b = e / (CH / 4);
         const int k4 = (e % (CH / 4)) * 4;
-        const float4 v = stage_fix(pre[k], bands, b, p + k4, p_end, false);
+        float4 v = pre[k];
+        if constexpr (!BUF) {
+          v = stage_fix(v, bands, b, p + k4, p_end, false);
+        } else if constexpr (decltype(partial)::value) {
+          const std::size_t px = p + k4;
+          if (px >= p_end) v.x = 0.f;
+          if (px + 1 >= p_end) v.y = 0.f;
+          if (px + 2 >= p_end) v.z = 0.f;
+          if (px + 3 >= p_end) v.w = 0.f;
+        }
         t[(k4 + 0) * LDB + b] = v.x;
         t[(k4 + 1) * LDB + b] = v.y;
         t[(k4 + 2) * LDB + b] = v.z;
         t[(k4 + 3) * LDB + b] = v.w;
       }
     }
+  };
+  auto store = [&](double* t, std::size_t p) {
+    if (!BUF || p + CH <= p_end)
+      store_rows(t, p, std::false_type{});
+    else
+      store_rows(t, p, std::true_type{});
   };
 
   load(p_begin);
@@ -387,7 +423,7 @@ __device__ __forceinline__ void moments_f64_wave(const float* __restrict__ cube,
   }
 }
 
-template <int NBM>
+template <int NBM, bool BUF>
 __global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const float* __restrict__ cube,
                                                                       std::size_t pixels, int bands,
                                                                       std::size_t stride,
@@ -404,7 +440,7 @@ __global__ __launch_bounds__(kWavesPerWg * 64) void moments_f64_kernel(const flo
   switch (threadIdx.x >> 6) {
 #define HSI_W(W)                                                                                   \
   case W:                                                                                          \
-    moments_f64_wave<NBM, W, wave_position(W)>(cube, p_begin, p_end, bands, stride, vec_rows, tiles, \
+    moments_f64_wave<NBM, W, wave_position(W), BUF>(cube, p_begin, p_end, bands, stride, vec_rows, tiles, \
                                                out);                                             \
     break;
     HSI_W(0) HSI_W(1) HSI_W(2) HSI_W(3) HSI_W(4) HSI_W(5) HSI_W(6) HSI_W(7)
@@ -1168,8 +1204,12 @@ hipError_t launch_f64_nbm(const float* cube, std::size_t pixels, int bands, std:
   const std::size_t wgs = mom_wgs(pixels, f64_max_wgs<NBM + 1>());
   const std::size_t px_per_wg = div_up(chunks, wgs) * kChunk;
   const std::size_t used = div_up(pixels, px_per_wg);
-  hipLaunchKernelGGL(moments_f64_kernel<NBM>, dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64), 0,
-                     stream, cube, pixels, bands, stride, px_per_wg, vec_rows, partial);
+  if (fits_buffer(bands, stride, sizeof(float)) && stage_offsets_fit(NBM + 1, f64_chunk<NBM + 1>(), stride, sizeof(float)))
+    hipLaunchKernelGGL((moments_f64_kernel<NBM, true>), dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64),
+                       0, stream, cube, pixels, bands, stride, px_per_wg, vec_rows, partial);
+  else
+    hipLaunchKernelGGL((moments_f64_kernel<NBM, false>), dim3(static_cast<unsigned>(used)), dim3(kWavesPerWg * 64),
+                       0, stream, cube, pixels, bands, stride, px_per_wg, vec_rows, partial);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   const std::size_t per_wg = f64_partial_doubles(NBM);
