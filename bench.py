@@ -49,8 +49,8 @@ H_, W_, B_, T_ = 1024, 1024, 113, 8
 # KiB x 2.0, WRITE KiB x 1.0); the runs and the table are recorded in
 # profiles/hbm_traffic_1024x1024x113.md.
 MEASURED_TRAFFIC_BYTES = {
-    "moments_f64_kernel": 231619.9 * 2.0 * 1024 + 29600.0 * 1024,
-    "sam_cem_fused_kernel": 232832.0 * 2.0 * 1024 + 18432.0 * 1024,
+    "moments_f64_kernel": 231624.7 * 2.0 * 1024 + 29600.4 * 1024,
+    "sam_cem_fused_kernel": 232838.4 * 2.0 * 1024 + 18432.0 * 1024,
 }
 LOADING = 1e-6
 SAM_THRESHOLD = 0.10
